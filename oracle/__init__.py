@@ -1,0 +1,287 @@
+"""The oracle — TEST INFRASTRUCTURE ONLY.
+
+Python bindings of the two CPU checkers built by oracle/Makefile:
+
+* ``restated()`` -> ``_lib/libpmoracle.so``: my plain-C restatement of the
+  reference path (pm_oracle.c) and the FP64 plane-mode restatement
+  (pm_oracle_plane.c);
+* ``reference()`` -> ``_ref/libreconref.so``: the reference's own CPU MVS code
+  (core/src/mvs/patchmatch_stereo.cpp, core/src/util/camera.cpp,
+  core/src/util/linalg.cpp) compiled in place, behind ref_shim.cpp.
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's ``cpu_baseline`` leg may
+import this package, and only as the checker / the timed CPU baseline; the
+product library never loads it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from recon_scanner_amd import _native as N
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_LIB = os.path.join(HERE, "_lib", "libpmoracle.so")
+REF_LIB = os.path.join(HERE, "_ref", "libreconref.so")
+REF_SRC = "/root/reference"
+
+_P = C.POINTER
+_cam_p = _P(N.Camera)
+_cfg_p = _P(N.Config)
+
+
+def build(with_reference: bool | None = None) -> None:
+    """Compile the restatement, and the reference where its sources exist."""
+    if with_reference is None:
+        with_reference = os.path.isdir(os.path.join(REF_SRC, "core"))
+    targets = ["oracle"] + (["ref"] if with_reference else [])
+    subprocess.run(["make", "-s", "-C", HERE] + targets, check=True)
+
+
+def _declare_common(lib, prefix):
+    f = getattr(lib, prefix + "compute_ncc")
+    f.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                  C.c_void_p, C.c_int, _P(N.NccResult)]
+    f.restype = None
+
+
+_restated = None
+_reference = None
+
+
+def restated():
+    global _restated
+    if _restated is None:
+        if not os.path.exists(ORACLE_LIB):
+            raise RuntimeError(f"{ORACLE_LIB} missing: run oracle.build()")
+        lib = C.CDLL(ORACLE_LIB)
+        _declare_common(lib, "orc_")
+        lib.orc_xorshift_stream.argtypes = [C.c_uint64, C.c_int, C.c_void_p]
+        lib.orc_sample_bilinear.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double,
+                                            _P(C.c_double)]
+        lib.orc_sample_bilinear.restype = C.c_int
+        lib.orc_evaluate.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, _P(C.c_void_p),
+                                     C.c_void_p, C.c_void_p, _cam_p, _cam_p, _cfg_p, C.c_int,
+                                     C.c_void_p, _P(N.CostResult)]
+        lib.orc_patchmatch.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, _P(C.c_void_p),
+                                       C.c_void_p, C.c_void_p, _cam_p, _cam_p, _cfg_p,
+                                       C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+        lib.orc_patchmatch.restype = C.c_uint32
+        lib.orc_refine_subpixel.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, _P(C.c_void_p),
+                                            C.c_void_p, C.c_void_p, _cam_p, _cam_p, _cfg_p,
+                                            C.c_void_p, C.c_void_p]
+        lib.orc_refine_subpixel.restype = C.c_uint32
+        lib.orc_fuse.argtypes = [C.c_int, C.c_void_p, C.c_void_p, _P(C.c_void_p), _P(C.c_void_p),
+                                 _P(C.c_void_p), _cam_p, _cfg_p, C.c_int64, C.c_void_p,
+                                 C.c_void_p, C.c_void_p, C.c_void_p]
+        lib.orc_fuse.restype = C.c_int64
+        lib.orc_select_views.argtypes = [_cam_p, C.c_int, _cam_p, _cfg_p, C.c_void_p, C.c_int]
+        lib.orc_select_views.restype = C.c_int
+        lib.orc_solve3.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        lib.orc_solve3.restype = C.c_int
+        lib.orc_pixel_to_ray_batch.argtypes = [_cam_p, C.c_int, C.c_void_p, C.c_void_p]
+        lib.orc_project_batch.argtypes = [_cam_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+        lib.orc_lowbias32.argtypes = [C.c_uint32]
+        lib.orc_lowbias32.restype = C.c_uint32
+        _declare_plane(lib)
+        _restated = lib
+    return _restated
+
+
+def _declare_plane(lib):
+    if hasattr(lib, "orc_plane_eval"):
+        lib.orc_plane_eval.restype = C.c_int
+
+
+def reference_available() -> bool:
+    return os.path.exists(REF_LIB)
+
+
+def reference():
+    global _reference
+    if _reference is None:
+        if not os.path.exists(REF_LIB):
+            raise RuntimeError(f"{REF_LIB} missing: the reference was not compiled here")
+        lib = C.CDLL(REF_LIB)
+        _declare_common(lib, "ref_")
+        lib.ref_config_default.argtypes = [_cfg_p]
+        lib.ref_pixel_to_ray.argtypes = [_cam_p, C.c_int, C.c_void_p, C.c_void_p]
+        lib.ref_project_point.argtypes = [_cam_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+        common = [C.c_void_p, C.c_int, C.c_int, C.c_int, _P(C.c_void_p), C.c_void_p, C.c_void_p,
+                  _cam_p, C.c_int, _cam_p, C.c_int, _cam_p, C.c_void_p, _cfg_p]
+        lib.ref_patchmatch.argtypes = common + [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                                                C.c_void_p]
+        lib.ref_patchmatch.restype = C.c_uint32
+        lib.ref_refine_subpixel.argtypes = common + [C.c_void_p, C.c_void_p, C.c_void_p]
+        lib.ref_refine_subpixel.restype = C.c_uint32
+        lib.ref_fuse.argtypes = [C.c_int, C.c_void_p, C.c_void_p, _P(C.c_void_p), _P(C.c_void_p),
+                                 _P(C.c_void_p), _cam_p, _cfg_p, C.c_int64, C.c_void_p,
+                                 C.c_void_p, C.c_void_p, C.c_void_p]
+        lib.ref_fuse.restype = C.c_int64
+        lib.ref_select_views.argtypes = [_cam_p, C.c_int, _cam_p, _cfg_p, C.c_void_p, C.c_int]
+        lib.ref_select_views.restype = C.c_int
+        _reference = lib
+    return _reference
+
+
+# --------------------------------------------------------------------------
+# Convenience wrappers (numpy in, numpy out)
+# --------------------------------------------------------------------------
+
+def _ptr_array(arrs):
+    return (C.c_void_p * max(len(arrs), 1))(*[a.ctypes.data for a in arrs])
+
+
+def _cams(cams):
+    return (N.Camera * max(len(cams), 1))(*cams)
+
+
+class _Src:
+    def __init__(self, srcs):
+        self.srcs = [np.ascontiguousarray(s) for s in srcs]
+        self.ptrs = _ptr_array(self.srcs)
+        self.w = np.array([s.shape[1] for s in self.srcs], np.int32)
+        self.h = np.array([s.shape[0] for s in self.srcs], np.int32)
+
+
+def compute_ncc(which, a, b, coords, patch):
+    lib = reference() if which == "ref" else restated()
+    fn = lib.ref_compute_ncc if which == "ref" else lib.orc_compute_ncc
+    a = np.ascontiguousarray(a, np.uint8)
+    b = np.ascontiguousarray(b, np.uint8)
+    coords = np.ascontiguousarray(coords, np.float64)
+    n = coords.shape[0]
+    out = (N.NccResult * max(n, 1))()
+    fn(a.ctypes.data, a.shape[1], a.shape[0], b.ctypes.data, b.shape[1], b.shape[0], n,
+       coords.ctypes.data, patch, out)
+    return out[:n]
+
+
+def ref_patchmatch(ref_img, src_imgs, ref_cam, poses, intrinsics, source_indices, cfg,
+                   seeds=None):
+    """Reference PatchMatchStereo with the full index contract (global poses)."""
+    lib = reference()
+    s = _Src(src_imgs)
+    ref_img = np.ascontiguousarray(ref_img, np.uint8)
+    h, w = ref_img.shape
+    depth = np.full((h, w), np.nan, np.float32)
+    sup = np.zeros((h, w), np.uint8)
+    conf = np.zeros((h, w), np.float32)
+    idx = np.ascontiguousarray(source_indices, np.int32)
+    pcams, icams = _cams(poses), _cams(intrinsics)
+    sd = None if seeds is None else np.ascontiguousarray(seeds, np.float64)
+    filled = lib.ref_patchmatch(ref_img.ctypes.data, w, h, len(s.srcs), s.ptrs, s.w.ctypes.data,
+                                s.h.ctypes.data, C.byref(ref_cam), len(poses), pcams,
+                                len(intrinsics), icams, idx.ctypes.data, C.byref(cfg),
+                                None if sd is None else sd.ctypes.data,
+                                0 if sd is None else sd.shape[0], depth.ctypes.data,
+                                sup.ctypes.data, conf.ctypes.data)
+    return filled, depth, sup, conf
+
+
+def ref_refine(ref_img, src_imgs, ref_cam, poses, intrinsics, source_indices, cfg, depth, sup,
+               conf):
+    lib = reference()
+    s = _Src(src_imgs)
+    ref_img = np.ascontiguousarray(ref_img, np.uint8)
+    h, w = ref_img.shape
+    depth = np.array(depth, np.float32, copy=True)
+    sup = np.array(sup, np.uint8, copy=True)
+    conf = np.array(conf, np.float32, copy=True)
+    idx = np.ascontiguousarray(source_indices, np.int32)
+    pcams, icams = _cams(poses), _cams(intrinsics)
+    n = lib.ref_refine_subpixel(ref_img.ctypes.data, w, h, len(s.srcs), s.ptrs, s.w.ctypes.data,
+                                s.h.ctypes.data, C.byref(ref_cam), len(poses), pcams,
+                                len(intrinsics), icams, idx.ctypes.data, C.byref(cfg),
+                                depth.ctypes.data, sup.ctypes.data, conf.ctypes.data)
+    return n, depth, conf
+
+
+def orc_patchmatch(ref_img, src_imgs, ref_cam, src_cams, cfg, seeds=None):
+    lib = restated()
+    s = _Src(src_imgs)
+    ref_img = np.ascontiguousarray(ref_img, np.uint8)
+    h, w = ref_img.shape
+    depth = np.full((h, w), np.nan, np.float32)
+    sup = np.zeros((h, w), np.uint8)
+    conf = np.zeros((h, w), np.float32)
+    sd = None if seeds is None else np.ascontiguousarray(seeds, np.float64)
+    filled = lib.orc_patchmatch(ref_img.ctypes.data, w, h, len(s.srcs), s.ptrs, s.w.ctypes.data,
+                                s.h.ctypes.data, C.byref(ref_cam), _cams(src_cams), C.byref(cfg),
+                                None if sd is None else sd.ctypes.data,
+                                0 if sd is None else sd.shape[0], depth.ctypes.data,
+                                sup.ctypes.data, conf.ctypes.data)
+    return filled, depth, sup, conf
+
+
+def orc_refine(ref_img, src_imgs, ref_cam, src_cams, cfg, depth, conf):
+    lib = restated()
+    s = _Src(src_imgs)
+    ref_img = np.ascontiguousarray(ref_img, np.uint8)
+    h, w = ref_img.shape
+    depth = np.array(depth, np.float32, copy=True)
+    conf = np.array(conf, np.float32, copy=True)
+    n = lib.orc_refine_subpixel(ref_img.ctypes.data, w, h, len(s.srcs), s.ptrs, s.w.ctypes.data,
+                                s.h.ctypes.data, C.byref(ref_cam), _cams(src_cams), C.byref(cfg),
+                                depth.ctypes.data, conf.ctypes.data)
+    return n, depth, conf
+
+
+def orc_evaluate(ref_img, src_imgs, ref_cam, src_cams, cfg, uvi):
+    lib = restated()
+    s = _Src(src_imgs)
+    ref_img = np.ascontiguousarray(ref_img, np.uint8)
+    uvi = np.ascontiguousarray(uvi, np.float64)
+    n = uvi.shape[0]
+    out = (N.CostResult * max(n, 1))()
+    lib.orc_evaluate(ref_img.ctypes.data, ref_img.shape[1], ref_img.shape[0], len(s.srcs), s.ptrs,
+                     s.w.ctypes.data, s.h.ctypes.data, C.byref(ref_cam), _cams(src_cams),
+                     C.byref(cfg), n, uvi.ctypes.data, out)
+    return out[:n]
+
+
+def _fuse_args(maps, cams):
+    ws = np.array([m[0].shape[1] for m in maps], np.int32)
+    hs = np.array([m[0].shape[0] for m in maps], np.int32)
+    ds = [np.ascontiguousarray(m[0], np.float32) for m in maps]
+    ss = [np.ascontiguousarray(m[1], np.uint8) for m in maps]
+    cs = [np.ascontiguousarray(m[2], np.float32) for m in maps]
+    keep = (ds, ss, cs)
+    return ws, hs, _ptr_array(ds), _ptr_array(ss), _ptr_array(cs), _cams(cams), keep
+
+
+def fuse(which, maps, cams, cfg):
+    """maps: list of (depth, support, conf).  Returns dict of numpy arrays."""
+    lib = reference() if which == "ref" else restated()
+    ws, hs, dp, sp, cp, cm, keep = _fuse_args(maps, cams)
+    fn = lib.ref_fuse if which == "ref" else lib.orc_fuse
+    n = fn(len(maps), ws.ctypes.data, hs.ctypes.data, dp, sp, cp, cm, C.byref(cfg), 0, None, None,
+           None, None)
+    pos = np.zeros((max(n, 1), 3)); nrm = np.zeros((max(n, 1), 3))
+    conf = np.zeros(max(n, 1), np.float32)
+    extra = np.zeros((max(n, 1), 3), np.float32) if which == "ref" else np.zeros(max(n, 1), np.int64)
+    fn(len(maps), ws.ctypes.data, hs.ctypes.data, dp, sp, cp, cm, C.byref(cfg), n,
+       pos.ctypes.data, nrm.ctypes.data, conf.ctypes.data, extra.ctypes.data)
+    out = dict(positions=pos[:n], normals=nrm[:n], confidences=conf[:n])
+    if which == "ref":
+        out["colors"] = extra[:n]
+    else:
+        out["keys"] = extra[:n]
+    return out
+
+
+def select_views(which, ref_cam, poses, cfg):
+    lib = reference() if which == "ref" else restated()
+    fn = lib.ref_select_views if which == "ref" else lib.orc_select_views
+    out = np.zeros(max(len(poses), 1), np.int32)
+    n = fn(C.byref(ref_cam), len(poses), _cams(poses), C.byref(cfg), out.ctypes.data, len(out))
+    return out[:n].tolist()
+
+
+def xorshift_stream(seed, n):
+    out = np.zeros(n, np.uint32)
+    restated().orc_xorshift_stream(seed, n, out.ctypes.data)
+    return out
