@@ -1,0 +1,805 @@
+// ref_mode.hip — reference mode: the reference CPU PatchMatch reproduced on
+// the GPU in FP64 (compiled with -ffp-contract=off, so every double operation
+// rounds exactly as in the reference's plain x86-64 build).
+//
+//   ComputeNcc        patchmatch_stereo.cpp:214-274  -> NccU8 / k_compute_ncc
+//   EvaluateCandidate patchmatch_stereo.cpp:344-418  -> Evaluate / k_evaluate
+//   PatchMatchStereo  patchmatch_stereo.cpp:422-764  -> k_pm_init_cost, k_pm_pass,
+//                                                      k_pm_output + host driver
+//   RefineSubpixel    patchmatch_stereo.cpp:770-979  -> k_refine
+//
+// The reference's directional passes read neighbours only from a snapshot of
+// the previous pass (lines 636-685), so every pixel of a pass is independent:
+// one GPU thread per pixel, ping-pong state buffers.  Its random numbers come
+// from sequential xorshift64* streams, one per row block and iteration
+// (lines 647-649), consumed in the pass's scan order; the host replays those
+// streams into a per-pass table (draws are data-independent), so the GPU
+// result equals the reference's for the same config.num_threads.
+#include <algorithm>
+#include <cstring>
+#include <thread>
+
+#include "mvs_common.hpp"
+
+namespace rmvs {
+
+constexpr int kMaxSrc = 32;
+constexpr int kMaxFitPoints = 31;
+
+struct RefProblem {
+  const uint8_t* ref;
+  int rw, rh;
+  int nsrc;
+  int patch;
+  const uint8_t* src[kMaxSrc];
+  int sw[kMaxSrc];
+  int sh[kMaxSrc];
+  const recon_mvs_camera* cams;  // [0] reference, [1 + k] source k
+  double min_depth, max_depth, photo_threshold;
+};
+
+// SampleBilinear (patchmatch_stereo.cpp:78-100): integer coordinates are pixel
+// centres, out-of-range samples are rejected, not clamped.
+__device__ __forceinline__ bool SampleU8(const uint8_t* __restrict__ img, int w, int h, double u,
+                                         double v, double* out) {
+  if (u < 0.0 || v < 0.0 || u > static_cast<double>(w - 1) || v > static_cast<double>(h - 1))
+    return false;
+  const int x0 = static_cast<int>(floor(u));
+  const int y0 = static_cast<int>(floor(v));
+  const int x1 = min(x0 + 1, w - 1);
+  const int y1 = min(y0 + 1, h - 1);
+  const double fx = u - x0;
+  const double fy = v - y0;
+  const double p00 = img[y0 * w + x0];
+  const double p10 = img[y0 * w + x1];
+  const double p01 = img[y1 * w + x0];
+  const double p11 = img[y1 * w + x1];
+  const double top = p00 * (1.0 - fx) + p10 * fx;
+  const double bot = p01 * (1.0 - fx) + p11 * fx;
+  *out = top * (1.0 - fy) + bot * fy;
+  return true;
+}
+
+struct Ncc {
+  bool valid;
+  bool weak;
+  int valid_samples;
+  double ncc;
+};
+
+__device__ Ncc NccU8(const uint8_t* __restrict__ a, int aw, int ah, const uint8_t* __restrict__ b,
+                     int bw, int bh, double ru, double rv, double su, double sv, int patch) {
+  Ncc r{false, false, 0, 0.0};
+  const int half = patch / 2;
+  double s_r = 0.0, s_s = 0.0, s_r2 = 0.0, s_s2 = 0.0, s_x = 0.0;
+  int valid = 0;
+  for (int dy = -half; dy <= half; ++dy) {
+    for (int dx = -half; dx <= half; ++dx) {
+      double vr = 0.0, vs = 0.0;
+      if (!SampleU8(a, aw, ah, ru + dx, rv + dy, &vr)) continue;
+      if (!SampleU8(b, bw, bh, su + dx, sv + dy, &vs)) continue;
+      s_r += vr;
+      s_s += vs;
+      s_r2 += vr * vr;
+      s_s2 += vs * vs;
+      s_x += vr * vs;
+      ++valid;
+    }
+  }
+  if (valid < patch * patch / 2) return r;
+  const double n = static_cast<double>(valid);
+  const double mr = s_r / n;
+  const double ms = s_s / n;
+  const double var_r = s_r2 / n - mr * mr;
+  const double var_s = s_s2 / n - ms * ms;
+  r.weak = var_r < 12.0;  // kDefaultWeakTextureVariance (line 38)
+  const double denom = sqrt(var_r * var_s);
+  if (denom < 1e-9) return r;
+  double ncc = (s_x / n - mr * ms) / denom;
+  ncc = fmin(fmax(ncc, -1.0), 1.0);
+  r.ncc = ncc;
+  r.valid = true;
+  r.valid_samples = valid;
+  return r;
+}
+
+struct Cost {
+  double cost, ncc_sum, weight_sum;
+  int weak, valid_views;
+};
+
+// EvaluateCandidate with the reference ray of (u, v) precomputed by the caller.
+__device__ Cost Evaluate(const RefProblem& p, double u, double v, const double* ray, double inv) {
+  Cost r{1e30, 0.0, 0.0, 0, 0};
+  if (inv <= 1e-9) return r;
+  const double depth = 1.0 / inv;
+  if (depth < p.min_depth || depth > p.max_depth) return r;
+  const double pc[3] = {ray[0] * depth, ray[1] * depth, ray[2] * depth};
+  double pw[3];
+  CamToWorld(p.cams[0], pc, pw);
+  for (int k = 0; k < p.nsrc; ++k) {
+    const recon_mvs_camera& sc = p.cams[1 + k];
+    double ps[3];
+    WorldToCam(sc, pw, ps);
+    if (ps[2] <= 1e-6) continue;
+    double su = 0.0, sv = 0.0;
+    if (!Project(sc, ps, &su, &sv)) continue;
+    const Ncc n = NccU8(p.ref, p.rw, p.rh, p.src[k], p.sw[k], p.sh[k], u, v, su, sv, p.patch);
+    if (!n.valid) continue;
+    ++r.valid_views;
+    if (n.weak) ++r.weak;
+    r.ncc_sum += 1.0 * n.ncc;
+    r.weight_sum += 1.0;
+  }
+  if (r.weight_sum <= 0.0) return r;
+  const double mean = r.ncc_sum / r.weight_sum;
+  r.cost = 1.0 - mean;
+  if (r.cost > 3.0 * p.photo_threshold) r.cost = 1e30;
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+
+__global__ void k_compute_ncc(const uint8_t* __restrict__ a, int aw, int ah,
+                              const uint8_t* __restrict__ b, int bw, int bh, int n,
+                              const double* __restrict__ coords, int patch,
+                              recon_mvs_ncc_result* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double* c = coords + 4 * i;
+  const Ncc r = NccU8(a, aw, ah, b, bw, bh, c[0], c[1], c[2], c[3], patch);
+  recon_mvs_ncc_result o;
+  o.valid = r.valid ? 1 : 0;
+  o.weak_texture = r.weak ? 1 : 0;
+  o.valid_samples = r.valid_samples;
+  o.reserved0 = 0;
+  o.ncc = r.ncc;
+  out[i] = o;
+}
+
+__global__ void k_evaluate(RefProblem p, int n, const double* __restrict__ uvi,
+                           recon_mvs_cost_result* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double u = uvi[3 * i], v = uvi[3 * i + 1], inv = uvi[3 * i + 2];
+  double ray[3];
+  PixelToRay(p.cams[0], u, v, ray);
+  const Cost c = Evaluate(p, u, v, ray, inv);
+  recon_mvs_cost_result o;
+  o.cost = c.cost;
+  o.ncc_sum = c.ncc_sum;
+  o.weight_sum = c.weight_sum;
+  o.weak_texture_count = c.weak;
+  o.valid_views = c.valid_views;
+  out[i] = o;
+}
+
+// Initial cost of every pixel (patchmatch_stereo.cpp:543-581).
+__global__ void k_pm_init_cost(RefProblem p, const double* __restrict__ inv,
+                               double* __restrict__ cost, int* __restrict__ support) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= p.rw * p.rh) return;
+  const int x = idx % p.rw, y = idx / p.rw;
+  const double u = x + 0.5, v = y + 0.5;
+  double ray[3];
+  PixelToRay(p.cams[0], u, v, ray);
+  const Cost c = Evaluate(p, u, v, ray, inv[idx]);
+  cost[idx] = c.cost;
+  support[idx] = c.valid_views;
+}
+
+// One directional pass (patchmatch_stereo.cpp:655-721) for every pixel.
+// tab holds this pass's replayed draws, SoA [K + 2][P]: K random-search draws,
+// the restart test draw, the restart value draw (0 when not drawn).
+__global__ void k_pm_pass(RefProblem p, const double* __restrict__ snap_inv,
+                          const double* __restrict__ snap_cost, const int* __restrict__ snap_sup,
+                          double* __restrict__ out_inv, double* __restrict__ out_cost,
+                          int* __restrict__ out_sup, const uint32_t* __restrict__ tab, int K,
+                          double ratio, double restart_ratio, double inv_min, double inv_max,
+                          int propagate) {
+  const int P = p.rw * p.rh;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= P) return;
+  const int x = idx % p.rw, y = idx / p.rw;
+  const double u = x + 0.5, v = y + 0.5;
+  double ray[3];
+  PixelToRay(p.cams[0], u, v, ray);
+  double best_c = snap_cost[idx];
+  double best_i = snap_inv[idx];
+  int best_s = snap_sup[idx];
+  if (propagate) {
+    const int nbx[4] = {-1, 1, 0, 0};
+    const int nby[4] = {0, 0, -1, 1};
+    for (int k = 0; k < 4; ++k) {
+      const int nx = x + nbx[k], ny = y + nby[k];
+      if (nx < 0 || ny < 0 || nx >= p.rw || ny >= p.rh) continue;
+      const int nidx = ny * p.rw + nx;
+      if (snap_cost[nidx] >= 1e29) continue;
+      const double cand = snap_inv[nidx];
+      const Cost c = Evaluate(p, u, v, ray, cand);
+      if (c.cost < best_c) {
+        best_c = c.cost;
+        best_i = cand;
+        best_s = c.valid_views;
+      }
+    }
+  }
+  double radius = best_i * ratio;
+  for (int k = 0; k < K; ++k) {
+    const double sgn = static_cast<double>(tab[static_cast<size_t>(k) * P + idx]) / 4294967296.0 * 2.0 - 1.0;
+    const double cand = best_i + sgn * radius;
+    if (cand < inv_min || cand > inv_max) {
+      radius *= 0.5;
+      continue;
+    }
+    const Cost c = Evaluate(p, u, v, ray, cand);
+    if (c.cost < best_c) {
+      best_c = c.cost;
+      best_i = cand;
+      best_s = c.valid_views;
+    }
+    radius *= 0.5;
+  }
+  const double ur = static_cast<double>(tab[static_cast<size_t>(K) * P + idx]) / 4294967296.0;
+  if (ur < restart_ratio) {
+    const double uv = static_cast<double>(tab[static_cast<size_t>(K + 1) * P + idx]) / 4294967296.0;
+    const double cand = inv_min + uv * (inv_max - inv_min);
+    const Cost c = Evaluate(p, u, v, ray, cand);
+    if (c.cost < best_c) {
+      best_c = c.cost;
+      best_i = cand;
+      best_s = c.valid_views;
+    }
+  }
+  out_inv[idx] = best_i;
+  out_cost[idx] = best_c;
+  out_sup[idx] = best_s;
+}
+
+// Output stage (patchmatch_stereo.cpp:746-763).
+__global__ void k_pm_output(int P, const double* __restrict__ inv, const double* __restrict__ cost,
+                            const int* __restrict__ sup, double min_depth, double max_depth,
+                            float* __restrict__ depth, uint8_t* __restrict__ support,
+                            float* __restrict__ conf, unsigned int* __restrict__ filled) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= P) return;
+  float d_out = __builtin_nanf("");
+  uint8_t s_out = 0;
+  float c_out = 0.0f;
+  bool ok = false;
+  const double c = cost[idx];
+  if (c < 1e29) {
+    const double d = InvToDepth(inv[idx]);
+    if (!(d < min_depth || d > max_depth)) {
+      d_out = static_cast<float>(d);
+      s_out = static_cast<uint8_t>(min(sup[idx], 255));
+      const double pc = fmin(fmax(c, 0.0), 1.0);
+      c_out = static_cast<float>(1.0 - pc);
+      ok = true;
+    }
+  }
+  depth[idx] = d_out;
+  support[idx] = s_out;
+  conf[idx] = c_out;
+  // one atomic per wavefront
+  const unsigned long long m = __ballot(ok);
+  if ((threadIdx.x & 63) == 0 && m) atomicAdd(filled, static_cast<unsigned int>(__popcll(m)));
+}
+
+__device__ int Solve3(double A[3][3], double b[3], double x[3]) {
+  double m[3][4];
+  for (int i = 0; i < 3; ++i) {
+    for (int j = 0; j < 3; ++j) m[i][j] = A[i][j];
+    m[i][3] = b[i];
+  }
+  for (int col = 0; col < 3; ++col) {
+    int pivot = col;
+    for (int r = col + 1; r < 3; ++r)
+      if (fabs(m[r][col]) > fabs(m[pivot][col])) pivot = r;
+    if (fabs(m[pivot][col]) < 1e-18) return 0;
+    if (pivot != col) {
+      for (int j = 0; j < 4; ++j) {
+        const double t = m[col][j];
+        m[col][j] = m[pivot][j];
+        m[pivot][j] = t;
+      }
+    }
+    for (int r = 0; r < 3; ++r) {
+      if (r == col) continue;
+      const double f = m[r][col] / m[col][col];
+      for (int j = col; j < 4; ++j) m[r][j] -= f * m[col][j];
+    }
+  }
+  for (int i = 0; i < 3; ++i) {
+    if (fabs(m[i][i]) < 1e-18) return 0;
+    x[i] = m[i][3] / m[i][i];
+  }
+  return 1;
+}
+
+// RefineSubpixel body for one pixel (patchmatch_stereo.cpp:807-976).
+__global__ void k_refine(RefProblem p, float* __restrict__ depth, float* __restrict__ conf,
+                         double inv_min, double inv_max, double inv_per_px, int fit_points,
+                         int iterations, unsigned int* __restrict__ refined) {
+  const int P = p.rw * p.rh;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  bool ok = false;
+  if (idx < P) {
+    const float d0 = depth[idx];
+    const int x = idx % p.rw, y = idx / p.rw;
+    const double u = x + 0.5, v = y + 0.5;
+    const double inv0 = DepthToInv(d0);
+    if (isfinite(d0) && !(inv0 < inv_min || inv0 > inv_max)) {
+      double ray[3];
+      PixelToRay(p.cams[0], u, v, ray);
+      const int half_fit = fit_points / 2;
+      const double fit_step = inv_per_px * 0.5;
+      double fit_cost[kMaxFitPoints];
+      for (int k = 0; k < fit_points; ++k) fit_cost[k] = 1e30;
+      bool fit_ok = true;
+      for (int k = 0; k < fit_points; ++k) {
+        const double d = inv0 + (k - half_fit) * fit_step;
+        if (d < inv_min || d > inv_max) {
+          fit_ok = false;
+          break;
+        }
+        fit_cost[k] = Evaluate(p, u, v, ray, d).cost;
+        if (fit_cost[k] >= 1e29) fit_ok = false;
+      }
+      double best_d = inv0;
+      double best_c = Evaluate(p, u, v, ray, inv0).cost;
+      if (fit_ok) {
+        double s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, t0 = 0, t1 = 0, t2 = 0;
+        for (int k = 0; k < fit_points; ++k) {
+          const double xx = static_cast<double>(k - half_fit);
+          const double c = fit_cost[k];
+          s0 += 1.0;
+          s1 += xx;
+          s2 += xx * xx;
+          s3 += xx * xx * xx;
+          s4 += xx * xx * xx * xx;
+          t0 += c;
+          t1 += xx * c;
+          t2 += xx * xx * c;
+        }
+        double A[3][3] = {{s4, s3, s2}, {s3, s2, s1}, {s2, s1, s0}};
+        double b[3] = {t2, t1, t0};
+        double sol[3] = {0.0, 0.0, 0.0};
+        if (Solve3(A, b, sol)) {
+          const double a = sol[0], bb = sol[1];
+          if (fabs(a) > 1e-12) {
+            const double vertex = -bb / (2.0 * a);
+            if (vertex >= -static_cast<double>(half_fit) && vertex <= static_cast<double>(half_fit)) {
+              // The reference adds the vertex in sample-index units (line 898).
+              const double cand = inv0 + vertex;
+              const double cc = Evaluate(p, u, v, ray, cand).cost;
+              if (cc < best_c) {
+                best_c = cc;
+                best_d = cand;
+              }
+            }
+          }
+        }
+      }
+      double lo = best_d - inv_per_px * 1.0;
+      double hi = best_d + inv_per_px * 1.0;
+      if (lo < inv_min) lo = inv_min;
+      if (hi > inv_max) hi = inv_max;
+      if (hi > lo) {
+        const double inv_phi = 0.6180339887498949;
+        double c1 = hi - inv_phi * (hi - lo);
+        double c2 = lo + inv_phi * (hi - lo);
+        double f1 = Evaluate(p, u, v, ray, c1).cost;
+        double f2 = Evaluate(p, u, v, ray, c2).cost;
+        for (int k = 0; k < iterations; ++k) {
+          if (f1 < f2) {
+            hi = c2;
+            c2 = c1;
+            f2 = f1;
+            c1 = hi - inv_phi * (hi - lo);
+            f1 = Evaluate(p, u, v, ray, c1).cost;
+          } else {
+            lo = c1;
+            c1 = c2;
+            f1 = f2;
+            c2 = lo + inv_phi * (hi - lo);
+            f2 = Evaluate(p, u, v, ray, c2).cost;
+          }
+        }
+        const double cm = 0.5 * (lo + hi);
+        const double fm = Evaluate(p, u, v, ray, cm).cost;
+        if (fm < best_c) {
+          best_c = fm;
+          best_d = cm;
+        }
+        if (f1 < best_c) {
+          best_c = f1;
+          best_d = c1;
+        }
+        if (f2 < best_c) {
+          best_c = f2;
+          best_d = c2;
+        }
+      }
+      if (!(best_d < inv_min || best_d > inv_max)) {
+        const double fd = InvToDepth(best_d);
+        if (!(fd < p.min_depth || fd > p.max_depth)) {
+          depth[idx] = static_cast<float>(fd);
+          const double pc = fmin(fmax(best_c, 0.0), 1.0);
+          conf[idx] = static_cast<float>(1.0 - pc);
+          ok = true;
+        }
+      }
+    }
+  }
+  const unsigned long long m = __ballot(ok);
+  if ((threadIdx.x & 63) == 0 && m) atomicAdd(refined, static_cast<unsigned int>(__popcll(m)));
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+namespace {
+
+inline uint32_t XsNext(uint64_t* s) {
+  *s ^= *s >> 12;
+  *s ^= *s << 25;
+  *s ^= *s >> 27;
+  return static_cast<uint32_t>((*s * 0x2545F4914F6CDD1DULL) >> 32);
+}
+inline uint64_t XsSeed(uint64_t seed) { return seed ? seed : 0x9E3779B97F4A7C15ULL; }
+
+int ResolveThreads(int n) {
+  if (n > 0) return n;
+  const unsigned hw = std::thread::hardware_concurrency();
+  return static_cast<int>(hw > 0 ? hw : 1);
+}
+
+// Replays the row-block xorshift streams of one pass (iteration `iter`,
+// direction `dir`) into tab (SoA [K + 2][P]); patchmatch_stereo.cpp:605-721.
+void FillPassTable(uint32_t* tab, int w, int h, int iter, int dir, int threads, int K,
+                   double restart_ratio) {
+  const size_t P = static_cast<size_t>(w) * h;
+  const bool rev_x = (dir == 1 || dir == 3);
+  const bool rev_y = (dir >= 2);
+  const int block_count = std::min(threads, std::max(1, h / 16));
+  const int rows = (h + block_count - 1) / block_count;
+  auto run = [&](int b) {
+    const int y0 = b * rows;
+    const int y1 = std::min(h, y0 + rows);
+    if (y0 >= y1) return;
+    uint64_t s = XsSeed(0x5DEECE66DULL + static_cast<uint64_t>(b) * 0x9E3779B97F4A7C15ULL +
+                        static_cast<uint64_t>(iter) * 0xC2B2AE3D27D4EB4FULL);
+    for (int yy = 0; yy < y1 - y0; ++yy) {
+      const int y = rev_y ? y1 - 1 - yy : y0 + yy;
+      for (int xx = 0; xx < w; ++xx) {
+        const int x = rev_x ? w - 1 - xx : xx;
+        const size_t idx = static_cast<size_t>(y) * w + x;
+        for (int k = 0; k < K; ++k) tab[k * P + idx] = XsNext(&s);
+        const uint32_t r = XsNext(&s);
+        tab[K * P + idx] = r;
+        tab[(K + 1) * P + idx] =
+            (static_cast<double>(r) / 4294967296.0 < restart_ratio) ? XsNext(&s) : 0u;
+      }
+    }
+  };
+  const int host_threads = static_cast<int>(std::max(1u, std::thread::hardware_concurrency()));
+  if (block_count <= 1 || host_threads <= 1) {
+    for (int b = 0; b < block_count; ++b) run(b);
+  } else {
+    std::vector<std::thread> ws;
+    const int nw = std::min(block_count, host_threads);
+    for (int t = 0; t < nw; ++t)
+      ws.emplace_back([&, t]() {
+        for (int b = t; b < block_count; b += nw) run(b);
+      });
+    for (auto& th : ws) th.join();
+  }
+}
+
+int32_t ValidateProblem(recon_mvs_hip_context* ctx, const recon_mvs_problem* pr,
+                        const recon_mvs_config* cfg) {
+  if (!ctx) return RECON_MVS_ERR_INVALID_ARGUMENT;
+  if (!pr || !cfg) return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "null problem or config");
+  if (pr->pixel_format != RECON_MVS_PIXEL_U8)
+    return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "reference mode takes uint8 images");
+  if (!pr->ref_pixels || pr->ref_w <= 0 || pr->ref_h <= 0)
+    return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "bad reference image");
+  if (pr->num_src < 0 || pr->num_src > kMaxSrc)
+    return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "num_src out of [0, 32]");
+  if (pr->num_src > 0 && (!pr->src_pixels || !pr->src_w || !pr->src_h || !pr->src_cameras))
+    return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "null source arrays");
+  for (int k = 0; k < pr->num_src; ++k)
+    if (!pr->src_pixels[k] || pr->src_w[k] <= 0 || pr->src_h[k] <= 0)
+      return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "bad source image");
+  if (cfg->patch_size < 1 || cfg->patch_size > 63)
+    return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "patch_size out of [1, 63]");
+  return RECON_MVS_OK;
+}
+
+// Uploads images and cameras of a problem; fills the device RefProblem.
+int32_t Upload(recon_mvs_hip_context* ctx, const recon_mvs_problem* pr,
+               const recon_mvs_config* cfg, RefProblem* dp) {
+  size_t total = static_cast<size_t>(pr->ref_w) * pr->ref_h;
+  std::vector<size_t> off(pr->num_src);
+  for (int k = 0; k < pr->num_src; ++k) {
+    total = (total + 255) & ~static_cast<size_t>(255);
+    off[k] = total;
+    total += static_cast<size_t>(pr->src_w[k]) * pr->src_h[k];
+  }
+  uint8_t* img = static_cast<uint8_t*>(Ensure(ctx, kSlotRefImg, total));
+  auto* cams = static_cast<recon_mvs_camera*>(
+      Ensure(ctx, kSlotCams, sizeof(recon_mvs_camera) * (1 + pr->num_src)));
+  if (!img || !cams) return Fail(ctx, RECON_MVS_ERR_OUT_OF_MEMORY, "device allocation failed");
+  RMVS_HIP(ctx, hipMemcpyAsync(img, pr->ref_pixels, static_cast<size_t>(pr->ref_w) * pr->ref_h,
+                               hipMemcpyHostToDevice, ctx->stream));
+  for (int k = 0; k < pr->num_src; ++k)
+    RMVS_HIP(ctx, hipMemcpyAsync(img + off[k], pr->src_pixels[k],
+                                 static_cast<size_t>(pr->src_w[k]) * pr->src_h[k],
+                                 hipMemcpyHostToDevice, ctx->stream));
+  std::vector<recon_mvs_camera> hc(1 + pr->num_src);
+  hc[0] = pr->ref_camera;
+  for (int k = 0; k < pr->num_src; ++k) hc[1 + k] = pr->src_cameras[k];
+  RMVS_HIP(ctx, hipMemcpyAsync(cams, hc.data(), sizeof(recon_mvs_camera) * hc.size(),
+                               hipMemcpyHostToDevice, ctx->stream));
+  // the host vector dies here: make the copy complete first
+  RMVS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  std::memset(dp, 0, sizeof(*dp));
+  dp->ref = img;
+  dp->rw = pr->ref_w;
+  dp->rh = pr->ref_h;
+  dp->nsrc = pr->num_src;
+  dp->patch = cfg->patch_size;
+  for (int k = 0; k < pr->num_src; ++k) {
+    dp->src[k] = img + off[k];
+    dp->sw[k] = pr->src_w[k];
+    dp->sh[k] = pr->src_h[k];
+  }
+  dp->cams = cams;
+  dp->min_depth = cfg->min_depth_m;
+  dp->max_depth = cfg->max_depth_m;
+  dp->photo_threshold = cfg->photo_threshold;
+  return RECON_MVS_OK;
+}
+
+inline unsigned Blocks(size_t n, unsigned t) { return static_cast<unsigned>((n + t - 1) / t); }
+
+}  // namespace
+}  // namespace rmvs
+
+using namespace rmvs;
+
+extern "C" {
+
+RECON_MVS_API int32_t recon_mvs_hip_compute_ncc(recon_mvs_hip_context* ctx, const uint8_t* ref_img,
+                                                int32_t ref_w, int32_t ref_h,
+                                                const uint8_t* src_img, int32_t src_w,
+                                                int32_t src_h, int32_t n, const double* coords,
+                                                int32_t patch_size, recon_mvs_ncc_result* out) {
+  if (!ctx) return RECON_MVS_ERR_INVALID_ARGUMENT;
+  if (!ref_img || !src_img || ref_w <= 0 || ref_h <= 0 || src_w <= 0 || src_h <= 0 || n < 0 ||
+      (n > 0 && (!coords || !out)) || patch_size < 1 || patch_size > 63)
+    return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "compute_ncc: bad arguments");
+  if (n == 0) return RECON_MVS_OK;
+  const size_t ra = static_cast<size_t>(ref_w) * ref_h, sa = static_cast<size_t>(src_w) * src_h;
+  auto* da = static_cast<uint8_t*>(Ensure(ctx, kSlotQueryImgA, ra));
+  auto* db = static_cast<uint8_t*>(Ensure(ctx, kSlotQueryImgB, sa));
+  auto* dc = static_cast<double*>(Ensure(ctx, kSlotQueryIn, sizeof(double) * 4 * n));
+  auto* dout = static_cast<recon_mvs_ncc_result*>(
+      Ensure(ctx, kSlotQueryOut, sizeof(recon_mvs_ncc_result) * n));
+  if (!da || !db || !dc || !dout) return Fail(ctx, RECON_MVS_ERR_OUT_OF_MEMORY, "alloc failed");
+  RMVS_HIP(ctx, hipMemcpyAsync(da, ref_img, ra, hipMemcpyHostToDevice, ctx->stream));
+  RMVS_HIP(ctx, hipMemcpyAsync(db, src_img, sa, hipMemcpyHostToDevice, ctx->stream));
+  RMVS_HIP(ctx, hipMemcpyAsync(dc, coords, sizeof(double) * 4 * n, hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(k_compute_ncc, dim3(Blocks(n, 256)), dim3(256), 0, ctx->stream, da, ref_w,
+                     ref_h, db, src_w, src_h, n, dc, patch_size, dout);
+  RMVS_LAUNCH_CHECK(ctx);
+  RMVS_HIP(ctx, hipMemcpyAsync(out, dout, sizeof(recon_mvs_ncc_result) * n, hipMemcpyDeviceToHost,
+                               ctx->stream));
+  RMVS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return RECON_MVS_OK;
+}
+
+RECON_MVS_API int32_t recon_mvs_hip_evaluate_candidates(recon_mvs_hip_context* ctx,
+                                                        const recon_mvs_problem* problem,
+                                                        const recon_mvs_config* config, int32_t n,
+                                                        const double* u_v_inv,
+                                                        recon_mvs_cost_result* out) {
+  int32_t st = ValidateProblem(ctx, problem, config);
+  if (st != RECON_MVS_OK) return st;
+  if (n < 0 || (n > 0 && (!u_v_inv || !out)))
+    return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "evaluate_candidates: bad query arrays");
+  if (n == 0) return RECON_MVS_OK;
+  RefProblem dp;
+  st = Upload(ctx, problem, config, &dp);
+  if (st != RECON_MVS_OK) return st;
+  auto* dq = static_cast<double*>(Ensure(ctx, kSlotQueryIn, sizeof(double) * 3 * n));
+  auto* dout = static_cast<recon_mvs_cost_result*>(
+      Ensure(ctx, kSlotQueryOut, sizeof(recon_mvs_cost_result) * n));
+  if (!dq || !dout) return Fail(ctx, RECON_MVS_ERR_OUT_OF_MEMORY, "alloc failed");
+  RMVS_HIP(ctx, hipMemcpyAsync(dq, u_v_inv, sizeof(double) * 3 * n, hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(k_evaluate, dim3(Blocks(n, 128)), dim3(128), 0, ctx->stream, dp, n, dq, dout);
+  RMVS_LAUNCH_CHECK(ctx);
+  RMVS_HIP(ctx, hipMemcpyAsync(out, dout, sizeof(recon_mvs_cost_result) * n, hipMemcpyDeviceToHost,
+                               ctx->stream));
+  RMVS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return RECON_MVS_OK;
+}
+
+RECON_MVS_API int32_t recon_mvs_hip_patchmatch_stereo(recon_mvs_hip_context* ctx,
+                                                      const recon_mvs_problem* problem,
+                                                      const recon_mvs_config* config,
+                                                      const double* seed_points, int32_t num_seeds,
+                                                      float* depth, uint8_t* support,
+                                                      float* confidence, uint32_t* filled) {
+  int32_t st = ValidateProblem(ctx, problem, config);
+  if (st != RECON_MVS_OK) return st;
+  if (!depth || !support || !confidence || !filled)
+    return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "null output");
+  if (num_seeds < 0 || (num_seeds > 0 && !seed_points))
+    return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "bad seed points");
+  if (config->random_search_count < 0 || config->random_search_count > 64)
+    return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "random_search_count out of [0, 64]");
+  *filled = 0;
+  // source_indices.empty() -> 0 before DepthMap::Allocate (lines 434-435)
+  if (problem->num_src == 0) return RECON_MVS_OK;
+  const int w = problem->ref_w, h = problem->ref_h;
+  const size_t P = static_cast<size_t>(w) * h;
+  for (size_t i = 0; i < P; ++i) {  // DepthMap::Allocate (mvs.h:95-102)
+    depth[i] = __builtin_nanf("");
+    support[i] = 0;
+    confidence[i] = 0.0f;
+  }
+  if (MeanBaseline(problem->ref_camera, problem->src_cameras, problem->num_src) < 1e-9)
+    return RECON_MVS_OK;
+  const double inv_min = DepthToInv(config->max_depth_m);
+  const double inv_max = DepthToInv(config->min_depth_m);
+  if (!(inv_max > inv_min) || inv_min <= 0.0) return RECON_MVS_OK;
+
+  RefProblem dp;
+  st = Upload(ctx, problem, config, &dp);
+  if (st != RECON_MVS_OK) return st;
+
+  // Seeding + random init on the host: one sequential stream (lines 496-539).
+  std::vector<double> inv0(P, 0.0);
+  std::vector<uint8_t> init(P, 0);
+  if (seed_points && num_seeds > 0) {
+    const int grid = 32;
+    const int gx = (w + grid - 1) / grid, gy = (h + grid - 1) / grid;
+    std::vector<std::vector<double>> cell(static_cast<size_t>(gx) * gy);
+    for (int i = 0; i < num_seeds; ++i) {
+      const double su = seed_points[3 * i], sv = seed_points[3 * i + 1], sd = seed_points[3 * i + 2];
+      if (su < 0 || sv < 0 || su >= w || sv >= h) continue;
+      if (sd <= 0.0) continue;
+      const int cx = std::min(gx - 1, static_cast<int>(su) / grid);
+      const int cy = std::min(gy - 1, static_cast<int>(sv) / grid);
+      cell[static_cast<size_t>(cy) * gx + cx].push_back(sd);
+    }
+    std::vector<double> mean(cell.size(), 0.0);
+    for (size_t c = 0; c < cell.size(); ++c) {
+      if (cell[c].empty()) continue;
+      double s = 0.0;
+      for (double d : cell[c]) s += d;
+      mean[c] = s / static_cast<double>(cell[c].size());
+    }
+    for (int y = 0; y < h; ++y)
+      for (int x = 0; x < w; ++x) {
+        const size_t c = static_cast<size_t>(std::min(gy - 1, y / grid)) * gx + std::min(gx - 1, x / grid);
+        if (cell[c].empty()) continue;
+        inv0[static_cast<size_t>(y) * w + x] = DepthToInv(mean[c]);
+        init[static_cast<size_t>(y) * w + x] = 1;
+      }
+  }
+  {
+    uint64_t s = XsSeed(0x5DEECE66DULL);
+    for (size_t i = 0; i < P; ++i) {
+      if (init[i]) continue;
+      inv0[i] = inv_min + (static_cast<double>(XsNext(&s)) / 4294967296.0) * (inv_max - inv_min);
+    }
+  }
+
+  double* invA = static_cast<double*>(Ensure(ctx, kSlotInvA, P * sizeof(double)));
+  double* invB = static_cast<double*>(Ensure(ctx, kSlotInvB, P * sizeof(double)));
+  double* costA = static_cast<double*>(Ensure(ctx, kSlotCostA, P * sizeof(double)));
+  double* costB = static_cast<double*>(Ensure(ctx, kSlotCostB, P * sizeof(double)));
+  int* supA = static_cast<int*>(Ensure(ctx, kSlotSupA, P * sizeof(int)));
+  int* supB = static_cast<int*>(Ensure(ctx, kSlotSupB, P * sizeof(int)));
+  const int K = config->random_search_count;
+  const size_t tab_bytes = static_cast<size_t>(K + 2) * P * sizeof(uint32_t);
+  uint32_t* tab[2] = {static_cast<uint32_t*>(Ensure(ctx, kSlotRng0, tab_bytes)),
+                      static_cast<uint32_t*>(Ensure(ctx, kSlotRng1, tab_bytes))};
+  float* d_depth = static_cast<float*>(Ensure(ctx, kSlotOutDepth, P * sizeof(float)));
+  uint8_t* d_sup = static_cast<uint8_t*>(Ensure(ctx, kSlotOutSupport, P));
+  float* d_conf = static_cast<float*>(Ensure(ctx, kSlotOutConf, P * sizeof(float)));
+  unsigned int* d_cnt = static_cast<unsigned int*>(Ensure(ctx, kSlotCounter, 64));
+  if (!invA || !invB || !costA || !costB || !supA || !supB || !tab[0] || !tab[1] || !d_depth ||
+      !d_sup || !d_conf || !d_cnt)
+    return Fail(ctx, RECON_MVS_ERR_OUT_OF_MEMORY, "device allocation failed");
+
+  RMVS_HIP(ctx, hipMemcpyAsync(invA, inv0.data(), P * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  const unsigned TB = 128;
+  hipLaunchKernelGGL(k_pm_init_cost, dim3(Blocks(P, TB)), dim3(TB), 0, ctx->stream, dp, invA, costA, supA);
+  RMVS_LAUNCH_CHECK(ctx);
+
+  const int threads = ResolveThreads(config->num_threads);
+  const int dirs = std::max(1, config->propagation_directions);
+  int pass = 0;
+  for (int iter = 0; iter < config->iterations; ++iter) {
+    for (int dir = 0; dir < dirs; ++dir, ++pass) {
+      const int b = pass & 1;
+      uint32_t* host = static_cast<uint32_t*>(EnsurePinned(ctx, b, tab_bytes));
+      if (!host) return Fail(ctx, RECON_MVS_ERR_OUT_OF_MEMORY, "pinned allocation failed");
+      // the previous upload from this staging buffer must have finished
+      RMVS_HIP(ctx, hipEventSynchronize(ctx->pinned_ready[b]));
+      FillPassTable(host, w, h, iter, dir, threads, K, config->restart_ratio);
+      RMVS_HIP(ctx, hipMemcpyAsync(tab[b], host, tab_bytes, hipMemcpyHostToDevice, ctx->stream));
+      RMVS_HIP(ctx, hipEventRecord(ctx->pinned_ready[b], ctx->stream));
+      hipLaunchKernelGGL(k_pm_pass, dim3(Blocks(P, TB)), dim3(TB), 0, ctx->stream, dp, invA, costA,
+                         supA, invB, costB, supB, tab[b], K, config->random_search_radius_ratio,
+                         config->restart_ratio, inv_min, inv_max, config->propagation_enabled ? 1 : 0);
+      RMVS_LAUNCH_CHECK(ctx);
+      std::swap(invA, invB);
+      std::swap(costA, costB);
+      std::swap(supA, supB);
+    }
+  }
+  RMVS_HIP(ctx, hipMemsetAsync(d_cnt, 0, sizeof(unsigned int), ctx->stream));
+  hipLaunchKernelGGL(k_pm_output, dim3(Blocks(P, 256)), dim3(256), 0, ctx->stream, static_cast<int>(P),
+                     invA, costA, supA, config->min_depth_m, config->max_depth_m, d_depth, d_sup,
+                     d_conf, d_cnt);
+  RMVS_LAUNCH_CHECK(ctx);
+  unsigned int cnt = 0;
+  RMVS_HIP(ctx, hipMemcpyAsync(depth, d_depth, P * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+  RMVS_HIP(ctx, hipMemcpyAsync(support, d_sup, P, hipMemcpyDeviceToHost, ctx->stream));
+  RMVS_HIP(ctx, hipMemcpyAsync(confidence, d_conf, P * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+  RMVS_HIP(ctx, hipMemcpyAsync(&cnt, d_cnt, sizeof(unsigned int), hipMemcpyDeviceToHost, ctx->stream));
+  RMVS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  *filled = cnt;
+  return RECON_MVS_OK;
+}
+
+RECON_MVS_API int32_t recon_mvs_hip_refine_subpixel(recon_mvs_hip_context* ctx,
+                                                    const recon_mvs_problem* problem,
+                                                    const recon_mvs_config* config, float* depth,
+                                                    float* confidence, uint32_t* refined) {
+  int32_t st = ValidateProblem(ctx, problem, config);
+  if (st != RECON_MVS_OK) return st;
+  if (!depth || !confidence || !refined) return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "null output");
+  const int fit_points = std::max(3, config->subpixel_fit_points);
+  if (fit_points > kMaxFitPoints)
+    return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "subpixel_fit_points above 31");
+  *refined = 0;
+  if (problem->num_src == 0) return RECON_MVS_OK;  // lines 780-781
+  const double baseline = MeanBaseline(problem->ref_camera, problem->src_cameras, problem->num_src);
+  if (baseline < 1e-9) return RECON_MVS_OK;
+  const double inv_min = DepthToInv(config->max_depth_m);
+  const double inv_max = DepthToInv(config->min_depth_m);
+  const double focal = 0.5 * (problem->ref_camera.fx + problem->ref_camera.fy);
+  const double inv_per_px = 1.0 / (focal * baseline);
+  RefProblem dp;
+  st = Upload(ctx, problem, config, &dp);
+  if (st != RECON_MVS_OK) return st;
+  const size_t P = static_cast<size_t>(problem->ref_w) * problem->ref_h;
+  float* d_depth = static_cast<float*>(Ensure(ctx, kSlotOutDepth, P * sizeof(float)));
+  float* d_conf = static_cast<float*>(Ensure(ctx, kSlotOutConf, P * sizeof(float)));
+  unsigned int* d_cnt = static_cast<unsigned int*>(Ensure(ctx, kSlotCounter, 64));
+  if (!d_depth || !d_conf || !d_cnt) return Fail(ctx, RECON_MVS_ERR_OUT_OF_MEMORY, "alloc failed");
+  RMVS_HIP(ctx, hipMemcpyAsync(d_depth, depth, P * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+  RMVS_HIP(ctx, hipMemcpyAsync(d_conf, confidence, P * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+  RMVS_HIP(ctx, hipMemsetAsync(d_cnt, 0, sizeof(unsigned int), ctx->stream));
+  hipLaunchKernelGGL(k_refine, dim3(Blocks(P, 128)), dim3(128), 0, ctx->stream, dp, d_depth, d_conf,
+                     inv_min, inv_max, inv_per_px, fit_points, config->subpixel_iterations, d_cnt);
+  RMVS_LAUNCH_CHECK(ctx);
+  unsigned int cnt = 0;
+  RMVS_HIP(ctx, hipMemcpyAsync(depth, d_depth, P * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+  RMVS_HIP(ctx, hipMemcpyAsync(confidence, d_conf, P * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+  RMVS_HIP(ctx, hipMemcpyAsync(&cnt, d_cnt, sizeof(unsigned int), hipMemcpyDeviceToHost, ctx->stream));
+  RMVS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  *refined = cnt;
+  return RECON_MVS_OK;
+}
+
+}  // extern "C"
