@@ -1,0 +1,199 @@
+"""Benchmark: plane-mode PatchMatch MVS on one MI355X.
+
+Workload (one step): the 9-view sphere + two tilted planes scene at 1920x1440
+(f = 2500 px, 100 mm ring baseline, synthetic, analytic depth).  Every view is
+a reference once with the other 8 as sources: random plane init + 5 red-black
+iterations, 11x11 window with stride 2 (36 samples), top-4 of 8 views, FP32
+arithmetic; then the geometric-consistency filter across the 9 depth maps.
+Inputs are resident in HBM before the timed region starts.
+
+metric: Gpix.view.iter/s = 9 maps x P x V x iterations / time / 1e9 (whole job).
+Also reported: hypothesis-view evaluations/s, median |Z - Z_true| of the
+consistency-filtered maps, the sweep kernel's roofline (HIP events on the
+library's stream) and the reference CPU path (compiled from the reference's
+own sources) timed on a bounded crop of the same workload on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+PEAK_FP32_TFLOPS = 157.3    # MI355X FP32 vector peak
+PEAK_HBM_GBS = 8000.0
+FLOP_PER_SAMPLE = 20        # warp 6 + bilinear 9 + accumulate 5 per window sample
+HYPS_PER_ITER = 8           # 4 propagated + 3 perturbed + 1 random
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1440)
+    ap.add_argument("--iterations", type=int, default=5)
+    ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16"])
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="target duration of the CPU-baseline sample (0 = skip)")
+    return ap.parse_args()
+
+
+def cpu_baseline(sc, iterations, target_s):
+    """Reference PatchMatchStereo (fronto-parallel, 11x11, all 8 views, 5
+    iterations x 4 directions) on a centred crop of view 0, all host cores."""
+    import ctypes as C
+
+    import oracle
+    from recon_scanner_amd import _native as N
+    if not oracle.reference_available():
+        return None
+    cores = len(os.sched_getaffinity(0))
+    u8 = [sc.u8(i) for i in range(sc.num_views)]
+    H, W = u8[0].shape
+
+    def run(cw, ch, iters):
+        x0, y0 = (W - cw) // 2, (H - ch) // 2
+        crop = np.ascontiguousarray(u8[0][y0:y0 + ch, x0:x0 + cw])
+        c0 = sc.cams[0]
+        ref = N.Camera.make(c0.fx, c0.fy, c0.cx - x0, c0.cy - y0, cw, ch, c0.R_np(), c0.t_np())
+        cfg = N.config_default()
+        cfg.patch_size, cfg.iterations, cfg.num_threads = 11, iters, cores
+        cfg.min_depth_m, cfg.max_depth_m = sc.min_depth, sc.max_depth
+        cfg.num_views = 8
+        poses = [ref] + list(sc.cams[1:])
+        t = time.perf_counter()
+        f, d, s, c = oracle.ref_patchmatch(crop, u8[1:], ref, poses, poses, list(range(1, 9)), cfg)
+        dt = time.perf_counter() - t
+        truth = sc.depths[0][y0:y0 + ch, x0:x0 + cw]
+        m = np.isfinite(d) & (truth > 0)
+        err = float(np.median(np.abs(d[m] - truth[m]))) if m.any() else float("nan")
+        return dt, err
+
+    # calibrate on a small crop, then size the sample to ~target_s
+    dt, _ = run(64, 48, 1)
+    per_unit = dt / (64 * 48 * 8 * 1)
+    area = max(64 * 48, min(W * H, int(target_s / (per_unit * 8 * iterations))))
+    ch = max(48, min(H, int((area * 3 / 4) ** 0.5)))
+    cw = max(64, min(W, int(ch * 4 / 3)))
+    dt, err = run(cw, ch, iterations)
+    rate = cw * ch * 8 * iterations / dt / 1e9
+    return {"value": rate, "unit": "Gpix*view*iter/s", "cores": cores, "kind": "reference",
+            "sample": f"reference PatchMatchStereo (fronto-parallel, 11x11, 8 views, {iterations} "
+                      f"iterations x 4 directions, num_threads={cores}) on a centred {cw}x{ch} "
+                      f"crop of view 0; {dt:.1f} s",
+            "median_depth_error_mm": err * 1000.0, "seconds": dt}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        sys.exit(2)
+    from recon_scanner_amd import plane, scenes
+
+    t0 = time.time()
+    f = 2500.0 * args.width / 1920.0
+    sc = scenes.sphere_planes(width=args.width, height=args.height, focal=f, n_views=9,
+                              supersample=2)
+    sys.stderr.write(f"[bench] scene rendered in {time.time() - t0:.1f} s\n")
+    nv = sc.num_views
+    views = list(range(nv))
+    p = plane.default_params(patch_size=11, window_stride=2, top_k=4, iterations=args.iterations,
+                             min_depth_m=sc.min_depth, max_depth_m=sc.max_depth, value_scale=1.0)
+    pm = plane.PlaneMVS(sc.cams, sc.images, storage=args.storage)
+
+    def step():
+        for r in views:
+            pm.patchmatch(r, [v for v in views if v != r], p)
+        pm.geo_consistency(views, 0.01)
+
+    for _ in range(args.warmup):
+        step()
+    pm.synchronize()
+    sys.stderr.write(f"[bench] warmup done at {time.time() - t0:.1f} s\n")
+    sys.stderr.flush()
+    pm.timing(enable=True, reset=True)
+    pm.synchronize()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    pm.synchronize()
+    dt = time.perf_counter() - t
+    tim = pm.timing(enable=False, reset=True)
+    sys.stderr.write(f"[bench] {args.steps} steps in {dt:.3f} s; kernels {tim}\n")
+
+    P = args.width * args.height
+    V = nv - 1
+    units = nv * P * V * args.iterations
+    value = units * args.steps / dt / 1e9
+    S = 36
+    hve = nv * P * V * (1 + HYPS_PER_ITER * args.iterations)
+    # roofline of the sweep kernel: algorithmic FLOPs per launch / mean duration
+    sweep_ms, sweep_n = tim["sweep"]
+    flop_launch = (P / 2) * HYPS_PER_ITER * V * S * FLOP_PER_SAMPLE
+    achieved = flop_launch / (sweep_ms / sweep_n * 1e-3) / 1e12 if sweep_n else None
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "sweep_traffic.txt")
+    if os.path.exists(tf):
+        for line in open(tf):
+            if line.startswith(f"traffic_bytes_per_launch {args.width}x{args.height} {args.storage}"):
+                traffic = float(line.split()[-1])
+
+    # accuracy: median |Z - Z_true| over consistency-kept pixels of all maps
+    errs = []
+    kept = 0
+    for r in views:
+        d = pm.depth_map(r, drop_inconsistent=2)["depth"]
+        m = np.isfinite(d) & (sc.depths[r] > 0)
+        kept += int(m.sum())
+        errs.append(np.abs(d[m] - sc.depths[r][m]))
+    med = float(np.median(np.concatenate(errs))) * 1000.0
+
+    cpu = None
+    if args.cpu_seconds > 0:
+        try:
+            cpu = cpu_baseline(sc, args.iterations, args.cpu_seconds)
+        except Exception as e:  # the baseline is reported, never required
+            sys.stderr.write(f"[bench] cpu baseline failed: {e}\n")
+    out = {
+        "metric": "PatchMatch Gpix*view*iter/s (+ median depth error vs analytic truth)",
+        "value": value,
+        "unit": "Gpix*view*iter/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1000.0,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32" if args.storage == "f32" else f"fp32 ({args.storage} images)",
+        "data": "synthetic (rendered sphere + 2 tilted planes, analytic depth)",
+        "config": {"workload": f"plane-mode PatchMatch, 9 views {args.width}x{args.height}, "
+                               f"each view vs 8 sources, 11x11 stride 2, top-4, "
+                               f"{args.iterations} red-black iterations + geometric-consistency "
+                               f"filter over 9 depth maps",
+                   "views": nv, "width": args.width, "height": args.height,
+                   "iterations": args.iterations, "storage": args.storage},
+        "hypothesis_view_evals_per_s": hve * args.steps / dt,
+        "median_depth_error_mm": med,
+        "kept_pixels": kept,
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+                     "unit": "TFLOP/s", "frac": (achieved / PEAK_FP32_TFLOPS) if achieved else None,
+                     "traffic": traffic, "kernel": "k_plane_sweep",
+                     "mean_launch_ms": sweep_ms / sweep_n if sweep_n else None,
+                     "flop_per_launch": flop_launch},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -193,6 +193,17 @@ RECON_MVS_API int32_t recon_mvs_hip_refine_subpixel(
     const recon_mvs_config* config, float* depth, float* confidence,
     uint32_t* refined);
 
+/*
+ * SelectSourceViews (mvs.h:133-141; patchmatch_stereo.cpp:133-211), host only:
+ * writes up to `capacity` global view indices to out and returns how many
+ * the reference would return (or -1 on a null argument).  Ties keep input
+ * order.
+ */
+RECON_MVS_API int32_t recon_mvs_select_source_views(const recon_mvs_camera* ref, int32_t n,
+                                                    const recon_mvs_camera* poses,
+                                                    const recon_mvs_config* config,
+                                                    int32_t* out, int32_t capacity);
+
 /* A depth map handed to the fusion entry point (DepthMap, mvs.h:85-109). */
 typedef struct recon_mvs_depth_map {
   int32_t width, height;
@@ -220,6 +231,125 @@ RECON_MVS_API int32_t recon_mvs_hip_fuse_depth_maps(
 RECON_MVS_API int32_t recon_mvs_hip_fetch_point_cloud(
     recon_mvs_hip_context* ctx, int64_t capacity, double* positions,
     double* normals, float* colors, float* confidences, int64_t* voxel_keys);
+
+/* ------------------------------------------------------------------------- */
+/* Plane mode: slanted-plane PatchMatch with homography warps, FP32 on the    */
+/* GPU, views resident in HBM.  It has no CPU counterpart in the reference:  */
+/* it implements the GPU design of docs/05_单目管线.md §5.3.3-§5.6 (plane    */
+/* hypotheses, red-black propagation, geometric consistency) behind the      */
+/* reference's GPU parameter block (shaders/common/gpu_algo.h:217-258).      */
+/* ------------------------------------------------------------------------- */
+
+/* One input view (image + camera) handed to recon_mvs_hip_set_views. */
+typedef struct recon_mvs_view {
+  const void* pixels;   /* host, row-major, stride = camera.width */
+  int32_t pixel_format; /* RECON_MVS_PIXEL_U8 or RECON_MVS_PIXEL_F32 */
+  int32_t reserved0;
+  recon_mvs_camera camera;
+} recon_mvs_view;
+
+/*
+ * Plane-mode parameters.  Image values are compared in the units they are
+ * given in; value_scale names that unit's full range (255 for 0..255 data,
+ * 1 for [0, 1]) and scales the reference's variance floors (weak-texture
+ * variance 12 and NCC denominator 1e-9, both in 0..255 units).
+ */
+typedef struct recon_mvs_plane_params {
+  int32_t patch_size;        /* window side, odd (the reference's patch_size) */
+  int32_t window_stride;     /* sample stride inside the window (1 or 2) */
+  int32_t top_k;             /* views aggregated per hypothesis; 0 = all */
+  int32_t iterations;        /* red-black iterations (red + black half-sweep) */
+  uint32_t seed;             /* counter-RNG seed */
+  int32_t propagation_reach; /* odd steps scanned per direction (1 = 4-neighbour) */
+  int32_t refine_iterations; /* golden-section steps of recon_mvs_hip_plane_refine */
+  int32_t reserved0;
+  double min_depth_m, max_depth_m;
+  double depth_perturb;      /* relative inverse-depth radius at iteration 0 */
+  double normal_perturb;     /* normal perturbation radius at iteration 0 */
+  double photo_threshold;    /* hypotheses with cost > 3 x this are rejected */
+  double value_scale;        /* full range of the image values (see above) */
+} recon_mvs_plane_params;
+
+/* Fills *p with the defaults: 11x11 window, stride 2, top-4, 5 iterations. */
+RECON_MVS_API void recon_mvs_plane_params_default(recon_mvs_plane_params* p);
+
+/*
+ * Uploads n views into HBM, stored as `storage_format` (F32, F16 or BF16),
+ * replacing any previous set.  Depth-map slots are per view index.
+ */
+RECON_MVS_API int32_t recon_mvs_hip_set_views(recon_mvs_hip_context* ctx, int32_t n,
+                                              const recon_mvs_view* views,
+                                              int32_t storage_format);
+
+/*
+ * Plane-mode PatchMatch of view `ref_view` against `src_views` (the
+ * PatchMatchStereo role, patchmatch_stereo.cpp:422-764): random plane init,
+ * `iterations` red-black sweeps.  Asynchronous on the context's stream; the
+ * depth map stays resident in slot ref_view.
+ */
+RECON_MVS_API int32_t recon_mvs_hip_plane_patchmatch(recon_mvs_hip_context* ctx,
+                                                     int32_t ref_view, int32_t num_src,
+                                                     const int32_t* src_views,
+                                                     const recon_mvs_plane_params* params);
+
+/*
+ * Plane-mode sub-pixel refinement of slot ref_view (the RefineSubpixel role,
+ * patchmatch_stereo.cpp:770-979): 5-point least-squares parabola in inverse
+ * depth along the pixel's ray with the plane's normal fixed, then golden
+ * section; the vertex is scaled by the sample step.  Asynchronous.
+ */
+RECON_MVS_API int32_t recon_mvs_hip_plane_refine(recon_mvs_hip_context* ctx, int32_t ref_view,
+                                                 int32_t num_src, const int32_t* src_views,
+                                                 const recon_mvs_plane_params* params);
+
+/*
+ * Geometric-consistency filter over the resident depth maps of `views`
+ * (docs/05 §5.6.2): each valid pixel of each map is projected into every
+ * other listed map; the view agrees when |z_proj - z_map| / max(z_proj,
+ * z_map) <= threshold.  geo_support = 1 + agreeing views; the pixel is kept
+ * when geo_support >= min_views.  FP64, deterministic.  Asynchronous.
+ */
+RECON_MVS_API int32_t recon_mvs_hip_geo_consistency(recon_mvs_hip_context* ctx, int32_t num_maps,
+                                                    const int32_t* views, double threshold,
+                                                    int32_t min_views);
+
+/*
+ * Copies slot `view` to the host (synchronous).  depth: NaN where invalid
+ * (or filtered out when the consistency filter ran and drop_inconsistent);
+ * normals: 3 floats per pixel; support: valid views of the best hypothesis;
+ * geo_support: see recon_mvs_hip_geo_consistency.  Any pointer may be NULL.
+ */
+RECON_MVS_API int32_t recon_mvs_hip_get_depth_map(recon_mvs_hip_context* ctx, int32_t view,
+                                                  int32_t drop_inconsistent, float* depth,
+                                                  float* normals, float* cost, uint8_t* support,
+                                                  float* confidence, uint8_t* geo_support);
+
+/* Sets slot `view`'s plane state (depth + normal per pixel) and scores it. */
+RECON_MVS_API int32_t recon_mvs_hip_set_plane_state(recon_mvs_hip_context* ctx, int32_t view,
+                                                    int32_t num_src, const int32_t* src_views,
+                                                    const recon_mvs_plane_params* params,
+                                                    const float* depth, const float* normals);
+
+/*
+ * Plane-mode cost of n hypotheses (x, y, depth, nx, ny, nz) of view ref_view
+ * (for parity tests): aggregated cost, valid views, and per-view NCC
+ * (n x num_src; NaN where the view is invalid).  Synchronous.
+ */
+RECON_MVS_API int32_t recon_mvs_hip_eval_plane_costs(recon_mvs_hip_context* ctx, int32_t ref_view,
+                                                     int32_t num_src, const int32_t* src_views,
+                                                     const recon_mvs_plane_params* params,
+                                                     int32_t n, const float* hyps, float* cost,
+                                                     int32_t* valid_views, float* view_ncc);
+
+/*
+ * Kernel timers.  enable != 0 records HIP events around every launch of the
+ * plane-mode kernels on the context's stream; ms / counts (8 entries each,
+ * may be NULL) receive the accumulated milliseconds and launch counts per
+ * kernel class: 0 sweep, 1 init, 2 refine, 3 consistency, 4 finalize.
+ * reset != 0 clears the accumulators.
+ */
+RECON_MVS_API int32_t recon_mvs_hip_timing(recon_mvs_hip_context* ctx, int32_t enable,
+                                           int32_t reset, double* ms, int64_t* counts);
 
 #ifdef __cplusplus
 }

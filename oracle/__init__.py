@@ -91,9 +91,29 @@ def restated():
     return _restated
 
 
+class PlaneParams(C.Structure):
+    """Layout of recon_mvs_plane_params (include/recon_mvs_hip.h)."""
+    _fields_ = [("patch_size", C.c_int32), ("window_stride", C.c_int32), ("top_k", C.c_int32),
+                ("iterations", C.c_int32), ("seed", C.c_uint32), ("propagation_reach", C.c_int32),
+                ("refine_iterations", C.c_int32), ("reserved0", C.c_int32),
+                ("min_depth_m", C.c_double), ("max_depth_m", C.c_double),
+                ("depth_perturb", C.c_double), ("normal_perturb", C.c_double),
+                ("photo_threshold", C.c_double), ("value_scale", C.c_double)]
+
+
 def _declare_plane(lib):
-    if hasattr(lib, "orc_plane_eval"):
-        lib.orc_plane_eval.restype = C.c_int
+    lib.orc_plane_eval.argtypes = [C.c_int, _P(C.c_void_p), _cam_p, C.c_int, C.c_int, C.c_void_p,
+                                   C.c_void_p, C.c_double, C.c_int, C.c_void_p, C.c_void_p,
+                                   C.c_void_p, C.c_void_p]
+    lib.orc_plane_eval.restype = C.c_int
+    lib.orc_plane_patchmatch.argtypes = [C.c_int, _P(C.c_void_p), _cam_p, C.c_int, C.c_int,
+                                         C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                         C.c_void_p, C.c_void_p]
+    lib.orc_plane_patchmatch.restype = C.c_int
+    lib.orc_geo_consistency.argtypes = [C.c_int, _P(C.c_void_p), _cam_p, C.c_double, _P(C.c_void_p)]
+    lib.orc_geo_consistency.restype = None
+    lib.orc_rng_uniform.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]
+    lib.orc_rng_uniform.restype = C.c_float
 
 
 def reference_available() -> bool:
@@ -285,3 +305,51 @@ def xorshift_stream(seed, n):
     out = np.zeros(n, np.uint32)
     restated().orc_xorshift_stream(seed, n, out.ctypes.data)
     return out
+
+
+# --------------------------------------------------------------------------
+# Plane mode (FP64)
+# --------------------------------------------------------------------------
+
+def _views(images, cams):
+    imgs = [np.ascontiguousarray(i, np.float32) for i in images]
+    return imgs, _ptr_array(imgs), _cams(cams)
+
+
+def plane_eval(images, cams, ref, srcs, params, hyps, sample_shift=0.5):
+    """FP64 cost of hypotheses rows (x, y, d, nx, ny, nz).  Returns (cost, nvalid, view_ncc)."""
+    lib = restated()
+    imgs, ptrs, cm = _views(images, cams)
+    srcs = np.ascontiguousarray(srcs, np.int32)
+    hyps = np.ascontiguousarray(hyps, np.float32).reshape(-1, 6)
+    n = hyps.shape[0]
+    cost = np.zeros(n); nv = np.zeros(n, np.int32); vn = np.zeros((n, len(srcs)))
+    ok = lib.orc_plane_eval(len(imgs), ptrs, cm, ref, len(srcs), srcs.ctypes.data, C.byref(params),
+                            sample_shift, n, hyps.ctypes.data, cost.ctypes.data, nv.ctypes.data,
+                            vn.ctypes.data)
+    assert ok
+    return cost, nv, vn
+
+
+def plane_patchmatch(images, cams, ref, srcs, params, init_planes=None):
+    """FP64 plane-mode PatchMatch.  Returns (planes HxWx4 float32, cost HxW float64, support)."""
+    lib = restated()
+    imgs, ptrs, cm = _views(images, cams)
+    srcs = np.ascontiguousarray(srcs, np.int32)
+    h, w = imgs[ref].shape
+    planes = np.zeros((h, w, 4), np.float32)
+    cost = np.zeros((h, w)); sup = np.zeros((h, w), np.uint8)
+    ip = None if init_planes is None else np.ascontiguousarray(init_planes, np.float32)
+    ok = lib.orc_plane_patchmatch(len(imgs), ptrs, cm, ref, len(srcs), srcs.ctypes.data,
+                                  C.byref(params), None if ip is None else ip.ctypes.data,
+                                  planes.ctypes.data, cost.ctypes.data, sup.ctypes.data)
+    assert ok
+    return planes, cost, sup
+
+
+def geo_consistency(depths, cams, thr):
+    lib = restated()
+    ds = [np.ascontiguousarray(d, np.float32) for d in depths]
+    outs = [np.zeros(d.shape, np.uint8) for d in ds]
+    lib.orc_geo_consistency(len(ds), _ptr_array(ds), _cams(cams), thr, _ptr_array(outs))
+    return outs

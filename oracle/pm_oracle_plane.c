@@ -1,8 +1,30 @@
 /*
  * pm_oracle_plane.c — TEST INFRASTRUCTURE ONLY (the oracle).
- * FP64 restatement of the plane-mode (slanted plane + homography) path.
+ *
+ * FP64 restatement of the plane-mode path of the GPU library (slanted planes,
+ * homography warps, top-k NCC, red-black sweeps, geometric consistency).
+ * The reference has no CPU code for these; they follow its design documents
+ * (docs/05_单目管线.md §5.3.3-§5.6), and the special case "fronto-parallel
+ * plane, translation-only cameras, k = all views, stride 1, reference
+ * sampling convention" reduces exactly to EvaluateCandidate
+ * (patchmatch_stereo.cpp:344-418), which tests/test_oracle_plane.py pins
+ * against the compiled reference.
+ *
+ * Costs are evaluated in FP64 with the exact homography of every sample.
+ * Candidate planes are generated with the same FP32 operations as the GPU
+ * (this file is built with -ffp-contract=off), so both sides score the same
+ * candidates; trajectories can then differ only where FP32 and FP64 costs
+ * order two candidates differently.
  */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
 #include "pm_oracle.h"
+
+#define ORC_MAXV 16
+#define ORC_COST_INF 1e30
 
 /* lowbias32 (shaders/common/gpu_algo.h:150-154; shaders/metal/gpu_common.h:239-244). */
 uint32_t orc_lowbias32(uint32_t x) {
@@ -12,4 +34,475 @@ uint32_t orc_lowbias32(uint32_t x) {
   x *= 0x846ca68bU;
   x ^= x >> 16;
   return x;
+}
+
+static uint32_t rng_key(uint32_t seed, uint32_t pass, uint32_t pixel) {
+  return orc_lowbias32(orc_lowbias32(orc_lowbias32(seed) ^ pass) ^ pixel);
+}
+
+static float rng_uniform(uint32_t key, uint32_t k) {
+  return (float)(orc_lowbias32(key ^ ((k + 1u) * 0x9E3779B9u)) >> 8) * (1.0f / 16777216.0f);
+}
+
+float orc_rng_uniform(uint32_t seed, uint32_t pass, uint32_t pixel, uint32_t k) {
+  return rng_uniform(rng_key(seed, pass, pixel), k);
+}
+
+typedef struct {
+  int w, h;
+  const float* ref;
+  double fx, fy, cx, cy;
+  float ffx, ffy, fcx, fcy;
+  int nsrc, top_k;
+  const float* src[ORC_MAXV];
+  int sw[ORC_MAXV], sh[ORC_MAXV];
+  double A[ORC_MAXV][9], b[ORC_MAXV][3];
+  float min_depth, max_depth, inv_min, inv_max;
+  double eps, photo_reject;
+  uint32_t seed;
+  int reach;
+  float depth_perturb, normal_perturb;
+  int half, stride;
+  double sample_shift; /* 0.5: pixel centres at integers (plane mode); 0: reference convention */
+} plane_args;
+
+typedef struct {
+  float d, nx, ny, nz;
+} hyp;
+
+/* H = K_s R_rel K_r^-1, b = K_s t_rel with R_rel = R_s R_r^T, t_rel = t_s - R_rel t_r. */
+static void view_pair(const recon_mvs_camera* R, const recon_mvs_camera* S, double A[9],
+                      double b[3]) {
+  double Rrel[9], trel[3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double s = 0.0;
+      for (int l = 0; l < 3; ++l) s += S->R[3 * i + l] * R->R[3 * j + l];
+      Rrel[3 * i + j] = s;
+    }
+  for (int i = 0; i < 3; ++i) {
+    double s = 0.0;
+    for (int l = 0; l < 3; ++l) s += Rrel[3 * i + l] * R->t[l];
+    trel[i] = S->t[i] - s;
+  }
+  const double Ks[9] = {S->fx, 0.0, S->cx, 0.0, S->fy, S->cy, 0.0, 0.0, 1.0};
+  const double Ki[9] = {1.0 / R->fx, 0.0, -R->cx / R->fx, 0.0, 1.0 / R->fy, -R->cy / R->fy,
+                        0.0, 0.0, 1.0};
+  double KR[9];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double s = 0.0;
+      for (int l = 0; l < 3; ++l) s += Ks[3 * i + l] * Rrel[3 * l + j];
+      KR[3 * i + j] = s;
+    }
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double s = 0.0;
+      for (int l = 0; l < 3; ++l) s += KR[3 * i + l] * Ki[3 * l + j];
+      A[3 * i + j] = s;
+    }
+  for (int i = 0; i < 3; ++i) {
+    double s = 0.0;
+    for (int l = 0; l < 3; ++l) s += Ks[3 * i + l] * trel[l];
+    b[i] = s;
+  }
+}
+
+/* Parameters: same layout as recon_mvs_plane_params in include/recon_mvs_hip.h. */
+typedef struct {
+  int32_t patch_size, window_stride, top_k, iterations;
+  uint32_t seed;
+  int32_t propagation_reach, refine_iterations, reserved0;
+  double min_depth_m, max_depth_m, depth_perturb, normal_perturb, photo_threshold, value_scale;
+} orc_plane_params;
+
+static int make_args(plane_args* a, const float* const* imgs, const recon_mvs_camera* cams,
+                     int ref, int nsrc, const int32_t* srcs, const orc_plane_params* p,
+                     double sample_shift) {
+  if (nsrc < 1 || nsrc > ORC_MAXV) return 0;
+  memset(a, 0, sizeof(*a));
+  const recon_mvs_camera* R = &cams[ref];
+  a->w = R->width;
+  a->h = R->height;
+  a->ref = imgs[ref];
+  a->fx = R->fx; a->fy = R->fy; a->cx = R->cx; a->cy = R->cy;
+  a->ffx = (float)R->fx; a->ffy = (float)R->fy; a->fcx = (float)R->cx; a->fcy = (float)R->cy;
+  a->nsrc = nsrc;
+  a->top_k = p->top_k == 0 ? nsrc : (p->top_k < nsrc ? p->top_k : nsrc);
+  for (int k = 0; k < nsrc; ++k) {
+    const recon_mvs_camera* S = &cams[srcs[k]];
+    a->src[k] = imgs[srcs[k]];
+    a->sw[k] = S->width;
+    a->sh[k] = S->height;
+    view_pair(R, S, a->A[k], a->b[k]);
+  }
+  a->min_depth = (float)p->min_depth_m;
+  a->max_depth = (float)p->max_depth_m;
+  a->inv_min = (float)(1.0 / p->max_depth_m);
+  a->inv_max = (float)(1.0 / p->min_depth_m);
+  a->eps = 1e-9 * (p->value_scale / 255.0) * (p->value_scale / 255.0);
+  a->photo_reject = 3.0 * p->photo_threshold;
+  a->seed = p->seed;
+  a->reach = p->propagation_reach;
+  a->depth_perturb = (float)p->depth_perturb;
+  a->normal_perturb = (float)p->normal_perturb;
+  a->half = p->patch_size / 2;
+  a->stride = p->window_stride;
+  a->sample_shift = sample_shift;
+  return 1;
+}
+
+/* Bilinear sample with the reference's semantics (patchmatch_stereo.cpp:78-100)
+   on float pixels. */
+static int sample_f(const float* img, int w, int h, double u, double v, double* out) {
+  if (u < 0.0 || v < 0.0 || u > (double)(w - 1) || v > (double)(h - 1)) return 0;
+  const int x0 = (int)floor(u), y0 = (int)floor(v);
+  const int x1 = x0 + 1 < w - 1 ? x0 + 1 : w - 1;
+  const int y1 = y0 + 1 < h - 1 ? y0 + 1 : h - 1;
+  const double fx = u - x0, fy = v - y0;
+  const double p00 = img[y0 * w + x0], p10 = img[y0 * w + x1];
+  const double p01 = img[y1 * w + x0], p11 = img[y1 * w + x1];
+  const double top = p00 * (1.0 - fx) + p10 * fx;
+  const double bot = p01 * (1.0 - fx) + p11 * fx;
+  *out = top * (1.0 - fy) + bot * fy;
+  return 1;
+}
+
+/* FP64 plane cost; view_ncc (may be NULL) gets NaN for invalid views. */
+static double plane_cost(const plane_args* a, int x, int y, hyp h, int* nvalid_out,
+                         double* view_ncc) {
+  *nvalid_out = 0;
+  for (int v = 0; view_ncc && v < a->nsrc; ++v) view_ncc[v] = NAN;
+  if (!(h.d >= a->min_depth && h.d <= a->max_depth)) return ORC_COST_INF;
+  const double qx = x + 0.5, qy = y + 0.5;
+  const double rx = (qx - a->cx) / a->fx, ry = (qy - a->cy) / a->fy;
+  const double nx = h.nx, ny = h.ny, nz = h.nz, D = h.d;
+  const double nd = nx * rx + ny * ry + nz;
+  if (!(nd < 0.0)) return ORC_COST_INF;
+  const double rho = D * nd;
+  const double m0 = nx / a->fx / rho;
+  const double m1 = ny / a->fy / rho;
+  const double m2 = (nz - nx * a->cx / a->fx - ny * a->cy / a->fy) / rho;
+  const int S1 = 2 * a->half / a->stride + 1;
+  const int S = S1 * S1;
+  double costs[ORC_MAXV];
+  int nvalid = 0;
+  for (int v = 0; v < a->nsrc; ++v) {
+    double H[9];
+    for (int i = 0; i < 3; ++i) {
+      H[3 * i + 0] = a->A[v][3 * i + 0] + a->b[v][i] * m0;
+      H[3 * i + 1] = a->A[v][3 * i + 1] + a->b[v][i] * m1;
+      H[3 * i + 2] = a->A[v][3 * i + 2] + a->b[v][i] * m2;
+    }
+    const double hc2 = H[6] * qx + H[7] * qy + H[8];
+    if (!(hc2 * D > 1e-6)) continue;
+    const double hc0 = H[0] * qx + H[1] * qy + H[2], hc1 = H[3] * qx + H[4] * qy + H[5];
+    if (!(fabs(hc0 / hc2 - 0.5) < 1e7 && fabs(hc1 / hc2 - 0.5) < 1e7)) continue;
+    double sr = 0, ss = 0, srr = 0, sss = 0, srs = 0;
+    int cnt = 0;
+    /* two passes: means, then centred sums */
+    double vr_[1024], vs_[1024];
+    for (int j = 0; j < S1; ++j) {
+      for (int i = 0; i < S1; ++i) {
+        const int dx = -a->half + i * a->stride, dy = -a->half + j * a->stride;
+        double rv;
+        if (a->sample_shift == 0.5) {
+          const int gx = x + dx, gy = y + dy;
+          if (gx < 0 || gy < 0 || gx >= a->w || gy >= a->h) continue;
+          rv = a->ref[gy * a->w + gx];
+        } else {
+          if (!sample_f(a->ref, a->w, a->h, x + 0.5 + dx - a->sample_shift,
+                        y + 0.5 + dy - a->sample_shift, &rv))
+            continue;
+        }
+        const double px = x + 0.5 + dx, py = y + 0.5 + dy;
+        const double h2 = H[6] * px + H[7] * py + H[8];
+        if (!(h2 > 0.0)) continue;
+        const double u = (H[0] * px + H[1] * py + H[2]) / h2 - a->sample_shift;
+        const double vv = (H[3] * px + H[4] * py + H[5]) / h2 - a->sample_shift;
+        double sv;
+        if (!sample_f(a->src[v], a->sw[v], a->sh[v], u, vv, &sv)) continue;
+        vr_[cnt] = rv;
+        vs_[cnt] = sv;
+        sr += rv;
+        ss += sv;
+        ++cnt;
+      }
+    }
+    if (cnt < S / 2 || cnt == 0) continue;
+    const double mr = sr / cnt, ms = ss / cnt;
+    for (int k = 0; k < cnt; ++k) {
+      const double a1 = vr_[k] - mr, b1 = vs_[k] - ms;
+      srr += a1 * a1;
+      sss += b1 * b1;
+      srs += a1 * b1;
+    }
+    const double var_r = srr / cnt, var_s = sss / cnt, cov = srs / cnt;
+    const double denom = sqrt(var_r * var_s);
+    if (!(denom >= a->eps) || !(denom > 0.0)) continue;
+    double ncc = cov / denom;
+    if (ncc < -1.0) ncc = -1.0;
+    if (ncc > 1.0) ncc = 1.0;
+    if (view_ncc) view_ncc[v] = ncc;
+    costs[nvalid++] = 1.0 - ncc;
+  }
+  *nvalid_out = nvalid;
+  if (nvalid == 0) return ORC_COST_INF;
+  /* mean of the top_k smallest */
+  for (int i = 1; i < nvalid; ++i) {
+    const double t = costs[i];
+    int j = i - 1;
+    while (j >= 0 && costs[j] > t) {
+      costs[j + 1] = costs[j];
+      --j;
+    }
+    costs[j + 1] = t;
+  }
+  const int kk = a->top_k < nvalid ? a->top_k : nvalid;
+  double sum = 0.0;
+  for (int k = 0; k < kk; ++k) sum += costs[k];
+  const double c = sum / kk;
+  return c > a->photo_reject ? ORC_COST_INF : c;
+}
+
+/* ---- candidate planes, FP32, same operations as the GPU ---- */
+
+static void pixel_ray(const plane_args* a, int x, int y, float r[3]) {
+  r[0] = ((float)x + 0.5f - a->fcx) / a->ffx;
+  r[1] = ((float)y + 0.5f - a->fcy) / a->ffy;
+  r[2] = 1.0f;
+}
+
+static int normalize3(float n[3]) {
+  const float l2 = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+  if (!(l2 > 1e-12f)) return 0;
+  const float l = sqrtf(l2);
+  n[0] = n[0] / l;
+  n[1] = n[1] / l;
+  n[2] = n[2] / l;
+  return 1;
+}
+
+static hyp random_plane(const plane_args* a, uint32_t key, uint32_t k0, const float ray[3]) {
+  const float u0 = rng_uniform(key, k0);
+  const float inv = a->inv_min + u0 * (a->inv_max - a->inv_min);
+  float n[3] = {2.0f * rng_uniform(key, k0 + 1) - 1.0f, 2.0f * rng_uniform(key, k0 + 2) - 1.0f,
+                2.0f * rng_uniform(key, k0 + 3) - 1.0f};
+  if (!normalize3(n)) {
+    n[0] = 0.0f;
+    n[1] = 0.0f;
+    n[2] = -1.0f;
+  }
+  const float dp = n[0] * ray[0] + n[1] * ray[1] + n[2] * ray[2];
+  if (dp > 0.0f) {
+    n[0] = -n[0];
+    n[1] = -n[1];
+    n[2] = -n[2];
+  }
+  hyp h = {1.0f / inv, n[0], n[1], n[2]};
+  return h;
+}
+
+static int transfer(const plane_args* a, const float p[4], int px, int py, const float ray[3],
+                    hyp* out) {
+  float rn[3];
+  pixel_ray(a, px, py, rn);
+  const float rho = p[0] * (p[1] * rn[0] + p[2] * rn[1] + p[3] * rn[2]);
+  const float dn = p[1] * ray[0] + p[2] * ray[1] + p[3] * ray[2];
+  if (!(dn < 0.0f)) return 0;
+  const float d = rho / dn;
+  if (!(d >= a->min_depth && d <= a->max_depth)) return 0;
+  out->d = d;
+  out->nx = p[1];
+  out->ny = p[2];
+  out->nz = p[3];
+  return 1;
+}
+
+/* Exported: cost of n hypotheses (x, y, d, nx, ny, nz). */
+int orc_plane_eval(int nviews, const float* const* imgs, const recon_mvs_camera* cams, int ref,
+                   int nsrc, const int32_t* srcs, const orc_plane_params* p, double sample_shift,
+                   int n, const float* hyps, double* cost, int32_t* nvalid, double* view_ncc) {
+  (void)nviews;
+  plane_args a;
+  if (!make_args(&a, imgs, cams, ref, nsrc, srcs, p, sample_shift)) return 0;
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int q = 0; q < n; ++q) {
+    const float* hq = hyps + 6 * q;
+    const int x = (int)hq[0], y = (int)hq[1];
+    hyp h = {hq[2], hq[3], hq[4], hq[5]};
+    int nv = 0;
+    if (x < 0 || y < 0 || x >= a.w || y >= a.h) {
+      cost[q] = ORC_COST_INF;
+      nvalid[q] = 0;
+      for (int v = 0; view_ncc && v < nsrc; ++v) view_ncc[q * nsrc + v] = NAN;
+      continue;
+    }
+    cost[q] = plane_cost(&a, x, y, h, &nv, view_ncc ? view_ncc + (size_t)q * nsrc : NULL);
+    nvalid[q] = nv;
+  }
+  return 1;
+}
+
+/* Exported: full plane-mode PatchMatch.  planes: P x 4 floats (d, nx, ny, nz);
+   cost: P doubles; support: P bytes.  If init_planes is non-NULL the random
+   init is replaced by those planes (then scored) and `iterations` sweeps run. */
+int orc_plane_patchmatch(int nviews, const float* const* imgs, const recon_mvs_camera* cams,
+                         int ref, int nsrc, const int32_t* srcs, const orc_plane_params* p,
+                         const float* init_planes, float* planes, double* cost,
+                         uint8_t* support) {
+  (void)nviews;
+  plane_args a;
+  if (!make_args(&a, imgs, cams, ref, nsrc, srcs, p, 0.5)) return 0;
+  const int W = a.w, Hh = a.h;
+  const int P = W * Hh;
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int idx = 0; idx < P; ++idx) {
+    const int x = idx % W, y = idx / W;
+    float ray[3];
+    pixel_ray(&a, x, y, ray);
+    hyp h;
+    if (init_planes) {
+      h.d = init_planes[4 * idx];
+      h.nx = init_planes[4 * idx + 1];
+      h.ny = init_planes[4 * idx + 2];
+      h.nz = init_planes[4 * idx + 3];
+    } else {
+      h = random_plane(&a, rng_key(a.seed, 0u, (uint32_t)idx), 0, ray);
+    }
+    int nv = 0;
+    cost[idx] = plane_cost(&a, x, y, h, &nv, NULL);
+    planes[4 * idx] = h.d;
+    planes[4 * idx + 1] = h.nx;
+    planes[4 * idx + 2] = h.ny;
+    planes[4 * idx + 3] = h.nz;
+    support[idx] = (uint8_t)(nv < 255 ? nv : 255);
+  }
+  for (int it = 0; it < p->iterations; ++it) {
+    for (int color = 0; color < 2; ++color) {
+      const float dscale = ldexpf(a.depth_perturb, -it);
+      const float nscale = ldexpf(a.normal_perturb, -it);
+#pragma omp parallel for schedule(dynamic, 64)
+      for (int idx = 0; idx < P; ++idx) {
+        const int x = idx % W, y = idx / W;
+        if (((x + y) & 1) != color) continue;
+        float ray[3];
+        pixel_ray(&a, x, y, ray);
+        float cur[4] = {planes[4 * idx], planes[4 * idx + 1], planes[4 * idx + 2], planes[4 * idx + 3]};
+        const float base[4] = {cur[0], cur[1], cur[2], cur[3]};
+        double ccur = cost[idx];
+        int scur = support[idx];
+        const uint32_t key = rng_key(a.seed, (uint32_t)(1 + 2 * it + color), (uint32_t)idx);
+        int sx[4], sy[4];
+        const int ddx[4] = {-1, 1, 0, 0}, ddy[4] = {0, 0, -1, 1};
+        for (int d = 0; d < 4; ++d) {
+          sx[d] = -1;
+          sy[d] = -1;
+          double best = ORC_COST_INF;
+          for (int s = 0; s < a.reach; ++s) {
+            const int dist = 2 * s + 1;
+            const int nx = x + ddx[d] * dist, ny = y + ddy[d] * dist;
+            if (nx < 0 || ny < 0 || nx >= W || ny >= Hh) break;
+            const double c = cost[ny * W + nx];
+            if (c < best) {
+              best = c;
+              sx[d] = nx;
+              sy[d] = ny;
+            }
+          }
+        }
+        for (int c = 0; c < 8; ++c) {
+          hyp h;
+          int ok = 1;
+          if (c < 4) {
+            if (sx[c] < 0) continue;
+            ok = transfer(&a, planes + 4 * (sy[c] * W + sx[c]), sx[c], sy[c], ray, &h);
+          } else if (c < 7) {
+            const uint32_t k0 = 4u * (uint32_t)(c - 4);
+            float d = base[0];
+            float n[3] = {base[1], base[2], base[3]};
+            if (c != 5) {
+              const float inv = 1.0f / d;
+              const float u = 2.0f * rng_uniform(key, k0) - 1.0f;
+              const float inv2 = inv + inv * (dscale * u);
+              ok = inv2 >= a.inv_min && inv2 <= a.inv_max;
+              d = 1.0f / inv2;
+            }
+            if (ok && c != 4) {
+              n[0] = n[0] + nscale * (2.0f * rng_uniform(key, k0 + 1) - 1.0f);
+              n[1] = n[1] + nscale * (2.0f * rng_uniform(key, k0 + 2) - 1.0f);
+              n[2] = n[2] + nscale * (2.0f * rng_uniform(key, k0 + 3) - 1.0f);
+              ok = normalize3(n) && (n[0] * ray[0] + n[1] * ray[1] + n[2] * ray[2]) < 0.0f;
+            }
+            h.d = d;
+            h.nx = n[0];
+            h.ny = n[1];
+            h.nz = n[2];
+          } else {
+            h = random_plane(&a, key, 12u, ray);
+          }
+          if (!ok) continue;
+          int nv = 0;
+          const double cc = plane_cost(&a, x, y, h, &nv, NULL);
+          if (cc < ccur) {
+            ccur = cc;
+            cur[0] = h.d;
+            cur[1] = h.nx;
+            cur[2] = h.ny;
+            cur[3] = h.nz;
+            scur = nv;
+          }
+        }
+        for (int k = 0; k < 4; ++k) planes[4 * idx + k] = cur[k];
+        cost[idx] = ccur;
+        support[idx] = (uint8_t)(scur < 255 ? scur : 255);
+      }
+    }
+  }
+  return 1;
+}
+
+/* Exported: geometric consistency (docs/05 §5.6.2), FP64, reference camera
+   arithmetic.  depth maps are float (NaN = invalid); geo_support gets
+   1 + number of other maps agreeing within `thr` (0 for invalid pixels). */
+void orc_geo_consistency(int nmaps, const float* const* depth, const recon_mvs_camera* cams,
+                         double thr, uint8_t* const* geo_support) {
+  for (int m = 0; m < nmaps; ++m) {
+    const recon_mvs_camera* me = &cams[m];
+    const int W = me->width, Hh = me->height;
+#pragma omp parallel for schedule(dynamic, 8)
+    for (int y = 0; y < Hh; ++y) {
+      for (int x = 0; x < W; ++x) {
+        const int idx = y * W + x;
+        const float d = depth[m][idx];
+        int support = 0;
+        if (isfinite(d)) {
+          double ray[3];
+          orc_pixel_to_ray(me, x + 0.5, y + 0.5, ray);
+          const double dd = (double)d;
+          const double pc[3] = {ray[0] * dd, ray[1] * dd, ray[2] * dd};
+          double pw[3];
+          orc_cam_to_world(me, pc, pw);
+          support = 1;
+          for (int o = 0; o < nmaps; ++o) {
+            if (o == m) continue;
+            const recon_mvs_camera* oc = &cams[o];
+            double ps[3];
+            orc_world_to_cam(oc, pw, ps);
+            if (ps[2] <= 1e-6) continue;
+            double u, v;
+            if (!orc_project(oc, ps, &u, &v)) continue;
+            const double fu = floor(u), fv = floor(v);
+            if (fu < 0.0 || fv < 0.0 || fu >= oc->width || fv >= oc->height) continue;
+            const float zq = depth[o][(int)fv * oc->width + (int)fu];
+            if (!isfinite(zq)) continue;
+            const double zm = (double)zq;
+            const double e = fabs(ps[2] - zm) / fmax(ps[2], zm);
+            if (e <= thr) ++support;
+          }
+        }
+        geo_support[m][idx] = (uint8_t)(support < 255 ? support : 255);
+      }
+    }
+  }
 }

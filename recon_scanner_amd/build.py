@@ -33,7 +33,7 @@ HIP_SOURCES = [
     ("context.hip", []),
     ("ref_mode.hip", ["-ffp-contract=off"]),
     ("fuse.hip", ["-ffp-contract=off"]),
-    ("plane_mode.hip", []),
+    ("plane_mode.hip", ["-ffp-contract=off"]),
     ("mvs_api.cpp", ["-ffp-contract=off"]),
 ]
 

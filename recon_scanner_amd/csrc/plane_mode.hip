@@ -1,3 +1,1280 @@
-// plane_mode.hip — placeholder, replaced by the plane-mode kernels.
+// plane_mode.hip — slanted-plane PatchMatch on MI355X (the bench path).
+//
+// Per pixel the state is a plane (depth D along the pixel's ray, unit normal n
+// in the reference camera frame) and its aggregated cost.  A hypothesis is
+// scored by warping the reference window into every source view through the
+// plane-induced homography H = K_s (R_rel + t_rel m^T) K_r^-1, m = K_r^-T n / rho
+// (rho = n . X0, X0 = D K_r^-1 q_c), computing a zero-mean NCC per view on a
+// strided window, and averaging 1 - NCC over the best top_k views.
+//
+// Precision split: H and the window-centre projection are composed in FP64 per
+// (hypothesis, view); the per-sample warp is applied in FP32 as an offset from
+// that centre (so FP32 only carries offsets of a few pixels), bilinear taps and
+// the NCC sums are FP32 (images stored as FP32, FP16 or BF16).
+//
+// Schedule: random plane init, then red-black half-sweeps: pixels with
+// (x + y) & 1 == colour only read pixels of the other colour, so each launch
+// updates half the image in place without races.  Candidates per pixel and
+// iteration: 4 propagated planes (per direction, the lowest-cost other-colour
+// pixel at odd distances 1..2*reach-1), 3 perturbed copies of the current
+// plane, 1 random plane.  Random numbers: lowbias32 hash chain keyed by
+// (seed, pass, pixel, draw) (shaders/common/gpu_algo.h:150-154).
+//
+// Candidate planes are generated with explicit FP32 operations and no FMA
+// contraction (this file is compiled with -ffp-contract=off; the hot loops
+// use fmaf() where a fused multiply-add is wanted), so the FP64 oracle can
+// regenerate exactly the same candidates.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
 #include "mvs_common.hpp"
-void DestroyPlaneState(recon_mvs_hip_context* ctx) { (void)ctx; }
+
+constexpr int kMaxViews = 16;
+constexpr int kMaxTopK = 16;
+constexpr int kTileW = 32;   // pixels per tile row (16 of each colour)
+constexpr int kTileH = 16;
+constexpr float kCostInf = 1e30f;
+
+struct ResidentView {
+  int w = 0, h = 0, pitch = 0;
+  int format = RECON_MVS_PIXEL_F32;
+  void* pixels = nullptr;   // (h + 1) rows x pitch, zero padded right and below
+  recon_mvs_camera cam{};
+};
+
+struct PlaneState {
+  int w = 0, h = 0;
+  float4* plane = nullptr;  // (depth, nx, ny, nz)
+  float* cost = nullptr;
+  uint8_t* support = nullptr;
+  uint8_t* geo_support = nullptr;
+  bool has_geo = false;
+};
+
+namespace rmvs {
+
+// ---------------------------------------------------------------------------
+// Counter RNG: lowbias32 (shaders/common/gpu_algo.h:150-154, gpu_common.h:239-244)
+// ---------------------------------------------------------------------------
+RMVS_HD uint32_t Lowbias32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352dU;
+  x ^= x >> 15;
+  x *= 0x846ca68bU;
+  x ^= x >> 16;
+  return x;
+}
+RMVS_HD uint32_t RngKey(uint32_t seed, uint32_t pass, uint32_t pixel) {
+  return Lowbias32(Lowbias32(Lowbias32(seed) ^ pass) ^ pixel);
+}
+// draw k of a key, uniform in [0, 1) with 24 random bits (exact in float)
+RMVS_HD float RngUniform(uint32_t key, uint32_t k) {
+  return static_cast<float>(Lowbias32(key ^ ((k + 1u) * 0x9E3779B9u)) >> 8) * (1.0f / 16777216.0f);
+}
+
+// ---------------------------------------------------------------------------
+// Kernel arguments
+// ---------------------------------------------------------------------------
+struct PlaneArgs {
+  int w, h, ref_pitch;
+  const void* ref_img;
+  double fx, fy, cx, cy;        // reference intrinsics
+  float ffx, ffy, fcx, fcy;     // same, FP32
+  int nsrc, top_k;
+  const void* src_img[kMaxViews];
+  int src_pitch[kMaxViews], src_w[kMaxViews], src_h[kMaxViews];
+  double A[kMaxViews][9];       // K_s R_rel K_r^-1 (row-major)
+  double b[kMaxViews][3];       // K_s t_rel
+  float4* plane;
+  float* cost;
+  uint8_t* support;
+  float min_depth, max_depth, inv_min, inv_max;
+  float eps2;                   // (NCC denominator floor)^2
+  float photo_reject;           // 3 x photo_threshold
+  uint32_t seed;
+  int color, iter, reach;
+  float depth_perturb, normal_perturb;
+  int tiles_x, tiles_y, tiles_per_xcd, num_tiles;
+};
+
+template <int FMT>
+__device__ __forceinline__ float Pix(const void* base, int i) {
+  if constexpr (FMT == RECON_MVS_PIXEL_F32) {
+    return static_cast<const float*>(base)[i];
+  } else if constexpr (FMT == RECON_MVS_PIXEL_F16) {
+    return static_cast<float>(static_cast<const _Float16*>(base)[i]);
+  } else {
+    return __uint_as_float(static_cast<uint32_t>(static_cast<const uint16_t*>(base)[i]) << 16);
+  }
+}
+
+// Reference window of one pixel: S values shifted by their mean, a bit mask of
+// the samples inside the image, and the sums over those samples.
+template <int HALF, int STRIDE>
+struct RefWindow {
+  static constexpr int NS = 2 * HALF / STRIDE + 1;
+  static constexpr int S = NS * NS;
+  static constexpr int NW = (S + 63) / 64;
+  float r[S];
+  uint64_t mask[NW];
+  __device__ __forceinline__ bool bit(int k) const { return (mask[k >> 6] >> (k & 63)) & 1ull; }
+  float sum_r, sum_r2, n;
+  float mean;
+  bool full;
+};
+
+template <int HALF, int STRIDE, typename Fetch>
+__device__ __forceinline__ void MakeRefWindow(RefWindow<HALF, STRIDE>& win, Fetch fetch) {
+  constexpr int NS = RefWindow<HALF, STRIDE>::NS;
+  float s = 0.0f, cnt = 0.0f;
+#pragma unroll
+  for (int k = 0; k < RefWindow<HALF, STRIDE>::NW; ++k) win.mask[k] = 0;
+#pragma unroll
+  for (int j = 0; j < NS; ++j) {
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+      const float v = fetch(-HALF + i * STRIDE, -HALF + j * STRIDE);  // NaN outside
+      const bool ok = (v == v);
+      win.r[j * NS + i] = ok ? v : 0.0f;
+      if (ok) win.mask[(j * NS + i) >> 6] |= (1ull << ((j * NS + i) & 63));
+      s += ok ? v : 0.0f;
+      cnt += ok ? 1.0f : 0.0f;
+    }
+  }
+  const float mean = cnt > 0.0f ? s / cnt : 0.0f;
+  float sr = 0.0f, sr2 = 0.0f;
+#pragma unroll
+  for (int k = 0; k < RefWindow<HALF, STRIDE>::S; ++k) {
+    const bool ok = win.bit(k);
+    const float v = ok ? win.r[k] - mean : 0.0f;
+    win.r[k] = v;
+    sr += v;
+    sr2 = fmaf(v, v, sr2);
+  }
+  win.sum_r = sr;
+  win.sum_r2 = sr2;
+  win.n = cnt;
+  win.mean = mean;
+  win.full = (cnt == static_cast<float>(RefWindow<HALF, STRIDE>::S));
+}
+
+struct Hyp {
+  float d, nx, ny, nz;
+};
+
+// Aggregated plane cost of hypothesis h at pixel (x, y).  Returns kCostInf
+// when no view is valid or the hypothesis is rejected.  view_ncc (optional)
+// receives NaN for invalid views.
+template <int HALF, int STRIDE, int FMT>
+__device__ float PlaneCost(const PlaneArgs& a, int x, int y, const RefWindow<HALF, STRIDE>& win,
+                           const Hyp& h, int* valid_views_out, float* view_ncc) {
+  constexpr int NS = RefWindow<HALF, STRIDE>::NS;
+  constexpr int S = RefWindow<HALF, STRIDE>::S;
+  *valid_views_out = 0;
+  if (!(h.d >= a.min_depth && h.d <= a.max_depth)) {
+    if (view_ncc)
+      for (int v = 0; v < a.nsrc; ++v) view_ncc[v] = __builtin_nanf("");
+    return kCostInf;
+  }
+  // FP64: plane -> m = K_r^-T n / rho
+  const double qx = x + 0.5, qy = y + 0.5;
+  const double rx = (qx - a.cx) / a.fx, ry = (qy - a.cy) / a.fy;
+  const double nx = h.nx, ny = h.ny, nz = h.nz, D = h.d;
+  const double nd = nx * rx + ny * ry + nz;
+  float topk[kMaxTopK];
+#pragma unroll
+  for (int k = 0; k < kMaxTopK; ++k) topk[k] = kCostInf;
+  int nvalid = 0;
+  if (nd < 0.0) {
+    const double rho = D * nd;
+    const double m0 = nx / a.fx / rho;
+    const double m1 = ny / a.fy / rho;
+    const double m2 = (nz - nx * a.cx / a.fx - ny * a.cy / a.fy) / rho;
+    for (int v = 0; v < a.nsrc; ++v) {
+      float ncc_v = __builtin_nanf("");
+      // H = A + b m^T, centre projection
+      double H[9];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        H[3 * i + 0] = fma(a.b[v][i], m0, a.A[v][3 * i + 0]);
+        H[3 * i + 1] = fma(a.b[v][i], m1, a.A[v][3 * i + 1]);
+        H[3 * i + 2] = fma(a.b[v][i], m2, a.A[v][3 * i + 2]);
+      }
+      const double hc0 = fma(H[0], qx, fma(H[1], qy, H[2]));
+      const double hc1 = fma(H[3], qx, fma(H[4], qy, H[5]));
+      const double hc2 = fma(H[6], qx, fma(H[7], qy, H[8]));
+      if (hc2 * D > 1e-6) {
+        const double ih = 1.0 / hc2;
+        const double uc = hc0 * ih, vc = hc1 * ih;       // continuous source coords
+        const double su = uc - 0.5, sv = vc - 0.5;       // sampler coords (pixel centres at integers)
+        if (fabs(su) < 1e7 && fabs(sv) < 1e7) {
+          const double fsu = floor(su), fsv = floor(sv);
+          const int iu = static_cast<int>(fsu), iv = static_cast<int>(fsv);
+          const float fu = static_cast<float>(su - fsu), fv = static_cast<float>(sv - fsv);
+          const float ucf = static_cast<float>(uc), vcf = static_cast<float>(vc);
+          const float h00 = static_cast<float>(H[0] * ih), h01 = static_cast<float>(H[1] * ih);
+          const float h10 = static_cast<float>(H[3] * ih), h11 = static_cast<float>(H[4] * ih);
+          const float h20 = static_cast<float>(H[6] * ih), h21 = static_cast<float>(H[7] * ih);
+          const int w = a.src_w[v], hgt = a.src_h[v], pitch = a.src_pitch[v];
+          const float lox = static_cast<float>(-iu), hix = static_cast<float>(w - 1 - iu);
+          const float loy = static_cast<float>(-iv), hiy = static_cast<float>(hgt - 1 - iv);
+          const void* img = a.src_img[v];
+          // The window is inside the image when its four corners are (the
+          // homography keeps the convex window convex while the denominator
+          // stays positive on it).
+          bool fast = win.full;
+          if (fast) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+              const float dx = (c & 1) ? HALF : -HALF, dy = (c & 2) ? HALF : -HALF;
+              const float dh0 = fmaf(h00, dx, h01 * dy), dh1 = fmaf(h10, dx, h11 * dy);
+              const float dh2 = fmaf(h20, dx, h21 * dy);
+              const float den = 1.0f + dh2;
+              const float r = __builtin_amdgcn_rcpf(den);
+              const float tx = fmaf(fmaf(-ucf, dh2, dh0), r, fu);
+              const float ty = fmaf(fmaf(-vcf, dh2, dh1), r, fv);
+              // 1e-3 px margin: rounding of interior samples stays inside
+              fast = fast && den > 0.0f && tx >= lox + 1e-3f && tx <= hix - 1e-3f &&
+                     ty >= loy + 1e-3f && ty <= hiy - 1e-3f;
+            }
+          }
+          float ss = 0.0f, sss = 0.0f, srs = 0.0f, sr = win.sum_r, srr = win.sum_r2, cnt = win.n;
+          const float mu = win.mean;
+          if (fast) {
+#pragma unroll
+            for (int j = 0; j < NS; ++j) {
+              const float dy = static_cast<float>(-HALF + j * STRIDE);
+              const float b0 = h01 * dy, b1 = h11 * dy, b2 = h21 * dy;
+#pragma unroll
+              for (int i = 0; i < NS; ++i) {
+                const float dx = static_cast<float>(-HALF + i * STRIDE);
+                const float dh0 = fmaf(h00, dx, b0), dh1 = fmaf(h10, dx, b1);
+                const float dh2 = fmaf(h20, dx, b2);
+                const float r = __builtin_amdgcn_rcpf(1.0f + dh2);
+                const float tx = fmaf(fmaf(-ucf, dh2, dh0), r, fu);
+                const float ty = fmaf(fmaf(-vcf, dh2, dh1), r, fv);
+                const float flx = floorf(tx), fly = floorf(ty);
+                const float fx = tx - flx, fy = ty - fly;
+                const int idx = (iv + static_cast<int>(fly)) * pitch + iu + static_cast<int>(flx);
+                const float p00 = Pix<FMT>(img, idx), p10 = Pix<FMT>(img, idx + 1);
+                const float p01 = Pix<FMT>(img, idx + pitch), p11 = Pix<FMT>(img, idx + pitch + 1);
+                const float top = fmaf(fx, p10 - p00, p00);
+                const float bot = fmaf(fx, p11 - p01, p01);
+                const float s = fmaf(fy, bot - top, top) - mu;
+                const float rr = win.r[j * NS + i];
+                ss += s;
+                sss = fmaf(s, s, sss);
+                srs = fmaf(rr, s, srs);
+              }
+            }
+          } else {
+            sr = 0.0f;
+            srr = 0.0f;
+            cnt = 0.0f;
+#pragma unroll
+            for (int j = 0; j < NS; ++j) {
+              const float dy = static_cast<float>(-HALF + j * STRIDE);
+              const float b0 = h01 * dy, b1 = h11 * dy, b2 = h21 * dy;
+#pragma unroll
+              for (int i = 0; i < NS; ++i) {
+                const float dx = static_cast<float>(-HALF + i * STRIDE);
+                const float dh0 = fmaf(h00, dx, b0), dh1 = fmaf(h10, dx, b1);
+                const float dh2 = fmaf(h20, dx, b2);
+                const float den = 1.0f + dh2;
+                const float r = __builtin_amdgcn_rcpf(den);
+                const float tx = fmaf(fmaf(-ucf, dh2, dh0), r, fu);
+                const float ty = fmaf(fmaf(-vcf, dh2, dh1), r, fv);
+                const bool ok = win.bit(j * NS + i) && den > 0.0f && tx >= lox &&
+                                tx <= hix && ty >= loy && ty <= hiy;
+                const float flx = floorf(tx), fly = floorf(ty);
+                const float fx = tx - flx, fy = ty - fly;
+                const int idx =
+                    ok ? (iv + static_cast<int>(fly)) * pitch + iu + static_cast<int>(flx) : 0;
+                const float p00 = Pix<FMT>(img, idx), p10 = Pix<FMT>(img, idx + 1);
+                const float p01 = Pix<FMT>(img, idx + pitch), p11 = Pix<FMT>(img, idx + pitch + 1);
+                const float top = fmaf(fx, p10 - p00, p00);
+                const float bot = fmaf(fx, p11 - p01, p01);
+                const float s = ok ? fmaf(fy, bot - top, top) - mu : 0.0f;
+                const float rr = ok ? win.r[j * NS + i] : 0.0f;
+                ss += s;
+                sss = fmaf(s, s, sss);
+                srs = fmaf(rr, s, srs);
+                sr += rr;
+                srr = fmaf(rr, rr, srr);
+                cnt += ok ? 1.0f : 0.0f;
+              }
+            }
+          }
+          if (cnt >= static_cast<float>(S / 2) && cnt > 0.0f) {
+            const float in = 1.0f / cnt;
+            const float mr = sr * in, ms = ss * in;
+            const float vr = fmaf(srr, in, -mr * mr);
+            const float vs = fmaf(sss, in, -ms * ms);
+            const float cv = fmaf(srs, in, -mr * ms);
+            const float d2 = vr * vs;
+            if (d2 >= a.eps2 && d2 > 0.0f) {
+              float ncc = cv * __builtin_amdgcn_rsqf(d2);
+              ncc = fminf(fmaxf(ncc, -1.0f), 1.0f);
+              ncc_v = ncc;
+              float c = 1.0f - ncc;
+              ++nvalid;
+              // insert into the sorted k smallest
+#pragma unroll
+              for (int k = 0; k < kMaxTopK; ++k) {
+                const float lo = fminf(topk[k], c);
+                c = fmaxf(topk[k], c);
+                topk[k] = lo;
+              }
+            }
+          }
+        }
+      }
+      if (view_ncc) view_ncc[v] = ncc_v;
+    }
+  } else if (view_ncc) {
+    for (int v = 0; v < a.nsrc; ++v) view_ncc[v] = __builtin_nanf("");
+  }
+  *valid_views_out = nvalid;
+  if (nvalid == 0) return kCostInf;
+  const int kk = min(a.top_k, nvalid);
+  float sum = 0.0f;
+#pragma unroll
+  for (int k = 0; k < kMaxTopK; ++k) sum += (k < kk) ? topk[k] : 0.0f;
+  const float cost = sum / static_cast<float>(kk);
+  return cost > a.photo_reject ? kCostInf : cost;
+}
+
+// ---------------------------------------------------------------------------
+// Candidate planes (FP32, no contraction)
+// ---------------------------------------------------------------------------
+
+// Ray of pixel (x, y) through its centre, z = 1.
+__device__ __forceinline__ void PixelRay(const PlaneArgs& a, int x, int y, float* r) {
+  r[0] = (static_cast<float>(x) + 0.5f - a.fcx) / a.ffx;
+  r[1] = (static_cast<float>(y) + 0.5f - a.fcy) / a.ffy;
+  r[2] = 1.0f;
+}
+
+__device__ __forceinline__ bool Normalize3(float* n) {
+  const float l2 = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+  if (!(l2 > 1e-12f)) return false;
+  const float l = sqrtf(l2);
+  n[0] = n[0] / l;
+  n[1] = n[1] / l;
+  n[2] = n[2] / l;
+  return true;
+}
+
+// Random plane: inverse depth uniform in [inv_min, inv_max]; normal from a
+// uniform cube sample, normalised and turned towards the camera.
+__device__ Hyp RandomPlane(const PlaneArgs& a, uint32_t key, uint32_t k0, const float* ray) {
+  const float u0 = RngUniform(key, k0);
+  const float inv = a.inv_min + u0 * (a.inv_max - a.inv_min);
+  float n[3] = {2.0f * RngUniform(key, k0 + 1) - 1.0f, 2.0f * RngUniform(key, k0 + 2) - 1.0f,
+                2.0f * RngUniform(key, k0 + 3) - 1.0f};
+  if (!Normalize3(n)) {
+    n[0] = 0.0f;
+    n[1] = 0.0f;
+    n[2] = -1.0f;
+  }
+  const float dp = n[0] * ray[0] + n[1] * ray[1] + n[2] * ray[2];
+  if (dp > 0.0f) {
+    n[0] = -n[0];
+    n[1] = -n[1];
+    n[2] = -n[2];
+  }
+  return Hyp{1.0f / inv, n[0], n[1], n[2]};
+}
+
+}  // namespace rmvs
+
+using namespace rmvs;
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+
+// Tile index -> (tile x, tile y), XCD-aware: workgroups are dealt round-robin
+// to the 8 XCDs, so consecutive spatial tiles are placed on one XCD (shared
+// L2 for their overlapping source footprints).
+__device__ __forceinline__ bool TileOf(const PlaneArgs& a, int* tx, int* ty) {
+  const int b = blockIdx.x;
+  const int t = (b & 7) * a.tiles_per_xcd + (b >> 3);
+  if (t >= a.num_tiles) return false;
+  *tx = t % a.tiles_x;
+  *ty = t / a.tiles_x;
+  return true;
+}
+
+// Loads the reference tile (+ halo) into LDS; NaN outside the image.
+template <int HALF, int FMT>
+__device__ __forceinline__ void LoadRefTile(const PlaneArgs& a, int x0, int y0,
+                                            float (*tile)[kTileW + 2 * HALF + 1]) {
+  constexpr int TW = kTileW + 2 * HALF, TH = kTileH + 2 * HALF;
+  for (int i = threadIdx.x; i < TW * TH; i += blockDim.x) {
+    const int ly = i / TW, lx = i % TW;
+    const int gx = x0 - HALF + lx, gy = y0 - HALF + ly;
+    float v = __builtin_nanf("");
+    if (gx >= 0 && gy >= 0 && gx < a.w && gy < a.h) v = Pix<FMT>(a.ref_img, gy * a.ref_pitch + gx);
+    tile[ly][lx] = v;
+  }
+}
+
+// Random init + initial cost.  Every pixel (both colours): 64 x 4 px per
+// workgroup of 256 threads in kTileW x kTileH tiles, two pixels per thread.
+template <int HALF, int STRIDE, int FMT>
+__global__ __launch_bounds__(256) void k_plane_init(PlaneArgs a) {
+  __shared__ float tile[kTileH + 2 * HALF][kTileW + 2 * HALF + 1];
+  int tx, ty;
+  const bool live = TileOf(a, &tx, &ty);
+  if (!live) return;
+  const int x0 = tx * kTileW, y0 = ty * kTileH;
+  LoadRefTile<HALF, FMT>(a, x0, y0, tile);
+  __syncthreads();
+  for (int half = 0; half < 2; ++half) {
+    const int r = threadIdx.x / 16, j = threadIdx.x % 16;
+    const int y = y0 + r;
+    const int x = x0 + 2 * j + (((x0 + y) & 1) ^ half);
+    if (x >= a.w || y >= a.h) continue;
+    RefWindow<HALF, STRIDE> win;
+    MakeRefWindow<HALF, STRIDE>(win, [&](int dx, int dy) {
+      return tile[r + HALF + dy][x - x0 + HALF + dx];
+    });
+    float ray[3];
+    PixelRay(a, x, y, ray);
+    const int idx = y * a.w + x;
+    const uint32_t key = RngKey(a.seed, 0u, static_cast<uint32_t>(idx));
+    const Hyp h = RandomPlane(a, key, 0, ray);
+    int nv = 0;
+    const float c = PlaneCost<HALF, STRIDE, FMT>(a, x, y, win, h, &nv, nullptr);
+    a.plane[idx] = make_float4(h.d, h.nx, h.ny, h.nz);
+    a.cost[idx] = c;
+    a.support[idx] = static_cast<uint8_t>(min(nv, 255));
+  }
+}
+
+// Re-expresses the plane of pixel (px, py) at the ray of (x, y).
+__device__ __forceinline__ bool TransferPlane(const PlaneArgs& a, float4 p, int px, int py,
+                                              const float* ray, Hyp* out) {
+  float rn[3];
+  PixelRay(a, px, py, rn);
+  const float rho = p.x * (p.y * rn[0] + p.z * rn[1] + p.w * rn[2]);
+  const float dn = p.y * ray[0] + p.z * ray[1] + p.w * ray[2];
+  if (!(dn < 0.0f)) return false;
+  const float d = rho / dn;
+  if (!(d >= a.min_depth && d <= a.max_depth)) return false;
+  *out = Hyp{d, p.y, p.z, p.w};
+  return true;
+}
+
+// One red-black half-sweep over the pixels of colour a.color.
+template <int HALF, int STRIDE, int FMT>
+__global__ __launch_bounds__(256) void k_plane_sweep(PlaneArgs a) {
+  __shared__ float tile[kTileH + 2 * HALF][kTileW + 2 * HALF + 1];
+  int tx, ty;
+  if (!TileOf(a, &tx, &ty)) return;
+  const int x0 = tx * kTileW, y0 = ty * kTileH;
+  LoadRefTile<HALF, FMT>(a, x0, y0, tile);
+  __syncthreads();
+  const int r = threadIdx.x / 16, j = threadIdx.x % 16;
+  const int y = y0 + r;
+  const int x = x0 + 2 * j + (((x0 + y) & 1) ^ a.color);
+  if (x >= a.w || y >= a.h) return;
+  RefWindow<HALF, STRIDE> win;
+  MakeRefWindow<HALF, STRIDE>(win, [&](int dx, int dy) {
+    return tile[r + HALF + dy][x - x0 + HALF + dx];
+  });
+  const int idx = y * a.w + x;
+  float ray[3];
+  PixelRay(a, x, y, ray);
+  float4 cur = a.plane[idx];
+  float ccur = a.cost[idx];
+  int scur = a.support[idx];
+  const uint32_t key = RngKey(a.seed, static_cast<uint32_t>(1 + 2 * a.iter + a.color),
+                              static_cast<uint32_t>(idx));
+  const float dscale = ldexpf(a.depth_perturb, -a.iter);
+  const float nscale = ldexpf(a.normal_perturb, -a.iter);
+  const float4 base = cur;
+  // propagation sources: best-cost other-colour pixel per direction
+  int src_x[4], src_y[4];
+  {
+    const int ddx[4] = {-1, 1, 0, 0};
+    const int ddy[4] = {0, 0, -1, 1};
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+      src_x[d] = -1;
+      src_y[d] = -1;
+      float best = kCostInf;
+      for (int s = 0; s < a.reach; ++s) {
+        const int dist = 2 * s + 1;
+        const int nx = x + ddx[d] * dist, ny = y + ddy[d] * dist;
+        if (nx < 0 || ny < 0 || nx >= a.w || ny >= a.h) break;
+        const float c = a.cost[ny * a.w + nx];
+        if (c < best) {
+          best = c;
+          src_x[d] = nx;
+          src_y[d] = ny;
+        }
+      }
+    }
+  }
+  for (int c = 0; c < 8; ++c) {
+    Hyp h;
+    bool ok = true;
+    if (c < 4) {
+      if (src_x[c] < 0) continue;
+      ok = TransferPlane(a, a.plane[src_y[c] * a.w + src_x[c]], src_x[c], src_y[c], ray, &h);
+    } else if (c < 7) {
+      // 4: depth, 5: normal, 6: both, perturbed around the plane the pixel
+      // had at the start of this sweep
+      const uint32_t k0 = 4u * static_cast<uint32_t>(c - 4);
+      float d = base.x;
+      float n[3] = {base.y, base.z, base.w};
+      if (c != 5) {
+        const float inv = 1.0f / d;
+        const float u = 2.0f * RngUniform(key, k0) - 1.0f;
+        const float inv2 = inv + inv * (dscale * u);
+        ok = inv2 >= a.inv_min && inv2 <= a.inv_max;
+        d = 1.0f / inv2;
+      }
+      if (ok && c != 4) {
+        n[0] = n[0] + nscale * (2.0f * RngUniform(key, k0 + 1) - 1.0f);
+        n[1] = n[1] + nscale * (2.0f * RngUniform(key, k0 + 2) - 1.0f);
+        n[2] = n[2] + nscale * (2.0f * RngUniform(key, k0 + 3) - 1.0f);
+        ok = Normalize3(n) && (n[0] * ray[0] + n[1] * ray[1] + n[2] * ray[2]) < 0.0f;
+      }
+      h = Hyp{d, n[0], n[1], n[2]};
+    } else {
+      h = RandomPlane(a, key, 12u, ray);
+    }
+    if (!ok) continue;
+    int nv = 0;
+    const float cc = PlaneCost<HALF, STRIDE, FMT>(a, x, y, win, h, &nv, nullptr);
+    if (cc < ccur) {
+      ccur = cc;
+      cur = make_float4(h.d, h.nx, h.ny, h.nz);
+      scur = nv;
+    }
+  }
+  a.plane[idx] = cur;
+  a.cost[idx] = ccur;
+  a.support[idx] = static_cast<uint8_t>(min(scur, 255));
+}
+
+// Scores a given plane state (both colours).
+template <int HALF, int STRIDE, int FMT>
+__global__ __launch_bounds__(256) void k_plane_score(PlaneArgs a) {
+  __shared__ float tile[kTileH + 2 * HALF][kTileW + 2 * HALF + 1];
+  int tx, ty;
+  if (!TileOf(a, &tx, &ty)) return;
+  const int x0 = tx * kTileW, y0 = ty * kTileH;
+  LoadRefTile<HALF, FMT>(a, x0, y0, tile);
+  __syncthreads();
+  for (int half = 0; half < 2; ++half) {
+    const int r = threadIdx.x / 16, j = threadIdx.x % 16;
+    const int y = y0 + r;
+    const int x = x0 + 2 * j + (((x0 + y) & 1) ^ half);
+    if (x >= a.w || y >= a.h) continue;
+    RefWindow<HALF, STRIDE> win;
+    MakeRefWindow<HALF, STRIDE>(win, [&](int dx, int dy) {
+      return tile[r + HALF + dy][x - x0 + HALF + dx];
+    });
+    const int idx = y * a.w + x;
+    const float4 p = a.plane[idx];
+    int nv = 0;
+    const float c = PlaneCost<HALF, STRIDE, FMT>(a, x, y, win, Hyp{p.x, p.y, p.z, p.w}, &nv, nullptr);
+    a.cost[idx] = c;
+    a.support[idx] = static_cast<uint8_t>(min(nv, 255));
+  }
+}
+
+// Sub-pixel refinement along each pixel's ray with its normal fixed.
+template <int HALF, int STRIDE, int FMT>
+__global__ __launch_bounds__(256) void k_plane_refine(PlaneArgs a, float inv_per_px, int iterations) {
+  __shared__ float tile[kTileH + 2 * HALF][kTileW + 2 * HALF + 1];
+  int tx, ty;
+  if (!TileOf(a, &tx, &ty)) return;
+  const int x0 = tx * kTileW, y0 = ty * kTileH;
+  LoadRefTile<HALF, FMT>(a, x0, y0, tile);
+  __syncthreads();
+  for (int half = 0; half < 2; ++half) {
+    const int r = threadIdx.x / 16, j = threadIdx.x % 16;
+    const int y = y0 + r;
+    const int x = x0 + 2 * j + (((x0 + y) & 1) ^ half);
+    if (x >= a.w || y >= a.h) continue;
+    const int idx = y * a.w + x;
+    const float4 p = a.plane[idx];
+    float best_c = a.cost[idx];
+    if (!(best_c < kCostInf)) continue;
+    RefWindow<HALF, STRIDE> win;
+    MakeRefWindow<HALF, STRIDE>(win, [&](int dx, int dy) {
+      return tile[r + HALF + dy][x - x0 + HALF + dx];
+    });
+    int best_s = a.support[idx];
+    const float inv0 = 1.0f / p.x;
+    float best_i = inv0;
+    auto eval = [&](float inv, int* nv) -> float {
+      if (!(inv >= a.inv_min && inv <= a.inv_max)) {
+        *nv = 0;
+        return kCostInf;
+      }
+      return PlaneCost<HALF, STRIDE, FMT>(a, x, y, win, Hyp{1.0f / inv, p.y, p.z, p.w}, nv, nullptr);
+    };
+    const float step = 0.5f * inv_per_px;
+    float f[5];
+    int nvs[5];
+    bool fit_ok = true;
+    for (int k = 0; k < 5; ++k) {
+      if (k == 2) {
+        f[k] = best_c;
+        nvs[k] = best_s;
+        continue;
+      }
+      f[k] = eval(inv0 + static_cast<float>(k - 2) * step, &nvs[k]);
+      fit_ok = fit_ok && f[k] < kCostInf;
+    }
+    if (fit_ok) {
+      // least-squares parabola through 5 equally spaced samples (x = -2..2)
+      const float aa = (2.0f * f[0] - f[1] - 2.0f * f[2] - f[3] + 2.0f * f[4]) / 14.0f;
+      const float bb = (-2.0f * f[0] - f[1] + f[3] + 2.0f * f[4]) / 10.0f;
+      if (aa > 1e-12f) {
+        const float vtx = -bb / (2.0f * aa);
+        if (vtx >= -2.0f && vtx <= 2.0f) {
+          int nv = 0;
+          const float cand = inv0 + vtx * step;
+          const float cc = eval(cand, &nv);
+          if (cc < best_c) {
+            best_c = cc;
+            best_i = cand;
+            best_s = nv;
+          }
+        }
+      }
+    }
+    float lo = fmaxf(best_i - inv_per_px, a.inv_min), hi = fminf(best_i + inv_per_px, a.inv_max);
+    if (hi > lo && iterations > 0) {
+      const float ip = 0.6180339887498949f;
+      float c1 = hi - ip * (hi - lo), c2 = lo + ip * (hi - lo);
+      int n1 = 0, n2 = 0;
+      float f1 = eval(c1, &n1), f2 = eval(c2, &n2);
+      for (int k = 0; k < iterations; ++k) {
+        if (f1 < f2) {
+          hi = c2; c2 = c1; f2 = f1; n2 = n1;
+          c1 = hi - ip * (hi - lo);
+          f1 = eval(c1, &n1);
+        } else {
+          lo = c1; c1 = c2; f1 = f2; n1 = n2;
+          c2 = lo + ip * (hi - lo);
+          f2 = eval(c2, &n2);
+        }
+      }
+      if (f1 < best_c) { best_c = f1; best_i = c1; best_s = n1; }
+      if (f2 < best_c) { best_c = f2; best_i = c2; best_s = n2; }
+    }
+    a.plane[idx] = make_float4(1.0f / best_i, p.y, p.z, p.w);
+    a.cost[idx] = best_c;
+    a.support[idx] = static_cast<uint8_t>(min(best_s, 255));
+  }
+}
+
+// Cost of explicit hypotheses (parity tests): one thread per query, reference
+// window read from global memory.
+template <int HALF, int STRIDE, int FMT>
+__global__ void k_plane_eval(PlaneArgs a, int n, const float* __restrict__ hyps,
+                             float* __restrict__ cost, int* __restrict__ nvalid,
+                             float* __restrict__ view_ncc) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const float* hq = hyps + 6 * q;
+  const int x = static_cast<int>(hq[0]), y = static_cast<int>(hq[1]);
+  if (x < 0 || y < 0 || x >= a.w || y >= a.h) {
+    cost[q] = kCostInf;
+    nvalid[q] = 0;
+    if (view_ncc)
+      for (int v = 0; v < a.nsrc; ++v) view_ncc[q * a.nsrc + v] = __builtin_nanf("");
+    return;
+  }
+  RefWindow<HALF, STRIDE> win;
+  MakeRefWindow<HALF, STRIDE>(win, [&](int dx, int dy) {
+    const int gx = x + dx, gy = y + dy;
+    if (gx < 0 || gy < 0 || gx >= a.w || gy >= a.h) return __builtin_nanf("");
+    return Pix<FMT>(a.ref_img, gy * a.ref_pitch + gx);
+  });
+  float vn[kMaxViews];
+  int nv = 0;
+  const float c = PlaneCost<HALF, STRIDE, FMT>(a, x, y, win, Hyp{hq[2], hq[3], hq[4], hq[5]}, &nv,
+                                               view_ncc ? vn : nullptr);
+  cost[q] = c;
+  nvalid[q] = nv;
+  if (view_ncc)
+    for (int v = 0; v < a.nsrc; ++v) view_ncc[q * a.nsrc + v] = vn[v];
+}
+
+// Converts an uploaded view (u8 or f32 rows) into the padded storage layout.
+template <int FMT>
+__global__ void k_store_view(const void* __restrict__ src, int src_format, int w, int h, int pitch,
+                             void* __restrict__ dst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= pitch * (h + 1)) return;
+  const int y = i / pitch, x = i % pitch;
+  float v = 0.0f;
+  if (x < w && y < h) {
+    v = src_format == RECON_MVS_PIXEL_U8 ? static_cast<float>(static_cast<const uint8_t*>(src)[y * w + x])
+                                         : static_cast<const float*>(src)[y * w + x];
+  }
+  if constexpr (FMT == RECON_MVS_PIXEL_F32) {
+    static_cast<float*>(dst)[i] = v;
+  } else if constexpr (FMT == RECON_MVS_PIXEL_F16) {
+    static_cast<_Float16*>(dst)[i] = static_cast<_Float16>(v);
+  } else {
+    // round to nearest even into bfloat16
+    const uint32_t b = __float_as_uint(v);
+    const uint32_t r = b + 0x7FFFu + ((b >> 16) & 1u);
+    static_cast<uint16_t*>(dst)[i] = static_cast<uint16_t>(r >> 16);
+  }
+}
+
+// Geometric consistency of map r against the other listed maps (FP64).
+struct GeoView {
+  int w, h;
+  const float4* plane;
+  const float* cost;
+  recon_mvs_camera cam;
+};
+constexpr int kMaxGeo = 16;
+struct GeoArgs {
+  int nmaps, self;
+  GeoView v[kMaxGeo];
+  double thr;
+  uint8_t* geo_support;
+};
+
+__global__ void k_geo_consistency(GeoArgs g) {
+  const GeoView& me = g.v[g.self];
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= me.w * me.h) return;
+  const float c = me.cost[idx];
+  const float d = me.plane[idx].x;
+  int support = 0;
+  if (c < kCostInf) {
+    const int x = idx % me.w, y = idx / me.w;
+    double ray[3];
+    PixelToRay(me.cam, x + 0.5, y + 0.5, ray);
+    const double dd = static_cast<double>(d);
+    const double pc[3] = {ray[0] * dd, ray[1] * dd, ray[2] * dd};
+    double pw[3];
+    CamToWorld(me.cam, pc, pw);
+    support = 1;
+    for (int m = 0; m < g.nmaps; ++m) {
+      if (m == g.self) continue;
+      const GeoView& o = g.v[m];
+      double ps[3];
+      WorldToCam(o.cam, pw, ps);
+      if (ps[2] <= 1e-6) continue;
+      double u, v;
+      if (!Project(o.cam, ps, &u, &v)) continue;
+      const double fu = floor(u), fv = floor(v);
+      if (fu < 0.0 || fv < 0.0 || fu >= o.w || fv >= o.h) continue;
+      const int q = static_cast<int>(fv) * o.w + static_cast<int>(fu);
+      if (!(o.cost[q] < kCostInf)) continue;
+      const double zm = static_cast<double>(o.plane[q].x);
+      const double e = fabs(ps[2] - zm) / fmax(ps[2], zm);
+      if (e <= g.thr) ++support;
+    }
+  }
+  g.geo_support[idx] = static_cast<uint8_t>(min(support, 255));
+}
+
+// Output of one slot (finalize): depth NaN where invalid or filtered.
+__global__ void k_plane_output(int P, const float4* __restrict__ plane, const float* __restrict__ cost,
+                               const uint8_t* __restrict__ geo, int min_views, float* __restrict__ depth,
+                               float* __restrict__ conf) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P) return;
+  const float c = cost[i];
+  bool ok = c < kCostInf;
+  if (geo) ok = ok && geo[i] >= min_views;
+  depth[i] = ok ? plane[i].x : __builtin_nanf("");
+  conf[i] = ok ? 1.0f - fminf(fmaxf(c, 0.0f), 1.0f) : 0.0f;
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+namespace {
+
+void FreeView(ResidentView* v) {
+  if (v && v->pixels) (void)hipFree(v->pixels);
+  delete v;
+}
+
+void FreeState(PlaneState* s) {
+  if (!s) return;
+  if (s->plane) (void)hipFree(s->plane);
+  if (s->cost) (void)hipFree(s->cost);
+  if (s->support) (void)hipFree(s->support);
+  if (s->geo_support) (void)hipFree(s->geo_support);
+  delete s;
+}
+
+int32_t CheckParams(recon_mvs_hip_context* ctx, const recon_mvs_plane_params* p) {
+  if (!p) return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "null plane params");
+  if (!(p->min_depth_m > 0.0 && p->max_depth_m > p->min_depth_m))
+    return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "bad depth range");
+  if (p->top_k < 0 || p->top_k > kMaxTopK) return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "top_k out of [0, 16]");
+  if (p->iterations < 0 || p->propagation_reach < 1 || p->propagation_reach > 64)
+    return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "bad iterations / propagation_reach");
+  if (!(p->value_scale > 0.0)) return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "value_scale must be > 0");
+  return RECON_MVS_OK;
+}
+
+// Builds the kernel arguments for ref view `ref` and its sources.
+int32_t MakeArgs(recon_mvs_hip_context* ctx, int ref, int nsrc, const int32_t* srcs,
+                 const recon_mvs_plane_params* p, PlaneArgs* a) {
+  int32_t st = CheckParams(ctx, p);
+  if (st != RECON_MVS_OK) return st;
+  const int nv = static_cast<int>(ctx->views.size());
+  if (ref < 0 || ref >= nv) return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "ref_view out of range");
+  if (nsrc < 1 || nsrc > kMaxViews || !srcs)
+    return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "num_src out of [1, 16]");
+  for (int k = 0; k < nsrc; ++k)
+    if (srcs[k] < 0 || srcs[k] >= nv || srcs[k] == ref)
+      return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "bad source view index");
+  const ResidentView& R = *ctx->views[ref];
+  std::memset(a, 0, sizeof(*a));
+  a->w = R.w;
+  a->h = R.h;
+  a->ref_pitch = R.pitch;
+  a->ref_img = R.pixels;
+  a->fx = R.cam.fx; a->fy = R.cam.fy; a->cx = R.cam.cx; a->cy = R.cam.cy;
+  a->ffx = static_cast<float>(R.cam.fx); a->ffy = static_cast<float>(R.cam.fy);
+  a->fcx = static_cast<float>(R.cam.cx); a->fcy = static_cast<float>(R.cam.cy);
+  a->nsrc = nsrc;
+  a->top_k = p->top_k == 0 ? nsrc : std::min(p->top_k, nsrc);
+  // K_r^-1
+  const double Ki[9] = {1.0 / R.cam.fx, 0.0, -R.cam.cx / R.cam.fx,
+                        0.0, 1.0 / R.cam.fy, -R.cam.cy / R.cam.fy,
+                        0.0, 0.0, 1.0};
+  for (int k = 0; k < nsrc; ++k) {
+    const ResidentView& S = *ctx->views[srcs[k]];
+    a->src_img[k] = S.pixels;
+    a->src_pitch[k] = S.pitch;
+    a->src_w[k] = S.w;
+    a->src_h[k] = S.h;
+    // R_rel = R_s R_r^T, t_rel = t_s - R_rel t_r (FP64 composition per view pair)
+    double Rrel[9], trel[3];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        double s = 0.0;
+        for (int l = 0; l < 3; ++l) s += S.cam.R[3 * i + l] * R.cam.R[3 * j + l];
+        Rrel[3 * i + j] = s;
+      }
+    for (int i = 0; i < 3; ++i) {
+      double s = 0.0;
+      for (int l = 0; l < 3; ++l) s += Rrel[3 * i + l] * R.cam.t[l];
+      trel[i] = S.cam.t[i] - s;
+    }
+    const double Ks[9] = {S.cam.fx, 0.0, S.cam.cx, 0.0, S.cam.fy, S.cam.cy, 0.0, 0.0, 1.0};
+    double KR[9];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        double s = 0.0;
+        for (int l = 0; l < 3; ++l) s += Ks[3 * i + l] * Rrel[3 * l + j];
+        KR[3 * i + j] = s;
+      }
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        double s = 0.0;
+        for (int l = 0; l < 3; ++l) s += KR[3 * i + l] * Ki[3 * l + j];
+        a->A[k][3 * i + j] = s;
+      }
+    for (int i = 0; i < 3; ++i) {
+      double s = 0.0;
+      for (int l = 0; l < 3; ++l) s += Ks[3 * i + l] * trel[l];
+      a->b[k][i] = s;
+    }
+  }
+  a->min_depth = static_cast<float>(p->min_depth_m);
+  a->max_depth = static_cast<float>(p->max_depth_m);
+  a->inv_min = static_cast<float>(1.0 / p->max_depth_m);
+  a->inv_max = static_cast<float>(1.0 / p->min_depth_m);
+  const double eps = 1e-9 * (p->value_scale / 255.0) * (p->value_scale / 255.0);
+  a->eps2 = static_cast<float>(eps * eps);
+  a->photo_reject = static_cast<float>(3.0 * p->photo_threshold);
+  a->seed = p->seed;
+  a->reach = p->propagation_reach;
+  a->depth_perturb = static_cast<float>(p->depth_perturb);
+  a->normal_perturb = static_cast<float>(p->normal_perturb);
+  a->tiles_x = (a->w + kTileW - 1) / kTileW;
+  a->tiles_y = (a->h + kTileH - 1) / kTileH;
+  a->num_tiles = a->tiles_x * a->tiles_y;
+  a->tiles_per_xcd = (a->num_tiles + 7) / 8;
+  return RECON_MVS_OK;
+}
+
+int32_t EnsureState(recon_mvs_hip_context* ctx, int view) {
+  if (static_cast<int>(ctx->states.size()) < static_cast<int>(ctx->views.size()))
+    ctx->states.resize(ctx->views.size(), nullptr);
+  PlaneState*& s = ctx->states[view];
+  const ResidentView& V = *ctx->views[view];
+  if (s && s->w == V.w && s->h == V.h) return RECON_MVS_OK;
+  FreeState(s);
+  s = new PlaneState();
+  s->w = V.w;
+  s->h = V.h;
+  const size_t P = static_cast<size_t>(V.w) * V.h;
+  if (hipMalloc(&s->plane, P * sizeof(float4)) != hipSuccess ||
+      hipMalloc(&s->cost, P * sizeof(float)) != hipSuccess ||
+      hipMalloc(&s->support, P) != hipSuccess || hipMalloc(&s->geo_support, P) != hipSuccess) {
+    FreeState(s);
+    s = nullptr;
+    return Fail(ctx, RECON_MVS_ERR_OUT_OF_MEMORY, "plane state allocation failed");
+  }
+  return RECON_MVS_OK;
+}
+
+enum KernelId { kSweep = 0, kInit = 1, kRefine = 2, kGeo = 3, kScore = 4, kEval = 5 };
+
+template <int HALF, int STRIDE, int FMT>
+void LaunchT(KernelId id, const PlaneArgs& a, hipStream_t s, float f0, int i0, int n,
+             const float* hyps, float* cost, int* nvalid, float* view_ncc) {
+  const dim3 grid(a.tiles_per_xcd * 8), block(256);
+  switch (id) {
+    case kSweep: hipLaunchKernelGGL((k_plane_sweep<HALF, STRIDE, FMT>), grid, block, 0, s, a); break;
+    case kInit: hipLaunchKernelGGL((k_plane_init<HALF, STRIDE, FMT>), grid, block, 0, s, a); break;
+    case kScore: hipLaunchKernelGGL((k_plane_score<HALF, STRIDE, FMT>), grid, block, 0, s, a); break;
+    case kRefine:
+      hipLaunchKernelGGL((k_plane_refine<HALF, STRIDE, FMT>), grid, block, 0, s, a, f0, i0);
+      break;
+    case kEval:
+      hipLaunchKernelGGL((k_plane_eval<HALF, STRIDE, FMT>), dim3((n + 127) / 128), dim3(128), 0, s, a,
+                         n, hyps, cost, nvalid, view_ncc);
+      break;
+    default: break;
+  }
+}
+
+template <int HALF, int STRIDE>
+bool LaunchFmt(int fmt, KernelId id, const PlaneArgs& a, hipStream_t s, float f0, int i0, int n,
+               const float* hyps, float* cost, int* nvalid, float* view_ncc) {
+  // FP16 / BF16 storage is compiled for the 11x11 stride-2 window only
+  constexpr bool kAllFormats = (HALF == 5 && STRIDE == 2);
+  switch (fmt) {
+    case RECON_MVS_PIXEL_F32: LaunchT<HALF, STRIDE, RECON_MVS_PIXEL_F32>(id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc); return true;
+    case RECON_MVS_PIXEL_F16:
+      if constexpr (kAllFormats) { LaunchT<HALF, STRIDE, RECON_MVS_PIXEL_F16>(id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc); return true; }
+      return false;
+    case RECON_MVS_PIXEL_BF16:
+      if constexpr (kAllFormats) { LaunchT<HALF, STRIDE, RECON_MVS_PIXEL_BF16>(id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc); return true; }
+      return false;
+    default: return false;
+  }
+}
+
+// Window shapes compiled in: (patch, stride) in {(11,2), (11,1), (7,1), (5,1), (7,2), (9,2)}.
+int32_t Launch(recon_mvs_hip_context* ctx, const recon_mvs_plane_params* p, KernelId id,
+               const PlaneArgs& a, float f0 = 0.0f, int i0 = 0, int n = 0,
+               const float* hyps = nullptr, float* cost = nullptr, int* nvalid = nullptr,
+               float* view_ncc = nullptr) {
+  const int fmt = ctx->storage_format;
+  hipEvent_t eb, ee;
+  TimerBegin(ctx, static_cast<int>(id), &eb, &ee);
+  bool ok = false;
+  const int ps = p->patch_size, st = p->window_stride;
+  if (ps == 11 && st == 2) ok = LaunchFmt<5, 2>(fmt, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc);
+  else if (ps == 11 && st == 1) ok = LaunchFmt<5, 1>(fmt, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc);
+  else if (ps == 7 && st == 1) ok = LaunchFmt<3, 1>(fmt, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc);
+  else if (ps == 5 && st == 1) ok = LaunchFmt<2, 1>(fmt, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc);
+  else if (ps == 7 && st == 2) ok = LaunchFmt<3, 2>(fmt, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc);
+  else if (ps == 9 && st == 2) ok = LaunchFmt<4, 2>(fmt, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc);
+  if (!ok) return Fail(ctx, RECON_MVS_ERR_NOT_IMPLEMENTED, "window (patch_size, window_stride) / storage format not compiled in");
+  TimerEnd(ctx, static_cast<int>(id), eb, ee);
+  RMVS_LAUNCH_CHECK(ctx);
+  return RECON_MVS_OK;
+}
+
+double MeanBaselineViews(recon_mvs_hip_context* ctx, int ref, int nsrc, const int32_t* srcs) {
+  std::vector<recon_mvs_camera> c(nsrc);
+  for (int k = 0; k < nsrc; ++k) c[k] = ctx->views[srcs[k]]->cam;
+  return MeanBaseline(ctx->views[ref]->cam, c.data(), nsrc);
+}
+
+}  // namespace
+
+void DestroyPlaneState(recon_mvs_hip_context* ctx) {
+  for (auto* s : ctx->states) FreeState(s);
+  ctx->states.clear();
+  for (auto* v : ctx->views) FreeView(v);
+  ctx->views.clear();
+}
+
+extern "C" {
+
+RECON_MVS_API void recon_mvs_plane_params_default(recon_mvs_plane_params* p) {
+  if (!p) return;
+  std::memset(p, 0, sizeof(*p));
+  p->patch_size = 11;
+  p->window_stride = 2;
+  p->top_k = 4;
+  p->iterations = 5;
+  p->seed = 0;
+  p->propagation_reach = 4;
+  p->refine_iterations = 4;
+  p->min_depth_m = 0.10;
+  p->max_depth_m = 1.50;
+  p->depth_perturb = 0.10;
+  p->normal_perturb = 0.30;
+  p->photo_threshold = 1.0;
+  p->value_scale = 1.0;
+}
+
+RECON_MVS_API int32_t recon_mvs_hip_set_views(recon_mvs_hip_context* ctx, int32_t n,
+                                              const recon_mvs_view* views, int32_t storage_format) {
+  if (!ctx) return RECON_MVS_ERR_INVALID_ARGUMENT;
+  if (n < 0 || (n > 0 && !views)) return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "bad views");
+  if (storage_format != RECON_MVS_PIXEL_F32 && storage_format != RECON_MVS_PIXEL_F16 &&
+      storage_format != RECON_MVS_PIXEL_BF16)
+    return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "storage format must be F32, F16 or BF16");
+  for (int i = 0; i < n; ++i) {
+    const recon_mvs_view& v = views[i];
+    if (!v.pixels || v.camera.width <= 0 || v.camera.height <= 0 ||
+        (v.pixel_format != RECON_MVS_PIXEL_U8 && v.pixel_format != RECON_MVS_PIXEL_F32))
+      return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "bad view");
+  }
+  RMVS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  DestroyPlaneState(ctx);
+  ctx->storage_format = storage_format;
+  const size_t esz = storage_format == RECON_MVS_PIXEL_F32 ? 4 : 2;
+  for (int i = 0; i < n; ++i) {
+    const recon_mvs_view& in = views[i];
+    auto* v = new ResidentView();
+    v->w = in.camera.width;
+    v->h = in.camera.height;
+    v->pitch = (v->w + 1 + 63) & ~63;  // >= w + 1: the right tap of the last column is padding
+    v->format = storage_format;
+    v->cam = in.camera;
+    ctx->views.push_back(v);
+    const size_t bytes = static_cast<size_t>(v->pitch) * (v->h + 1) * esz;
+    if (hipMalloc(&v->pixels, bytes) != hipSuccess)
+      return Fail(ctx, RECON_MVS_ERR_OUT_OF_MEMORY, "view allocation failed");
+    const size_t in_bytes = static_cast<size_t>(v->w) * v->h * (in.pixel_format == RECON_MVS_PIXEL_U8 ? 1 : 4);
+    void* staging = Ensure(ctx, kSlotQueryImgA, in_bytes);
+    if (!staging) return Fail(ctx, RECON_MVS_ERR_OUT_OF_MEMORY, "staging allocation failed");
+    RMVS_HIP(ctx, hipMemcpyAsync(staging, in.pixels, in_bytes, hipMemcpyHostToDevice, ctx->stream));
+    const int total = v->pitch * (v->h + 1);
+    const dim3 g((total + 255) / 256), b(256);
+    if (storage_format == RECON_MVS_PIXEL_F32)
+      hipLaunchKernelGGL(k_store_view<RECON_MVS_PIXEL_F32>, g, b, 0, ctx->stream, staging, in.pixel_format, v->w, v->h, v->pitch, v->pixels);
+    else if (storage_format == RECON_MVS_PIXEL_F16)
+      hipLaunchKernelGGL(k_store_view<RECON_MVS_PIXEL_F16>, g, b, 0, ctx->stream, staging, in.pixel_format, v->w, v->h, v->pitch, v->pixels);
+    else
+      hipLaunchKernelGGL(k_store_view<RECON_MVS_PIXEL_BF16>, g, b, 0, ctx->stream, staging, in.pixel_format, v->w, v->h, v->pitch, v->pixels);
+    RMVS_LAUNCH_CHECK(ctx);
+    // staging is reused by the next view: finish this one first
+    RMVS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  ctx->states.assign(ctx->views.size(), nullptr);
+  return RECON_MVS_OK;
+}
+
+RECON_MVS_API int32_t recon_mvs_hip_plane_patchmatch(recon_mvs_hip_context* ctx, int32_t ref_view,
+                                                     int32_t num_src, const int32_t* src_views,
+                                                     const recon_mvs_plane_params* params) {
+  if (!ctx) return RECON_MVS_ERR_INVALID_ARGUMENT;
+  PlaneArgs a;
+  int32_t st = MakeArgs(ctx, ref_view, num_src, src_views, params, &a);
+  if (st != RECON_MVS_OK) return st;
+  st = EnsureState(ctx, ref_view);
+  if (st != RECON_MVS_OK) return st;
+  PlaneState& S = *ctx->states[ref_view];
+  S.has_geo = false;
+  a.plane = S.plane;
+  a.cost = S.cost;
+  a.support = S.support;
+  st = Launch(ctx, params, kInit, a);
+  if (st != RECON_MVS_OK) return st;
+  for (int it = 0; it < params->iterations; ++it) {
+    for (int color = 0; color < 2; ++color) {
+      a.iter = it;
+      a.color = color;
+      st = Launch(ctx, params, kSweep, a);
+      if (st != RECON_MVS_OK) return st;
+    }
+  }
+  return RECON_MVS_OK;
+}
+
+RECON_MVS_API int32_t recon_mvs_hip_plane_refine(recon_mvs_hip_context* ctx, int32_t ref_view,
+                                                 int32_t num_src, const int32_t* src_views,
+                                                 const recon_mvs_plane_params* params) {
+  if (!ctx) return RECON_MVS_ERR_INVALID_ARGUMENT;
+  PlaneArgs a;
+  int32_t st = MakeArgs(ctx, ref_view, num_src, src_views, params, &a);
+  if (st != RECON_MVS_OK) return st;
+  if (ref_view >= static_cast<int>(ctx->states.size()) || !ctx->states[ref_view])
+    return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "no depth map in this slot");
+  PlaneState& S = *ctx->states[ref_view];
+  a.plane = S.plane;
+  a.cost = S.cost;
+  a.support = S.support;
+  const double baseline = MeanBaselineViews(ctx, ref_view, num_src, src_views);
+  const ResidentView& R = *ctx->views[ref_view];
+  const double focal = 0.5 * (R.cam.fx + R.cam.fy);
+  if (!(baseline > 1e-9)) return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "zero baseline");
+  const float inv_per_px = static_cast<float>(1.0 / (focal * baseline));
+  return Launch(ctx, params, kRefine, a, inv_per_px, params->refine_iterations);
+}
+
+RECON_MVS_API int32_t recon_mvs_hip_set_plane_state(recon_mvs_hip_context* ctx, int32_t view,
+                                                    int32_t num_src, const int32_t* src_views,
+                                                    const recon_mvs_plane_params* params,
+                                                    const float* depth, const float* normals) {
+  if (!ctx) return RECON_MVS_ERR_INVALID_ARGUMENT;
+  if (!depth || !normals) return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "null state");
+  PlaneArgs a;
+  int32_t st = MakeArgs(ctx, view, num_src, src_views, params, &a);
+  if (st != RECON_MVS_OK) return st;
+  st = EnsureState(ctx, view);
+  if (st != RECON_MVS_OK) return st;
+  PlaneState& S = *ctx->states[view];
+  const size_t P = static_cast<size_t>(S.w) * S.h;
+  std::vector<float4> host(P);
+  for (size_t i = 0; i < P; ++i)
+    host[i] = make_float4(depth[i], normals[3 * i], normals[3 * i + 1], normals[3 * i + 2]);
+  RMVS_HIP(ctx, hipMemcpyAsync(S.plane, host.data(), P * sizeof(float4), hipMemcpyHostToDevice, ctx->stream));
+  a.plane = S.plane;
+  a.cost = S.cost;
+  a.support = S.support;
+  st = Launch(ctx, params, kScore, a);
+  if (st != RECON_MVS_OK) return st;
+  RMVS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  S.has_geo = false;
+  return RECON_MVS_OK;
+}
+
+RECON_MVS_API int32_t recon_mvs_hip_geo_consistency(recon_mvs_hip_context* ctx, int32_t num_maps,
+                                                    const int32_t* views, double threshold,
+                                                    int32_t min_views) {
+  (void)min_views;
+  if (!ctx) return RECON_MVS_ERR_INVALID_ARGUMENT;
+  if (num_maps < 1 || num_maps > kMaxGeo || !views)
+    return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "num_maps out of [1, 16]");
+  GeoArgs g;
+  std::memset(&g, 0, sizeof(g));
+  g.nmaps = num_maps;
+  g.thr = threshold;
+  for (int m = 0; m < num_maps; ++m) {
+    const int v = views[m];
+    if (v < 0 || v >= static_cast<int>(ctx->states.size()) || !ctx->states[v])
+      return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "view without a depth map");
+    const PlaneState& S = *ctx->states[v];
+    g.v[m].w = S.w;
+    g.v[m].h = S.h;
+    g.v[m].plane = S.plane;
+    g.v[m].cost = S.cost;
+    g.v[m].cam = ctx->views[v]->cam;
+  }
+  for (int m = 0; m < num_maps; ++m) {
+    PlaneState& S = *ctx->states[views[m]];
+    g.self = m;
+    g.geo_support = S.geo_support;
+    hipEvent_t eb, ee;
+    TimerBegin(ctx, kGeo, &eb, &ee);
+    hipLaunchKernelGGL(k_geo_consistency, dim3((S.w * S.h + 255) / 256), dim3(256), 0, ctx->stream, g);
+    TimerEnd(ctx, kGeo, eb, ee);
+    RMVS_LAUNCH_CHECK(ctx);
+    S.has_geo = true;
+  }
+  return RECON_MVS_OK;
+}
+
+RECON_MVS_API int32_t recon_mvs_hip_get_depth_map(recon_mvs_hip_context* ctx, int32_t view,
+                                                  int32_t drop_inconsistent, float* depth,
+                                                  float* normals, float* cost, uint8_t* support,
+                                                  float* confidence, uint8_t* geo_support) {
+  if (!ctx) return RECON_MVS_ERR_INVALID_ARGUMENT;
+  if (view < 0 || view >= static_cast<int>(ctx->states.size()) || !ctx->states[view])
+    return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "no depth map in this slot");
+  PlaneState& S = *ctx->states[view];
+  const size_t P = static_cast<size_t>(S.w) * S.h;
+  float* d_depth = static_cast<float*>(Ensure(ctx, kSlotOutDepth, P * 4));
+  float* d_conf = static_cast<float*>(Ensure(ctx, kSlotOutConf, P * 4));
+  if (!d_depth || !d_conf) return Fail(ctx, RECON_MVS_ERR_OUT_OF_MEMORY, "alloc failed");
+  const int min_views = drop_inconsistent ? drop_inconsistent : 0;
+  hipLaunchKernelGGL(k_plane_output, dim3((P + 255) / 256), dim3(256), 0, ctx->stream, static_cast<int>(P),
+                     S.plane, S.cost, (drop_inconsistent && S.has_geo) ? S.geo_support : nullptr,
+                     min_views, d_depth, d_conf);
+  RMVS_LAUNCH_CHECK(ctx);
+  if (depth) RMVS_HIP(ctx, hipMemcpyAsync(depth, d_depth, P * 4, hipMemcpyDeviceToHost, ctx->stream));
+  if (confidence) RMVS_HIP(ctx, hipMemcpyAsync(confidence, d_conf, P * 4, hipMemcpyDeviceToHost, ctx->stream));
+  if (cost) RMVS_HIP(ctx, hipMemcpyAsync(cost, S.cost, P * 4, hipMemcpyDeviceToHost, ctx->stream));
+  if (support) RMVS_HIP(ctx, hipMemcpyAsync(support, S.support, P, hipMemcpyDeviceToHost, ctx->stream));
+  if (geo_support) {
+    if (!S.has_geo) return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "consistency filter has not run");
+    RMVS_HIP(ctx, hipMemcpyAsync(geo_support, S.geo_support, P, hipMemcpyDeviceToHost, ctx->stream));
+  }
+  if (normals) {
+    std::vector<float4> host(P);
+    RMVS_HIP(ctx, hipMemcpyAsync(host.data(), S.plane, P * sizeof(float4), hipMemcpyDeviceToHost, ctx->stream));
+    RMVS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (size_t i = 0; i < P; ++i) {
+      normals[3 * i] = host[i].y;
+      normals[3 * i + 1] = host[i].z;
+      normals[3 * i + 2] = host[i].w;
+    }
+  }
+  RMVS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return RECON_MVS_OK;
+}
+
+RECON_MVS_API int32_t recon_mvs_hip_eval_plane_costs(recon_mvs_hip_context* ctx, int32_t ref_view,
+                                                     int32_t num_src, const int32_t* src_views,
+                                                     const recon_mvs_plane_params* params, int32_t n,
+                                                     const float* hyps, float* cost,
+                                                     int32_t* valid_views, float* view_ncc) {
+  if (!ctx) return RECON_MVS_ERR_INVALID_ARGUMENT;
+  if (n < 0 || (n > 0 && (!hyps || !cost || !valid_views)))
+    return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "bad query arrays");
+  PlaneArgs a;
+  int32_t st = MakeArgs(ctx, ref_view, num_src, src_views, params, &a);
+  if (st != RECON_MVS_OK) return st;
+  if (n == 0) return RECON_MVS_OK;
+  float* dh = static_cast<float*>(Ensure(ctx, kSlotQueryIn, sizeof(float) * 6 * n));
+  float* dc = static_cast<float*>(Ensure(ctx, kSlotQueryOut, sizeof(float) * n * (2 + num_src)));
+  if (!dh || !dc) return Fail(ctx, RECON_MVS_ERR_OUT_OF_MEMORY, "alloc failed");
+  int* dv = reinterpret_cast<int*>(dc + n);
+  float* dn = dc + 2 * static_cast<size_t>(n);
+  RMVS_HIP(ctx, hipMemcpyAsync(dh, hyps, sizeof(float) * 6 * n, hipMemcpyHostToDevice, ctx->stream));
+  st = Launch(ctx, params, kEval, a, 0.0f, 0, n, dh, dc, dv, view_ncc ? dn : nullptr);
+  if (st != RECON_MVS_OK) return st;
+  RMVS_HIP(ctx, hipMemcpyAsync(cost, dc, sizeof(float) * n, hipMemcpyDeviceToHost, ctx->stream));
+  RMVS_HIP(ctx, hipMemcpyAsync(valid_views, dv, sizeof(int) * n, hipMemcpyDeviceToHost, ctx->stream));
+  if (view_ncc)
+    RMVS_HIP(ctx, hipMemcpyAsync(view_ncc, dn, sizeof(float) * n * num_src, hipMemcpyDeviceToHost, ctx->stream));
+  RMVS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return RECON_MVS_OK;
+}
+
+RECON_MVS_API int32_t recon_mvs_hip_timing(recon_mvs_hip_context* ctx, int32_t enable, int32_t reset,
+                                           double* ms, int64_t* counts) {
+  if (!ctx) return RECON_MVS_ERR_INVALID_ARGUMENT;
+  TimerCollect(ctx);
+  if (ms)
+    for (int i = 0; i < 8; ++i) ms[i] = ctx->timed_ms[i];
+  if (counts)
+    for (int i = 0; i < 8; ++i) counts[i] = ctx->timed_count[i];
+  if (reset)
+    for (int i = 0; i < 8; ++i) {
+      ctx->timed_ms[i] = 0.0;
+      ctx->timed_count[i] = 0;
+    }
+  ctx->timing = enable != 0;
+  return RECON_MVS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,184 @@
+"""Plane-mode PatchMatch on the GPU: views resident in HBM, slanted-plane
+hypotheses with homography warps, top-k NCC, red-black sweeps, geometric
+consistency (include/recon_mvs_hip.h, "Plane mode").
+
+    pm = PlaneMVS(scene_cams, scene_images, storage="f32")
+    pm.patchmatch(ref, srcs, params)        # asynchronous
+    pm.refine(ref, srcs, params)            # optional sub-pixel pass
+    pm.geo_consistency(views, 0.01)
+    d = pm.depth_map(ref)                   # synchronous copy to the host
+
+No CPU fallback: every call goes through librecon_mvs_hip.so.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+from . import mvs
+
+FORMATS = {"f32": N.PIXEL_F32, "f16": N.PIXEL_F16, "bf16": N.PIXEL_BF16}
+KERNEL_CLASSES = ("sweep", "init", "refine", "consistency", "score", "eval")
+
+
+class PlaneParams(C.Structure):
+    _fields_ = [("patch_size", C.c_int32), ("window_stride", C.c_int32), ("top_k", C.c_int32),
+                ("iterations", C.c_int32), ("seed", C.c_uint32), ("propagation_reach", C.c_int32),
+                ("refine_iterations", C.c_int32), ("reserved0", C.c_int32),
+                ("min_depth_m", C.c_double), ("max_depth_m", C.c_double),
+                ("depth_perturb", C.c_double), ("normal_perturb", C.c_double),
+                ("photo_threshold", C.c_double), ("value_scale", C.c_double)]
+
+    def copy(self, **kw):
+        p = PlaneParams()
+        C.memmove(C.addressof(p), C.addressof(self), C.sizeof(p))
+        for k, v in kw.items():
+            if not hasattr(p, k):
+                raise AttributeError(k)
+            setattr(p, k, v)
+        return p
+
+
+class View(C.Structure):
+    _fields_ = [("pixels", C.c_void_p), ("pixel_format", C.c_int32), ("reserved0", C.c_int32),
+                ("camera", N.Camera)]
+
+
+_declared = False
+
+
+def _lib():
+    global _declared
+    L = mvs.lib()
+    if not _declared:
+        P = C.POINTER
+        v = C.c_void_p
+        L.recon_mvs_plane_params_default.argtypes = [P(PlaneParams)]
+        L.recon_mvs_plane_params_default.restype = None
+        L.recon_mvs_hip_set_views.argtypes = [v, C.c_int32, P(View), C.c_int32]
+        L.recon_mvs_hip_plane_patchmatch.argtypes = [v, C.c_int32, C.c_int32, v, P(PlaneParams)]
+        L.recon_mvs_hip_plane_refine.argtypes = [v, C.c_int32, C.c_int32, v, P(PlaneParams)]
+        L.recon_mvs_hip_geo_consistency.argtypes = [v, C.c_int32, v, C.c_double, C.c_int32]
+        L.recon_mvs_hip_get_depth_map.argtypes = [v, C.c_int32, C.c_int32, v, v, v, v, v, v]
+        L.recon_mvs_hip_set_plane_state.argtypes = [v, C.c_int32, C.c_int32, v, P(PlaneParams), v, v]
+        L.recon_mvs_hip_eval_plane_costs.argtypes = [v, C.c_int32, C.c_int32, v, P(PlaneParams),
+                                                     C.c_int32, v, v, v, v]
+        L.recon_mvs_hip_timing.argtypes = [v, C.c_int32, C.c_int32, v, v]
+        _declared = True
+    return L
+
+
+def default_params(**kw) -> PlaneParams:
+    p = PlaneParams()
+    _lib().recon_mvs_plane_params_default(C.byref(p))
+    return p.copy(**kw)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, np.int32)
+
+
+class PlaneMVS:
+    """A set of views resident on the GPU and their depth-map slots."""
+
+    def __init__(self, cams, images, storage="f32"):
+        self.L = _lib()
+        self.ctx = mvs.context()
+        self.cams = list(cams)
+        self.shapes = [(c.height, c.width) for c in self.cams]
+        views = (View * len(self.cams))()
+        keep = []
+        for i, (c, img) in enumerate(zip(self.cams, images)):
+            img = np.asarray(img)
+            if img.dtype == np.uint8:
+                a, fmt = np.ascontiguousarray(img), N.PIXEL_U8
+            else:
+                a, fmt = np.ascontiguousarray(img, np.float32), N.PIXEL_F32
+            assert a.shape == (c.height, c.width), (a.shape, c.height, c.width)
+            keep.append(a)
+            views[i].pixels = a.ctypes.data
+            views[i].pixel_format = fmt
+            views[i].camera = c
+        self._check(self.L.recon_mvs_hip_set_views(self.ctx, len(self.cams), views, FORMATS[storage]),
+                    "set_views")
+        self.storage = storage
+
+    def _check(self, st, what):
+        mvs.check(st, what)
+
+    def patchmatch(self, ref, srcs, params):
+        s = _i32(srcs)
+        self._check(self.L.recon_mvs_hip_plane_patchmatch(self.ctx, ref, len(s), s.ctypes.data,
+                                                          C.byref(params)), "plane_patchmatch")
+
+    def refine(self, ref, srcs, params):
+        s = _i32(srcs)
+        self._check(self.L.recon_mvs_hip_plane_refine(self.ctx, ref, len(s), s.ctypes.data,
+                                                      C.byref(params)), "plane_refine")
+
+    def geo_consistency(self, views, threshold, min_views=2):
+        v = _i32(views)
+        self._check(self.L.recon_mvs_hip_geo_consistency(self.ctx, len(v), v.ctypes.data, threshold,
+                                                         min_views), "geo_consistency")
+
+    def set_state(self, ref, srcs, params, depth, normals):
+        s = _i32(srcs)
+        d = np.ascontiguousarray(depth, np.float32)
+        n = np.ascontiguousarray(normals, np.float32)
+        self._check(self.L.recon_mvs_hip_set_plane_state(self.ctx, ref, len(s), s.ctypes.data,
+                                                         C.byref(params), d.ctypes.data,
+                                                         n.ctypes.data), "set_plane_state")
+
+    def synchronize(self):
+        self._check(self.L.recon_mvs_hip_synchronize(self.ctx), "synchronize")
+
+    def depth_map(self, view, drop_inconsistent=0, geo=False):
+        h, w = self.shapes[view]
+        depth = np.zeros((h, w), np.float32)
+        normals = np.zeros((h, w, 3), np.float32)
+        cost = np.zeros((h, w), np.float32)
+        sup = np.zeros((h, w), np.uint8)
+        conf = np.zeros((h, w), np.float32)
+        gs = np.zeros((h, w), np.uint8) if geo else None
+        self._check(self.L.recon_mvs_hip_get_depth_map(
+            self.ctx, view, drop_inconsistent, depth.ctypes.data, normals.ctypes.data,
+            cost.ctypes.data, sup.ctypes.data, conf.ctypes.data,
+            None if gs is None else gs.ctypes.data), "get_depth_map")
+        out = dict(depth=depth, normals=normals, cost=cost, support=sup, confidence=conf)
+        if geo:
+            out["geo_support"] = gs
+        return out
+
+    def eval_costs(self, ref, srcs, params, hyps):
+        s = _i32(srcs)
+        hyps = np.ascontiguousarray(hyps, np.float32).reshape(-1, 6)
+        n = hyps.shape[0]
+        cost = np.zeros(n, np.float32)
+        nv = np.zeros(n, np.int32)
+        vn = np.zeros((n, len(s)), np.float32)
+        self._check(self.L.recon_mvs_hip_eval_plane_costs(self.ctx, ref, len(s), s.ctypes.data,
+                                                          C.byref(params), n, hyps.ctypes.data,
+                                                          cost.ctypes.data, nv.ctypes.data,
+                                                          vn.ctypes.data), "eval_plane_costs")
+        return cost, nv, vn
+
+    def timing(self, enable=True, reset=False):
+        ms = np.zeros(8)
+        cnt = np.zeros(8, np.int64)
+        self._check(self.L.recon_mvs_hip_timing(self.ctx, int(enable), int(reset), ms.ctypes.data,
+                                                cnt.ctypes.data), "timing")
+        return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(KERNEL_CLASSES)}
+
+
+def round_storage(img, storage):
+    """The values the GPU stores for `img` in `storage` (round to nearest even)."""
+    a = np.asarray(img, np.float32)
+    if storage == "f32":
+        return a.copy()
+    if storage == "f16":
+        return a.astype(np.float16).astype(np.float32)
+    b = a.view(np.uint32).astype(np.uint64)
+    r = ((b + 0x7FFF + ((b >> 16) & 1)) >> 16) << 16
+    return r.astype(np.uint32).view(np.float32)

@@ -1,0 +1,156 @@
+"""GPU parity of plane mode (FP32 on the GPU) against the FP64 oracle.
+
+Tolerances (stated per test):
+  * per-view NCC of a hypothesis: |NCC_gpu - NCC_oracle| <= 1e-5 (FP32 warp
+    offsets and sums vs FP64 exact homography), on views valid on both sides;
+  * aggregated cost: |c_gpu - c_oracle| <= 1e-5 * max(1, |c|);
+  * random init planes: bit-exact (both sides generate them with the same FP32
+    operations from the same counter RNG);
+  * final depth after the red-black sweeps: within 0.1 % of the oracle's on
+    >= 99 % of the pixels valid on both sides;
+  * geometric-consistency support counts: bit-exact given the same depth maps.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from recon_scanner_amd import plane, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+def oparams(p):
+    o = oracle.PlaneParams()
+    for f, _ in oracle.PlaneParams._fields_:
+        setattr(o, f, getattr(p, f))
+    return o
+
+
+def random_hyps(sc, n, rng, truth_frac=0.5):
+    h, w = sc.depths[0].shape
+    xs = rng.integers(0, w, n)
+    ys = rng.integers(0, h, n)
+    nrm = rng.normal(size=(n, 3))
+    nrm[:, 2] = -np.abs(nrm[:, 2]) - 0.3
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    d = rng.uniform(sc.min_depth, sc.max_depth, n)
+    k = int(n * truth_frac)
+    t = sc.depths[0][ys[:k], xs[:k]]
+    d[:k] = np.where(t > 0, t * rng.uniform(0.995, 1.005, k), d[:k])
+    nrm[:k] = [0, 0, -1.0]
+    return np.concatenate([xs[:, None], ys[:, None], d[:, None], nrm], 1).astype(np.float32)
+
+
+def compare_eval(sc, srcs, p, storage, n=4000, seed=0):
+    rng = np.random.default_rng(seed)
+    imgs = [plane.round_storage(i, storage) for i in sc.images]
+    pm = plane.PlaneMVS(sc.cams, sc.images, storage=storage)
+    hyps = random_hyps(sc, n, rng)
+    gc, gn, gv = pm.eval_costs(0, srcs, p, hyps)
+    oc, on, ov = oracle.plane_eval(imgs, sc.cams, 0, srcs, oparams(p), hyps)
+    both = np.isfinite(gv) & np.isfinite(ov)
+    agree = np.mean(np.isfinite(gv) == np.isfinite(ov))
+    dn = np.abs(gv[both] - ov[both])
+    say(storage, "views valid both", int(both.sum()), "validity agreement", agree,
+        "max |dNCC|", float(dn.max()), "p99.9", float(np.quantile(dn, 0.999)))
+    assert agree > 0.999
+    assert both.sum() > n
+    assert np.quantile(dn, 0.999) <= 1e-5 and dn.max() <= 1e-4
+    fin = (gc < 1e29) & (oc < 1e29) & (gn == on)
+    dc = np.abs(gc[fin] - oc[fin]) / np.maximum(1.0, np.abs(oc[fin]))
+    assert np.quantile(dc, 0.999) <= 1e-5
+    return gc, oc
+
+
+def test_eval_sphere_arc_7x7_f32():
+    say("eval sphere_arc 320x240 7x7 f32")
+    sc = scenes.sphere_arc()
+    p = plane.default_params(patch_size=7, window_stride=1, top_k=0, min_depth_m=0.25,
+                             max_depth_m=0.45, value_scale=1.0)
+    compare_eval(sc, [1, 2, 3, 4], p, "f32")
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16", "bf16"])
+def test_eval_sphere_planes_11x11_stride2_top4(storage):
+    say("eval sphere_planes 480x360 11x11/2 top-4", storage)
+    sc = scenes.sphere_planes(width=480, height=360, focal=625.0, supersample=1)
+    p = plane.default_params(patch_size=11, window_stride=2, top_k=4, min_depth_m=0.25,
+                             max_depth_m=0.60, value_scale=1.0)
+    compare_eval(sc, list(range(1, 9)), p, storage, n=3000)
+
+
+def test_init_planes_bit_exact_and_sweeps_match_oracle():
+    say("plane patchmatch sphere_arc 320x240 7x7, 3 iterations")
+    sc = scenes.sphere_arc()
+    srcs = [1, 2, 3, 4]
+    p0 = plane.default_params(patch_size=7, window_stride=1, top_k=0, iterations=0,
+                              min_depth_m=0.25, max_depth_m=0.45, value_scale=1.0)
+    pm = plane.PlaneMVS(sc.cams, sc.images)
+    pm.patchmatch(0, srcs, p0)
+    g0 = pm.depth_map(0)
+    op, oc, _ = oracle.plane_patchmatch(sc.images, sc.cams, 0, srcs, oparams(p0))
+    gpl = np.concatenate([g0["depth"][..., None], g0["normals"]], -1)
+    fin = np.isfinite(g0["depth"])
+    assert np.array_equal(gpl[..., 1:], op[..., 1:])           # normals bit-exact
+    assert np.array_equal(g0["depth"][fin], op[..., 0][fin])    # depths bit-exact where scored
+    p = p0.copy(iterations=3)
+    pm.patchmatch(0, srcs, p)
+    g = pm.depth_map(0)
+    op, oc, _ = oracle.plane_patchmatch(sc.images, sc.cams, 0, srcs, oparams(p))
+    od = np.where(oc < 1e29, op[..., 0], np.nan)
+    both = np.isfinite(g["depth"]) & np.isfinite(od)
+    rel = np.abs(g["depth"][both] - od[both]) / od[both]
+    frac = np.mean(rel <= 1e-3)
+    truth = sc.depths[0]
+    m = both & (truth > 0)
+    eg = np.median(np.abs(g["depth"][m] - truth[m]))
+    eo = np.median(np.abs(od[m] - truth[m]))
+    say("within 0.1%:", frac, "median |dZ| gpu", eg, "oracle", eo, "valid", both.sum())
+    assert both.sum() > 0.95 * truth.size
+    assert frac >= 0.99
+    assert eg < 0.001
+
+
+def test_geo_consistency_bit_exact():
+    say("geo consistency 3 views 240x180")
+    sc = scenes.sphere_planes(width=240, height=180, focal=312.5, n_views=3, supersample=1)
+    p = plane.default_params(patch_size=11, window_stride=2, top_k=2, iterations=3,
+                             min_depth_m=0.25, max_depth_m=0.60, value_scale=1.0)
+    pm = plane.PlaneMVS(sc.cams, sc.images)
+    views = [0, 1, 2]
+    for r in views:
+        pm.patchmatch(r, [v for v in views if v != r], p)
+    pm.geo_consistency(views, 0.01)
+    maps = [pm.depth_map(v, geo=True) for v in views]
+    og = oracle.geo_consistency([m["depth"] for m in maps], sc.cams, 0.01)
+    for m, o in zip(maps, og):
+        assert np.array_equal(m["geo_support"], o)
+    kept = np.mean(maps[0]["geo_support"][np.isfinite(maps[0]["depth"])] >= 2)
+    say("kept fraction", kept)
+    assert kept > 0.5
+
+
+def test_refine_keeps_accuracy():
+    say("plane refine sphere_arc")
+    sc = scenes.sphere_arc()
+    srcs = [1, 2, 3, 4]
+    p = plane.default_params(patch_size=7, window_stride=1, top_k=0, iterations=3,
+                             min_depth_m=0.25, max_depth_m=0.45, value_scale=1.0)
+    pm = plane.PlaneMVS(sc.cams, sc.images)
+    pm.patchmatch(0, srcs, p)
+    a = pm.depth_map(0)
+    pm.refine(0, srcs, p)
+    b = pm.depth_map(0)
+    truth = sc.depths[0]
+    m = np.isfinite(a["depth"]) & np.isfinite(b["depth"]) & (truth > 0)
+    ea = np.median(np.abs(a["depth"][m] - truth[m]))
+    eb = np.median(np.abs(b["depth"][m] - truth[m]))
+    say("median before", ea, "after", eb)
+    assert np.all(b["cost"][m] <= a["cost"][m])
+    assert eb <= ea * 1.05
