@@ -75,9 +75,21 @@ def build_scene():
     os.replace(tmp, os.path.join(LIB, "libreconscene.so"))
 
 
+def build_cpp_test():
+    """tests/cpp/test_mvs_api.cpp: a recon:: API caller linked against the library."""
+    out_dir = os.path.join(ROOT, "_build")
+    os.makedirs(out_dir, exist_ok=True)
+    tmp = os.path.join(out_dir, "test_mvs_api.tmp")
+    _run(["g++", "-std=c++17", "-O2", "-I", INC, os.path.join(ROOT, "tests", "cpp", "test_mvs_api.cpp"),
+          "-L", LIB, "-lrecon_mvs_hip", "-lreconscene", f"-Wl,-rpath,{LIB}",
+          "-Wl,-rpath,$ORIGIN/../recon_scanner_amd/_lib", "-o", tmp])
+    os.replace(tmp, os.path.join(out_dir, "test_mvs_api"))
+
+
 def build_all():
     build_scene()
     build_hip()
+    build_cpp_test()
 
 
 if __name__ == "__main__":

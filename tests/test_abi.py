@@ -68,3 +68,30 @@ def test_null_context_is_rejected():
     lib.recon_mvs_hip_last_error.restype = C.c_char_p
     lib.recon_mvs_hip_last_error.argtypes = [C.c_void_p]
     assert lib.recon_mvs_hip_last_error(None) == b"null context"
+
+
+def test_select_source_views_host_entry_matches_reference_golden():
+    """recon_mvs_select_source_views is host-only (no GPU needed)."""
+    from tests.helpers import cams_from_array, load
+    g = load("select_views.npz")
+    cams = cams_from_array(g["cams"])
+    lib = C.CDLL(N.HIP_LIB)
+    lib.recon_mvs_select_source_views.argtypes = [C.POINTER(N.Camera), C.c_int32,
+                                                  C.POINTER(N.Camera), C.POINTER(N.Config),
+                                                  C.c_void_p, C.c_int32]
+    cfg = N.config_default()
+    cfg.num_views = 3
+    arr = (N.Camera * len(cams))(*cams)
+    import numpy as np
+    out = np.zeros(8, np.int32)
+    n = lib.recon_mvs_select_source_views(C.byref(cams[0]), len(cams), arr, C.byref(cfg),
+                                          out.ctypes.data, 8)
+    assert out[:n].tolist() == g["selected"].tolist()
+
+
+def test_cpp_caller_links_against_library():
+    """A program written against include/recon/mvs.h builds and links."""
+    binp = os.path.join(ROOT, "_build", "test_mvs_api")
+    assert os.path.exists(binp)
+    out = subprocess.run(["ldd", binp], capture_output=True, text=True).stdout
+    assert "librecon_mvs_hip.so" in out and "not found" not in out
