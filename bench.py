@@ -41,6 +41,7 @@ def parse():
     ap.add_argument("--height", type=int, default=1440)
     ap.add_argument("--iterations", type=int, default=5)
     ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16"])
+    ap.add_argument("--tile-schedule", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target duration of the CPU-baseline sample (0 = skip)")
     return ap.parse_args()
@@ -107,7 +108,8 @@ def main():
     nv = sc.num_views
     views = list(range(nv))
     p = plane.default_params(patch_size=11, window_stride=2, top_k=4, iterations=args.iterations,
-                             min_depth_m=sc.min_depth, max_depth_m=sc.max_depth, value_scale=1.0)
+                             min_depth_m=sc.min_depth, max_depth_m=sc.max_depth, value_scale=1.0,
+                             tile_schedule=args.tile_schedule)
     pm = plane.PlaneMVS(sc.cams, sc.images, storage=args.storage)
 
     def step():

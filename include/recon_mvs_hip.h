@@ -260,9 +260,9 @@ typedef struct recon_mvs_plane_params {
   int32_t top_k;             /* views aggregated per hypothesis; 0 = all */
   int32_t iterations;        /* red-black iterations (red + black half-sweep) */
   uint32_t seed;             /* counter-RNG seed */
-  int32_t propagation_reach; /* odd steps scanned per direction (1 = 4-neighbour) */
+  int32_t propagation_reach; /* odd steps scanned per direction, 1..16 (1 = 4-neighbour) */
   int32_t refine_iterations; /* golden-section steps of recon_mvs_hip_plane_refine */
-  int32_t reserved0;
+  int32_t tile_schedule;     /* workgroup tile: 0 = 64x8, 1 = 32x16, 2 = 128x4 px */
   double min_depth_m, max_depth_m;
   double depth_perturb;      /* relative inverse-depth radius at iteration 0 */
   double normal_perturb;     /* normal perturbation radius at iteration 0 */

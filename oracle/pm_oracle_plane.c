@@ -112,7 +112,7 @@ static void view_pair(const recon_mvs_camera* R, const recon_mvs_camera* S, doub
 typedef struct {
   int32_t patch_size, window_stride, top_k, iterations;
   uint32_t seed;
-  int32_t propagation_reach, refine_iterations, reserved0;
+  int32_t propagation_reach, refine_iterations, tile_schedule;
   double min_depth_m, max_depth_m, depth_perturb, normal_perturb, photo_threshold, value_scale;
 } orc_plane_params;
 

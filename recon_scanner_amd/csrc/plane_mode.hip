@@ -35,8 +35,9 @@
 
 constexpr int kMaxViews = 16;
 constexpr int kMaxTopK = 16;
-constexpr int kTileW = 32;   // pixels per tile row (16 of each colour)
-constexpr int kTileH = 16;
+// Tile schedules (pixels per workgroup tile, 256 threads): 0 = 64 x 8 (the
+// default), 1 = 32 x 16, 2 = 128 x 4.  A wave covers 64 / (TW / 2) rows.
+constexpr int kNumSchedules = 3;
 constexpr float kCostInf = 1e30f;
 
 struct ResidentView {
@@ -112,48 +113,87 @@ __device__ __forceinline__ float Pix(const void* base, int i) {
   }
 }
 
-// Reference window of one pixel: S values shifted by their mean, a bit mask of
-// the samples inside the image, and the sums over those samples.
+// Pixels i and i + 1 of one row in one load (8 bytes for FP32, 4 for FP16 /
+// BF16; unaligned global access is enabled on gfx950 under HSA).
+// Pixels at byte offset `off` and the next one, through a raw buffer
+// descriptor (32-bit offsets; `soff` is a wave-uniform extra byte offset).
+template <int FMT>
+__device__ __forceinline__ float2 BufPair(__amdgpu_buffer_rsrc_t rsrc, int off, int soff) {
+  if constexpr (FMT == RECON_MVS_PIXEL_F32) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b64(rsrc, off, soff, 0);
+    return make_float2(__uint_as_float(v[0]), __uint_as_float(v[1]));
+  } else {
+    const uint32_t u = __builtin_amdgcn_raw_buffer_load_b32(rsrc, off, soff, 0);
+    if constexpr (FMT == RECON_MVS_PIXEL_F16) {
+      const _Float16 lo = __builtin_bit_cast(_Float16, static_cast<uint16_t>(u & 0xFFFFu));
+      const _Float16 hi = __builtin_bit_cast(_Float16, static_cast<uint16_t>(u >> 16));
+      return make_float2(static_cast<float>(lo), static_cast<float>(hi));
+    } else {
+      return make_float2(__uint_as_float(u << 16), __uint_as_float(u & 0xFFFF0000u));
+    }
+  }
+}
+
+template <int FMT>
+__device__ __forceinline__ float2 PixPair(const void* base, int i) {
+  if constexpr (FMT == RECON_MVS_PIXEL_F32) {
+    float2 v;
+    __builtin_memcpy(&v, static_cast<const float*>(base) + i, 8);
+    return v;
+  } else {
+    uint32_t u;
+    __builtin_memcpy(&u, static_cast<const uint16_t*>(base) + i, 4);
+    if constexpr (FMT == RECON_MVS_PIXEL_F16) {
+      const _Float16 lo = __builtin_bit_cast(_Float16, static_cast<uint16_t>(u & 0xFFFFu));
+      const _Float16 hi = __builtin_bit_cast(_Float16, static_cast<uint16_t>(u >> 16));
+      return make_float2(static_cast<float>(lo), static_cast<float>(hi));
+    } else {
+      return make_float2(__uint_as_float(u << 16), __uint_as_float(u & 0xFFFF0000u));
+    }
+  }
+}
+
+// Reference window of one pixel: a pointer to its top-left sample in LDS
+// (row step rs floats, column step STRIDE), NaN marking samples outside the
+// image, plus the mean and the centred sums over the samples inside.
 template <int HALF, int STRIDE>
 struct RefWindow {
   static constexpr int NS = 2 * HALF / STRIDE + 1;
   static constexpr int S = NS * NS;
-  static constexpr int NW = (S + 63) / 64;
-  float r[S];
-  uint64_t mask[NW];
-  __device__ __forceinline__ bool bit(int k) const { return (mask[k >> 6] >> (k & 63)) & 1ull; }
-  float sum_r, sum_r2, n;
-  float mean;
+  const float* base;  // LDS values are pre-shifted by `shift`
+  int rs;
+  float sum_r, sum_r2, n, mean, shift;
   bool full;
+  __device__ __forceinline__ float at(int i, int j) const { return base[j * rs + i * STRIDE]; }
 };
 
-template <int HALF, int STRIDE, typename Fetch>
-__device__ __forceinline__ void MakeRefWindow(RefWindow<HALF, STRIDE>& win, Fetch fetch) {
+template <int HALF, int STRIDE>
+__device__ __forceinline__ void MakeRefWindow(RefWindow<HALF, STRIDE>& win, const float* base, int rs,
+                                              float shift) {
   constexpr int NS = RefWindow<HALF, STRIDE>::NS;
+  win.base = base;
+  win.rs = rs;
+  win.shift = shift;
   float s = 0.0f, cnt = 0.0f;
-#pragma unroll
-  for (int k = 0; k < RefWindow<HALF, STRIDE>::NW; ++k) win.mask[k] = 0;
-#pragma unroll
   for (int j = 0; j < NS; ++j) {
 #pragma unroll
     for (int i = 0; i < NS; ++i) {
-      const float v = fetch(-HALF + i * STRIDE, -HALF + j * STRIDE);  // NaN outside
+      const float v = win.at(i, j);
       const bool ok = (v == v);
-      win.r[j * NS + i] = ok ? v : 0.0f;
-      if (ok) win.mask[(j * NS + i) >> 6] |= (1ull << ((j * NS + i) & 63));
       s += ok ? v : 0.0f;
       cnt += ok ? 1.0f : 0.0f;
     }
   }
   const float mean = cnt > 0.0f ? s / cnt : 0.0f;
   float sr = 0.0f, sr2 = 0.0f;
+  for (int j = 0; j < NS; ++j) {
 #pragma unroll
-  for (int k = 0; k < RefWindow<HALF, STRIDE>::S; ++k) {
-    const bool ok = win.bit(k);
-    const float v = ok ? win.r[k] - mean : 0.0f;
-    win.r[k] = v;
-    sr += v;
-    sr2 = fmaf(v, v, sr2);
+    for (int i = 0; i < NS; ++i) {
+      const float v0 = win.at(i, j);
+      const float v = (v0 == v0) ? v0 : 0.0f;
+      sr += v;
+      sr2 = fmaf(v, v, sr2);
+    }
   }
   win.sum_r = sr;
   win.sum_r2 = sr2;
@@ -165,6 +205,8 @@ __device__ __forceinline__ void MakeRefWindow(RefWindow<HALF, STRIDE>& win, Fetc
 struct Hyp {
   float d, nx, ny, nz;
 };
+
+constexpr int kMaxTopKNet = 8;  // top_k below the view count is limited to 8
 
 // Aggregated plane cost of hypothesis h at pixel (x, y).  Returns kCostInf
 // when no view is valid or the hypothesis is rejected.  view_ncc (optional)
@@ -185,9 +227,11 @@ __device__ float PlaneCost(const PlaneArgs& a, int x, int y, const RefWindow<HAL
   const double rx = (qx - a.cx) / a.fx, ry = (qy - a.cy) / a.fy;
   const double nx = h.nx, ny = h.ny, nz = h.nz, D = h.d;
   const double nd = nx * rx + ny * ry + nz;
-  float topk[kMaxTopK];
+  const bool use_topk = a.top_k < a.nsrc;
+  float topk[kMaxTopKNet];
 #pragma unroll
-  for (int k = 0; k < kMaxTopK; ++k) topk[k] = kCostInf;
+  for (int k = 0; k < kMaxTopKNet; ++k) topk[k] = kCostInf;
+  float csum = 0.0f;
   int nvalid = 0;
   if (nd < 0.0) {
     const double rho = D * nd;
@@ -222,10 +266,19 @@ __device__ float PlaneCost(const PlaneArgs& a, int x, int y, const RefWindow<HAL
           const int w = a.src_w[v], hgt = a.src_h[v], pitch = a.src_pitch[v];
           const float lox = static_cast<float>(-iu), hix = static_cast<float>(w - 1 - iu);
           const float loy = static_cast<float>(-iv), hiy = static_cast<float>(hgt - 1 - iv);
-          const void* img = a.src_img[v];
+          constexpr int kEsz = FMT == RECON_MVS_PIXEL_F32 ? 4 : 2;
+          const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+              const_cast<void*>(a.src_img[v]), 0, pitch * (hgt + 1) * kEsz, 0x00020000);
+          const int row_bytes = pitch * kEsz;
+          // the window origin's row/column in the source, as a float index
+          // offset (exact: image sizes stay below 2^24 pixels)
+          const float fpitch = static_cast<float>(pitch);
+          const float obase = static_cast<float>(iv) * fpitch + static_cast<float>(iu);
+          const float shift = win.shift;
           // The window is inside the image when its four corners are (the
           // homography keeps the convex window convex while the denominator
-          // stays positive on it).
+          // stays positive on it); a 1e-3 px margin keeps rounded interior
+          // samples inside too.
           bool fast = win.full;
           if (fast) {
 #pragma unroll
@@ -237,49 +290,67 @@ __device__ float PlaneCost(const PlaneArgs& a, int x, int y, const RefWindow<HAL
               const float r = __builtin_amdgcn_rcpf(den);
               const float tx = fmaf(fmaf(-ucf, dh2, dh0), r, fu);
               const float ty = fmaf(fmaf(-vcf, dh2, dh1), r, fv);
-              // 1e-3 px margin: rounding of interior samples stays inside
               fast = fast && den > 0.0f && tx >= lox + 1e-3f && tx <= hix - 1e-3f &&
                      ty >= loy + 1e-3f && ty <= hiy - 1e-3f;
             }
           }
           float ss = 0.0f, sss = 0.0f, srs = 0.0f, sr = win.sum_r, srr = win.sum_r2, cnt = win.n;
-          const float mu = win.mean;
-          if (fast) {
+          // Every lane runs the branch-free loop (indices clamped into the
+          // image so lanes that need the checked path still read in
+          // bounds); those lanes then redo the window with per-sample checks.
+          // Each window row is processed in three phases (addresses, all
+          // loads, arithmetic) so the wave keeps 2 x NS loads in flight.
+          const int max_idx = (hgt - 1) * pitch + (w - 1);
+#pragma unroll 1
+          for (int j = 0; j < NS; ++j) {
+            const float dy = static_cast<float>(-HALF + j * STRIDE);
+            const float b0 = h01 * dy, b1 = h11 * dy, b2 = h21 * dy;
+            int idx[NS];
+            float fxs[NS], fys[NS];
 #pragma unroll
-            for (int j = 0; j < NS; ++j) {
-              const float dy = static_cast<float>(-HALF + j * STRIDE);
-              const float b0 = h01 * dy, b1 = h11 * dy, b2 = h21 * dy;
-#pragma unroll
-              for (int i = 0; i < NS; ++i) {
-                const float dx = static_cast<float>(-HALF + i * STRIDE);
-                const float dh0 = fmaf(h00, dx, b0), dh1 = fmaf(h10, dx, b1);
-                const float dh2 = fmaf(h20, dx, b2);
-                const float r = __builtin_amdgcn_rcpf(1.0f + dh2);
-                const float tx = fmaf(fmaf(-ucf, dh2, dh0), r, fu);
-                const float ty = fmaf(fmaf(-vcf, dh2, dh1), r, fv);
-                const float flx = floorf(tx), fly = floorf(ty);
-                const float fx = tx - flx, fy = ty - fly;
-                const int idx = (iv + static_cast<int>(fly)) * pitch + iu + static_cast<int>(flx);
-                const float p00 = Pix<FMT>(img, idx), p10 = Pix<FMT>(img, idx + 1);
-                const float p01 = Pix<FMT>(img, idx + pitch), p11 = Pix<FMT>(img, idx + pitch + 1);
-                const float top = fmaf(fx, p10 - p00, p00);
-                const float bot = fmaf(fx, p11 - p01, p01);
-                const float s = fmaf(fy, bot - top, top) - mu;
-                const float rr = win.r[j * NS + i];
-                ss += s;
-                sss = fmaf(s, s, sss);
-                srs = fmaf(rr, s, srs);
-              }
+            for (int i = 0; i < NS; ++i) {
+              const float dx = static_cast<float>(-HALF + i * STRIDE);
+              const float dh0 = fmaf(h00, dx, b0), dh1 = fmaf(h10, dx, b1);
+              const float dh2 = fmaf(h20, dx, b2);
+              const float r = __builtin_amdgcn_rcpf(1.0f + dh2);
+              const float tx = fmaf(fmaf(-ucf, dh2, dh0), r, fu);
+              const float ty = fmaf(fmaf(-vcf, dh2, dh1), r, fv);
+              const float flx = floorf(tx), fly = floorf(ty);
+              fxs[i] = tx - flx;
+              fys[i] = ty - fly;
+              idx[i] = __builtin_amdgcn_fmed3f(fmaf(fly, fpitch, flx + obase), 0.0f,
+                                               static_cast<float>(max_idx)) ;
             }
-          } else {
+            float2 p0[NS], p1[NS];
+#pragma unroll
+            for (int i = 0; i < NS; ++i) {
+              p0[i] = BufPair<FMT>(rsrc, idx[i] * kEsz, 0);
+              p1[i] = BufPair<FMT>(rsrc, idx[i] * kEsz, row_bytes);
+            }
+#pragma unroll
+            for (int i = 0; i < NS; ++i) {
+              const float top = fmaf(fxs[i], p0[i].y - p0[i].x, p0[i].x);
+              const float bot = fmaf(fxs[i], p1[i].y - p1[i].x, p1[i].x);
+              const float s = fmaf(fys[i], bot - top, top) - shift;
+              const float rr = win.at(i, j);
+              ss += s;
+              sss = fmaf(s, s, sss);
+              srs = fmaf(rr, s, srs);
+            }
+          }
+          if (!fast) {
+            // windows touching the image border or leaving it (rare)
+            ss = 0.0f;
+            sss = 0.0f;
+            srs = 0.0f;
             sr = 0.0f;
             srr = 0.0f;
             cnt = 0.0f;
-#pragma unroll
+#pragma unroll 1
             for (int j = 0; j < NS; ++j) {
               const float dy = static_cast<float>(-HALF + j * STRIDE);
               const float b0 = h01 * dy, b1 = h11 * dy, b2 = h21 * dy;
-#pragma unroll
+#pragma unroll 1
               for (int i = 0; i < NS; ++i) {
                 const float dx = static_cast<float>(-HALF + i * STRIDE);
                 const float dh0 = fmaf(h00, dx, b0), dh1 = fmaf(h10, dx, b1);
@@ -288,24 +359,25 @@ __device__ float PlaneCost(const PlaneArgs& a, int x, int y, const RefWindow<HAL
                 const float r = __builtin_amdgcn_rcpf(den);
                 const float tx = fmaf(fmaf(-ucf, dh2, dh0), r, fu);
                 const float ty = fmaf(fmaf(-vcf, dh2, dh1), r, fv);
-                const bool ok = win.bit(j * NS + i) && den > 0.0f && tx >= lox &&
-                                tx <= hix && ty >= loy && ty <= hiy;
-                const float flx = floorf(tx), fly = floorf(ty);
-                const float fx = tx - flx, fy = ty - fly;
-                const int idx =
-                    ok ? (iv + static_cast<int>(fly)) * pitch + iu + static_cast<int>(flx) : 0;
-                const float p00 = Pix<FMT>(img, idx), p10 = Pix<FMT>(img, idx + 1);
-                const float p01 = Pix<FMT>(img, idx + pitch), p11 = Pix<FMT>(img, idx + pitch + 1);
-                const float top = fmaf(fx, p10 - p00, p00);
-                const float bot = fmaf(fx, p11 - p01, p01);
-                const float s = ok ? fmaf(fy, bot - top, top) - mu : 0.0f;
-                const float rr = ok ? win.r[j * NS + i] : 0.0f;
-                ss += s;
-                sss = fmaf(s, s, sss);
-                srs = fmaf(rr, s, srs);
-                sr += rr;
-                srr = fmaf(rr, rr, srr);
-                cnt += ok ? 1.0f : 0.0f;
+                const float rv = win.at(i, j);
+                const bool ok = (rv == rv) && den > 0.0f && tx >= lox && tx <= hix && ty >= loy &&
+                                ty <= hiy;
+                if (ok) {
+                  const float flx = floorf(tx), fly = floorf(ty);
+                  const float fx = tx - flx, fy = ty - fly;
+                  const int idx = static_cast<int>(fmaf(fly, fpitch, flx + obase));
+                  const float2 p0 = BufPair<FMT>(rsrc, idx * kEsz, 0);
+                  const float2 p1 = BufPair<FMT>(rsrc, idx * kEsz, row_bytes);
+                  const float top = fmaf(fx, p0.y - p0.x, p0.x);
+                  const float bot = fmaf(fx, p1.y - p1.x, p1.x);
+                  const float s = fmaf(fy, bot - top, top) - shift;
+                  ss += s;
+                  sss = fmaf(s, s, sss);
+                  srs = fmaf(rv, s, srs);
+                  sr += rv;
+                  srr = fmaf(rv, rv, srr);
+                  cnt += 1.0f;
+                }
               }
             }
           }
@@ -322,12 +394,15 @@ __device__ float PlaneCost(const PlaneArgs& a, int x, int y, const RefWindow<HAL
               ncc_v = ncc;
               float c = 1.0f - ncc;
               ++nvalid;
-              // insert into the sorted k smallest
+              csum += c;
+              if (use_topk) {
+                // insert into the sorted k smallest
 #pragma unroll
-              for (int k = 0; k < kMaxTopK; ++k) {
-                const float lo = fminf(topk[k], c);
-                c = fmaxf(topk[k], c);
-                topk[k] = lo;
+                for (int k = 0; k < kMaxTopKNet; ++k) {
+                  const float lo = fminf(topk[k], c);
+                  c = fmaxf(topk[k], c);
+                  topk[k] = lo;
+                }
               }
             }
           }
@@ -340,11 +415,16 @@ __device__ float PlaneCost(const PlaneArgs& a, int x, int y, const RefWindow<HAL
   }
   *valid_views_out = nvalid;
   if (nvalid == 0) return kCostInf;
-  const int kk = min(a.top_k, nvalid);
-  float sum = 0.0f;
+  float cost;
+  if (use_topk) {
+    const int kk = min(a.top_k, nvalid);
+    float sum = 0.0f;
 #pragma unroll
-  for (int k = 0; k < kMaxTopK; ++k) sum += (k < kk) ? topk[k] : 0.0f;
-  const float cost = sum / static_cast<float>(kk);
+    for (int k = 0; k < kMaxTopKNet; ++k) sum += (k < kk) ? topk[k] : 0.0f;
+    cost = sum / static_cast<float>(kk);
+  } else {
+    cost = csum / static_cast<float>(nvalid);
+  }
   return cost > a.photo_reject ? kCostInf : cost;
 }
 
@@ -410,40 +490,61 @@ __device__ __forceinline__ bool TileOf(const PlaneArgs& a, int* tx, int* ty) {
   return true;
 }
 
-// Loads the reference tile (+ halo) into LDS; NaN outside the image.
-template <int HALF, int FMT>
-__device__ __forceinline__ void LoadRefTile(const PlaneArgs& a, int x0, int y0,
-                                            float (*tile)[kTileW + 2 * HALF + 1]) {
-  constexpr int TW = kTileW + 2 * HALF, TH = kTileH + 2 * HALF;
-  for (int i = threadIdx.x; i < TW * TH; i += blockDim.x) {
-    const int ly = i / TW, lx = i % TW;
-    const int gx = x0 - HALF + lx, gy = y0 - HALF + ly;
-    float v = __builtin_nanf("");
-    if (gx >= 0 && gy >= 0 && gx < a.w && gy < a.h) v = Pix<FMT>(a.ref_img, gy * a.ref_pitch + gx);
-    tile[ly][lx] = v;
-  }
-}
+// Reference tile of a workgroup: TW x TH pixels plus a HALF-pixel halo in
+// LDS, every value shifted by the tile's centre pixel (NaN outside the
+// image).  256 threads; a thread owns one pixel of each colour: thread t ->
+// row t / (TW/2), the (t % (TW/2))-th pixel of the colour in that row.
+template <int HALF, int FMT, int TW, int TH>
+struct RefTile {
+  static_assert(TW * TH == 512, "256 threads, two pixels each");
+  static constexpr int LW = TW + 2 * HALF + 1;  // +1: odd row pitch against bank conflicts
+  static constexpr int LH = TH + 2 * HALF;
+  float v[LH][LW];
+  float shift;
 
-// Random init + initial cost.  Every pixel (both colours): 64 x 4 px per
-// workgroup of 256 threads in kTileW x kTileH tiles, two pixels per thread.
-template <int HALF, int STRIDE, int FMT>
+  __device__ __forceinline__ void load(const PlaneArgs& a, int x0, int y0) {
+    if (threadIdx.x == 0) {
+      const int cx = min(x0 + TW / 2, a.w - 1), cy = min(y0 + TH / 2, a.h - 1);
+      shift = Pix<FMT>(a.ref_img, cy * a.ref_pitch + cx);
+    }
+    __syncthreads();
+    const float sh = shift;
+    for (int i = threadIdx.x; i < (LW - 1) * LH; i += blockDim.x) {
+      const int ly = i / (LW - 1), lx = i % (LW - 1);
+      const int gx = x0 - HALF + lx, gy = y0 - HALF + ly;
+      float val = __builtin_nanf("");
+      if (gx >= 0 && gy >= 0 && gx < a.w && gy < a.h) val = Pix<FMT>(a.ref_img, gy * a.ref_pitch + gx) - sh;
+      v[ly][lx] = val;
+    }
+  }
+  // pixel of colour `color` owned by this thread
+  __device__ __forceinline__ void pixel(int x0, int y0, int color, int* x, int* y, int* r) const {
+    *r = threadIdx.x / (TW / 2);
+    const int j = threadIdx.x % (TW / 2);
+    *y = y0 + *r;
+    *x = x0 + 2 * j + (((x0 + *y) & 1) ^ color);
+  }
+  template <int STRIDE>
+  __device__ __forceinline__ void window(RefWindow<HALF, STRIDE>& win, int x, int x0, int r) const {
+    MakeRefWindow<HALF, STRIDE>(win, &v[r][x - x0], STRIDE * LW, shift);
+  }
+};
+
+// Random init + initial cost of every pixel (both colours).
+template <int HALF, int STRIDE, int FMT, int TW, int TH>
 __global__ __launch_bounds__(256) void k_plane_init(PlaneArgs a) {
-  __shared__ float tile[kTileH + 2 * HALF][kTileW + 2 * HALF + 1];
+  __shared__ RefTile<HALF, FMT, TW, TH> tile;
   int tx, ty;
-  const bool live = TileOf(a, &tx, &ty);
-  if (!live) return;
-  const int x0 = tx * kTileW, y0 = ty * kTileH;
-  LoadRefTile<HALF, FMT>(a, x0, y0, tile);
+  if (!TileOf(a, &tx, &ty)) return;
+  const int x0 = tx * TW, y0 = ty * TH;
+  tile.load(a, x0, y0);
   __syncthreads();
-  for (int half = 0; half < 2; ++half) {
-    const int r = threadIdx.x / 16, j = threadIdx.x % 16;
-    const int y = y0 + r;
-    const int x = x0 + 2 * j + (((x0 + y) & 1) ^ half);
+  for (int color = 0; color < 2; ++color) {
+    int x, y, r;
+    tile.pixel(x0, y0, color, &x, &y, &r);
     if (x >= a.w || y >= a.h) continue;
     RefWindow<HALF, STRIDE> win;
-    MakeRefWindow<HALF, STRIDE>(win, [&](int dx, int dy) {
-      return tile[r + HALF + dy][x - x0 + HALF + dx];
-    });
+    tile.template window<STRIDE>(win, x, x0, r);
     float ray[3];
     PixelRay(a, x, y, ray);
     const int idx = y * a.w + x;
@@ -471,23 +572,22 @@ __device__ __forceinline__ bool TransferPlane(const PlaneArgs& a, float4 p, int 
   return true;
 }
 
+constexpr int kMaxReach = 16;
+
 // One red-black half-sweep over the pixels of colour a.color.
-template <int HALF, int STRIDE, int FMT>
+template <int HALF, int STRIDE, int FMT, int TW, int TH>
 __global__ __launch_bounds__(256) void k_plane_sweep(PlaneArgs a) {
-  __shared__ float tile[kTileH + 2 * HALF][kTileW + 2 * HALF + 1];
+  __shared__ RefTile<HALF, FMT, TW, TH> tile;
   int tx, ty;
   if (!TileOf(a, &tx, &ty)) return;
-  const int x0 = tx * kTileW, y0 = ty * kTileH;
-  LoadRefTile<HALF, FMT>(a, x0, y0, tile);
+  const int x0 = tx * TW, y0 = ty * TH;
+  tile.load(a, x0, y0);
   __syncthreads();
-  const int r = threadIdx.x / 16, j = threadIdx.x % 16;
-  const int y = y0 + r;
-  const int x = x0 + 2 * j + (((x0 + y) & 1) ^ a.color);
+  int x, y, r;
+  tile.pixel(x0, y0, a.color, &x, &y, &r);
   if (x >= a.w || y >= a.h) return;
   RefWindow<HALF, STRIDE> win;
-  MakeRefWindow<HALF, STRIDE>(win, [&](int dx, int dy) {
-    return tile[r + HALF + dy][x - x0 + HALF + dx];
-  });
+  tile.template window<STRIDE>(win, x, x0, r);
   const int idx = y * a.w + x;
   float ray[3];
   PixelRay(a, x, y, ray);
@@ -499,25 +599,30 @@ __global__ __launch_bounds__(256) void k_plane_sweep(PlaneArgs a) {
   const float dscale = ldexpf(a.depth_perturb, -a.iter);
   const float nscale = ldexpf(a.normal_perturb, -a.iter);
   const float4 base = cur;
-  // propagation sources: best-cost other-colour pixel per direction
+  // propagation sources: per direction the lowest-cost other-colour pixel at
+  // odd distances 1 .. 2*reach-1 (all reads issued before any comparison)
   int src_x[4], src_y[4];
   {
     const int ddx[4] = {-1, 1, 0, 0};
     const int ddy[4] = {0, 0, -1, 1};
 #pragma unroll
     for (int d = 0; d < 4; ++d) {
+      float cs[kMaxReach];
+#pragma unroll
+      for (int s = 0; s < kMaxReach; ++s) {
+        const int nx = x + ddx[d] * (2 * s + 1), ny = y + ddy[d] * (2 * s + 1);
+        const bool in = s < a.reach && nx >= 0 && ny >= 0 && nx < a.w && ny < a.h;
+        cs[s] = in ? a.cost[ny * a.w + nx] : kCostInf;
+      }
       src_x[d] = -1;
       src_y[d] = -1;
       float best = kCostInf;
-      for (int s = 0; s < a.reach; ++s) {
-        const int dist = 2 * s + 1;
-        const int nx = x + ddx[d] * dist, ny = y + ddy[d] * dist;
-        if (nx < 0 || ny < 0 || nx >= a.w || ny >= a.h) break;
-        const float c = a.cost[ny * a.w + nx];
-        if (c < best) {
-          best = c;
-          src_x[d] = nx;
-          src_y[d] = ny;
+#pragma unroll
+      for (int s = 0; s < kMaxReach; ++s) {
+        if (cs[s] < best) {
+          best = cs[s];
+          src_x[d] = x + ddx[d] * (2 * s + 1);
+          src_y[d] = y + ddy[d] * (2 * s + 1);
         }
       }
     }
@@ -566,23 +671,20 @@ __global__ __launch_bounds__(256) void k_plane_sweep(PlaneArgs a) {
 }
 
 // Scores a given plane state (both colours).
-template <int HALF, int STRIDE, int FMT>
+template <int HALF, int STRIDE, int FMT, int TW, int TH>
 __global__ __launch_bounds__(256) void k_plane_score(PlaneArgs a) {
-  __shared__ float tile[kTileH + 2 * HALF][kTileW + 2 * HALF + 1];
+  __shared__ RefTile<HALF, FMT, TW, TH> tile;
   int tx, ty;
   if (!TileOf(a, &tx, &ty)) return;
-  const int x0 = tx * kTileW, y0 = ty * kTileH;
-  LoadRefTile<HALF, FMT>(a, x0, y0, tile);
+  const int x0 = tx * TW, y0 = ty * TH;
+  tile.load(a, x0, y0);
   __syncthreads();
-  for (int half = 0; half < 2; ++half) {
-    const int r = threadIdx.x / 16, j = threadIdx.x % 16;
-    const int y = y0 + r;
-    const int x = x0 + 2 * j + (((x0 + y) & 1) ^ half);
+  for (int color = 0; color < 2; ++color) {
+    int x, y, r;
+    tile.pixel(x0, y0, color, &x, &y, &r);
     if (x >= a.w || y >= a.h) continue;
     RefWindow<HALF, STRIDE> win;
-    MakeRefWindow<HALF, STRIDE>(win, [&](int dx, int dy) {
-      return tile[r + HALF + dy][x - x0 + HALF + dx];
-    });
+    tile.template window<STRIDE>(win, x, x0, r);
     const int idx = y * a.w + x;
     const float4 p = a.plane[idx];
     int nv = 0;
@@ -593,27 +695,24 @@ __global__ __launch_bounds__(256) void k_plane_score(PlaneArgs a) {
 }
 
 // Sub-pixel refinement along each pixel's ray with its normal fixed.
-template <int HALF, int STRIDE, int FMT>
+template <int HALF, int STRIDE, int FMT, int TW, int TH>
 __global__ __launch_bounds__(256) void k_plane_refine(PlaneArgs a, float inv_per_px, int iterations) {
-  __shared__ float tile[kTileH + 2 * HALF][kTileW + 2 * HALF + 1];
+  __shared__ RefTile<HALF, FMT, TW, TH> tile;
   int tx, ty;
   if (!TileOf(a, &tx, &ty)) return;
-  const int x0 = tx * kTileW, y0 = ty * kTileH;
-  LoadRefTile<HALF, FMT>(a, x0, y0, tile);
+  const int x0 = tx * TW, y0 = ty * TH;
+  tile.load(a, x0, y0);
   __syncthreads();
-  for (int half = 0; half < 2; ++half) {
-    const int r = threadIdx.x / 16, j = threadIdx.x % 16;
-    const int y = y0 + r;
-    const int x = x0 + 2 * j + (((x0 + y) & 1) ^ half);
+  for (int color = 0; color < 2; ++color) {
+    int x, y, r;
+    tile.pixel(x0, y0, color, &x, &y, &r);
     if (x >= a.w || y >= a.h) continue;
     const int idx = y * a.w + x;
     const float4 p = a.plane[idx];
     float best_c = a.cost[idx];
     if (!(best_c < kCostInf)) continue;
     RefWindow<HALF, STRIDE> win;
-    MakeRefWindow<HALF, STRIDE>(win, [&](int dx, int dy) {
-      return tile[r + HALF + dy][x - x0 + HALF + dx];
-    });
+    tile.template window<STRIDE>(win, x, x0, r);
     int best_s = a.support[idx];
     const float inv0 = 1.0f / p.x;
     float best_i = inv0;
@@ -681,12 +780,14 @@ __global__ __launch_bounds__(256) void k_plane_refine(PlaneArgs a, float inv_per
   }
 }
 
-// Cost of explicit hypotheses (parity tests): one thread per query, reference
-// window read from global memory.
+// Cost of explicit hypotheses (parity tests): one thread per query, 64 per
+// workgroup; each thread copies its reference window into LDS first.
 template <int HALF, int STRIDE, int FMT>
-__global__ void k_plane_eval(PlaneArgs a, int n, const float* __restrict__ hyps,
-                             float* __restrict__ cost, int* __restrict__ nvalid,
-                             float* __restrict__ view_ncc) {
+__global__ __launch_bounds__(64) void k_plane_eval(PlaneArgs a, int n, const float* __restrict__ hyps,
+                                                   float* __restrict__ cost, int* __restrict__ nvalid,
+                                                   float* __restrict__ view_ncc) {
+  constexpr int WS = 2 * HALF + 1;
+  __shared__ float wbuf[64][WS * WS];
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n) return;
   const float* hq = hyps + 6 * q;
@@ -698,12 +799,17 @@ __global__ void k_plane_eval(PlaneArgs a, int n, const float* __restrict__ hyps,
       for (int v = 0; v < a.nsrc; ++v) view_ncc[q * a.nsrc + v] = __builtin_nanf("");
     return;
   }
+  float* mine = wbuf[threadIdx.x];
+  const float sh = Pix<FMT>(a.ref_img, y * a.ref_pitch + x);
+  for (int dy = -HALF; dy <= HALF; ++dy)
+    for (int dx = -HALF; dx <= HALF; ++dx) {
+      const int gx = x + dx, gy = y + dy;
+      mine[(dy + HALF) * WS + dx + HALF] = (gx < 0 || gy < 0 || gx >= a.w || gy >= a.h)
+                                               ? __builtin_nanf("")
+                                               : Pix<FMT>(a.ref_img, gy * a.ref_pitch + gx) - sh;
+    }
   RefWindow<HALF, STRIDE> win;
-  MakeRefWindow<HALF, STRIDE>(win, [&](int dx, int dy) {
-    const int gx = x + dx, gy = y + dy;
-    if (gx < 0 || gy < 0 || gx >= a.w || gy >= a.h) return __builtin_nanf("");
-    return Pix<FMT>(a.ref_img, gy * a.ref_pitch + gx);
-  });
+  MakeRefWindow<HALF, STRIDE>(win, mine, STRIDE * WS, sh);
   float vn[kMaxViews];
   int nv = 0;
   const float c = PlaneCost<HALF, STRIDE, FMT>(a, x, y, win, Hyp{hq[2], hq[3], hq[4], hq[5]}, &nv,
@@ -826,10 +932,15 @@ int32_t CheckParams(recon_mvs_hip_context* ctx, const recon_mvs_plane_params* p)
   if (!p) return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "null plane params");
   if (!(p->min_depth_m > 0.0 && p->max_depth_m > p->min_depth_m))
     return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "bad depth range");
-  if (p->top_k < 0 || p->top_k > kMaxTopK) return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "top_k out of [0, 16]");
+  if (p->top_k < 0 || p->top_k > kMaxTopKNet)
+    return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "top_k out of [0, 8] (0 = all views)");
   if (p->iterations < 0 || p->propagation_reach < 1 || p->propagation_reach > 64)
     return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "bad iterations / propagation_reach");
   if (!(p->value_scale > 0.0)) return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "value_scale must be > 0");
+  if (p->tile_schedule < 0 || p->tile_schedule >= kNumSchedules)
+    return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "tile_schedule out of range");
+  if (p->propagation_reach > kMaxReach)
+    return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "propagation_reach above 16");
   return RECON_MVS_OK;
 }
 
@@ -910,10 +1021,6 @@ int32_t MakeArgs(recon_mvs_hip_context* ctx, int ref, int nsrc, const int32_t* s
   a->reach = p->propagation_reach;
   a->depth_perturb = static_cast<float>(p->depth_perturb);
   a->normal_perturb = static_cast<float>(p->normal_perturb);
-  a->tiles_x = (a->w + kTileW - 1) / kTileW;
-  a->tiles_y = (a->h + kTileH - 1) / kTileH;
-  a->num_tiles = a->tiles_x * a->tiles_y;
-  a->tiles_per_xcd = (a->num_tiles + 7) / 8;
   return RECON_MVS_OK;
 }
 
@@ -940,37 +1047,59 @@ int32_t EnsureState(recon_mvs_hip_context* ctx, int view) {
 
 enum KernelId { kSweep = 0, kInit = 1, kRefine = 2, kGeo = 3, kScore = 4, kEval = 5 };
 
-template <int HALF, int STRIDE, int FMT>
-void LaunchT(KernelId id, const PlaneArgs& a, hipStream_t s, float f0, int i0, int n,
-             const float* hyps, float* cost, int* nvalid, float* view_ncc) {
+template <int HALF, int STRIDE, int FMT, int TW, int TH>
+void LaunchTile(KernelId id, PlaneArgs a, hipStream_t s, float f0, int i0, int n, const float* hyps,
+                float* cost, int* nvalid, float* view_ncc) {
+  a.tiles_x = (a.w + TW - 1) / TW;
+  a.tiles_y = (a.h + TH - 1) / TH;
+  a.num_tiles = a.tiles_x * a.tiles_y;
+  a.tiles_per_xcd = (a.num_tiles + 7) / 8;
   const dim3 grid(a.tiles_per_xcd * 8), block(256);
   switch (id) {
-    case kSweep: hipLaunchKernelGGL((k_plane_sweep<HALF, STRIDE, FMT>), grid, block, 0, s, a); break;
-    case kInit: hipLaunchKernelGGL((k_plane_init<HALF, STRIDE, FMT>), grid, block, 0, s, a); break;
-    case kScore: hipLaunchKernelGGL((k_plane_score<HALF, STRIDE, FMT>), grid, block, 0, s, a); break;
+    case kSweep: hipLaunchKernelGGL((k_plane_sweep<HALF, STRIDE, FMT, TW, TH>), grid, block, 0, s, a); break;
+    case kInit: hipLaunchKernelGGL((k_plane_init<HALF, STRIDE, FMT, TW, TH>), grid, block, 0, s, a); break;
+    case kScore: hipLaunchKernelGGL((k_plane_score<HALF, STRIDE, FMT, TW, TH>), grid, block, 0, s, a); break;
     case kRefine:
-      hipLaunchKernelGGL((k_plane_refine<HALF, STRIDE, FMT>), grid, block, 0, s, a, f0, i0);
+      hipLaunchKernelGGL((k_plane_refine<HALF, STRIDE, FMT, TW, TH>), grid, block, 0, s, a, f0, i0);
       break;
     case kEval:
-      hipLaunchKernelGGL((k_plane_eval<HALF, STRIDE, FMT>), dim3((n + 127) / 128), dim3(128), 0, s, a,
+      hipLaunchKernelGGL((k_plane_eval<HALF, STRIDE, FMT>), dim3((n + 63) / 64), dim3(64), 0, s, a,
                          n, hyps, cost, nvalid, view_ncc);
       break;
     default: break;
   }
 }
 
+template <int HALF, int STRIDE, int FMT>
+bool LaunchT(int schedule, KernelId id, const PlaneArgs& a, hipStream_t s, float f0, int i0, int n,
+             const float* hyps, float* cost, int* nvalid, float* view_ncc) {
+  // the 11x11 stride-2 window is compiled for every tile schedule, the others
+  // for the default one
+  constexpr bool kAllSchedules = (HALF == 5 && STRIDE == 2);
+  switch (schedule) {
+    case 0: LaunchTile<HALF, STRIDE, FMT, 64, 8>(id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc); return true;
+    case 1:
+      if constexpr (kAllSchedules) { LaunchTile<HALF, STRIDE, FMT, 32, 16>(id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc); return true; }
+      return false;
+    case 2:
+      if constexpr (kAllSchedules) { LaunchTile<HALF, STRIDE, FMT, 128, 4>(id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc); return true; }
+      return false;
+    default: return false;
+  }
+}
+
 template <int HALF, int STRIDE>
-bool LaunchFmt(int fmt, KernelId id, const PlaneArgs& a, hipStream_t s, float f0, int i0, int n,
-               const float* hyps, float* cost, int* nvalid, float* view_ncc) {
+bool LaunchFmt(int fmt, int schedule, KernelId id, const PlaneArgs& a, hipStream_t s, float f0, int i0,
+               int n, const float* hyps, float* cost, int* nvalid, float* view_ncc) {
   // FP16 / BF16 storage is compiled for the 11x11 stride-2 window only
   constexpr bool kAllFormats = (HALF == 5 && STRIDE == 2);
   switch (fmt) {
-    case RECON_MVS_PIXEL_F32: LaunchT<HALF, STRIDE, RECON_MVS_PIXEL_F32>(id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc); return true;
+    case RECON_MVS_PIXEL_F32: return LaunchT<HALF, STRIDE, RECON_MVS_PIXEL_F32>(schedule, id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc);
     case RECON_MVS_PIXEL_F16:
-      if constexpr (kAllFormats) { LaunchT<HALF, STRIDE, RECON_MVS_PIXEL_F16>(id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc); return true; }
+      if constexpr (kAllFormats) return LaunchT<HALF, STRIDE, RECON_MVS_PIXEL_F16>(schedule, id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc);
       return false;
     case RECON_MVS_PIXEL_BF16:
-      if constexpr (kAllFormats) { LaunchT<HALF, STRIDE, RECON_MVS_PIXEL_BF16>(id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc); return true; }
+      if constexpr (kAllFormats) return LaunchT<HALF, STRIDE, RECON_MVS_PIXEL_BF16>(schedule, id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc);
       return false;
     default: return false;
   }
@@ -982,17 +1111,22 @@ int32_t Launch(recon_mvs_hip_context* ctx, const recon_mvs_plane_params* p, Kern
                const float* hyps = nullptr, float* cost = nullptr, int* nvalid = nullptr,
                float* view_ncc = nullptr) {
   const int fmt = ctx->storage_format;
+  const int sch = p->tile_schedule;
   hipEvent_t eb, ee;
   TimerBegin(ctx, static_cast<int>(id), &eb, &ee);
   bool ok = false;
   const int ps = p->patch_size, st = p->window_stride;
-  if (ps == 11 && st == 2) ok = LaunchFmt<5, 2>(fmt, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc);
-  else if (ps == 11 && st == 1) ok = LaunchFmt<5, 1>(fmt, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc);
-  else if (ps == 7 && st == 1) ok = LaunchFmt<3, 1>(fmt, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc);
-  else if (ps == 5 && st == 1) ok = LaunchFmt<2, 1>(fmt, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc);
-  else if (ps == 7 && st == 2) ok = LaunchFmt<3, 2>(fmt, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc);
-  else if (ps == 9 && st == 2) ok = LaunchFmt<4, 2>(fmt, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc);
-  if (!ok) return Fail(ctx, RECON_MVS_ERR_NOT_IMPLEMENTED, "window (patch_size, window_stride) / storage format not compiled in");
+  if (ps == 11 && st == 2) ok = LaunchFmt<5, 2>(fmt, sch, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc);
+#ifndef RMVS_BENCH_WINDOW_ONLY  // (development builds compile the 11x11 stride-2 window only)
+  else if (ps == 11 && st == 1) ok = LaunchFmt<5, 1>(fmt, sch, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc);
+  else if (ps == 7 && st == 1) ok = LaunchFmt<3, 1>(fmt, sch, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc);
+  else if (ps == 5 && st == 1) ok = LaunchFmt<2, 1>(fmt, sch, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc);
+  else if (ps == 7 && st == 2) ok = LaunchFmt<3, 2>(fmt, sch, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc);
+  else if (ps == 9 && st == 2) ok = LaunchFmt<4, 2>(fmt, sch, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc);
+#endif
+  if (!ok)
+    return Fail(ctx, RECON_MVS_ERR_NOT_IMPLEMENTED,
+                "window (patch_size, window_stride) / storage format / tile schedule not compiled in");
   TimerEnd(ctx, static_cast<int>(id), eb, ee);
   RMVS_LAUNCH_CHECK(ctx);
   return RECON_MVS_OK;

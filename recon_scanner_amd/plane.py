@@ -26,7 +26,7 @@ KERNEL_CLASSES = ("sweep", "init", "refine", "consistency", "score", "eval")
 class PlaneParams(C.Structure):
     _fields_ = [("patch_size", C.c_int32), ("window_stride", C.c_int32), ("top_k", C.c_int32),
                 ("iterations", C.c_int32), ("seed", C.c_uint32), ("propagation_reach", C.c_int32),
-                ("refine_iterations", C.c_int32), ("reserved0", C.c_int32),
+                ("refine_iterations", C.c_int32), ("tile_schedule", C.c_int32),
                 ("min_depth_m", C.c_double), ("max_depth_m", C.c_double),
                 ("depth_perturb", C.c_double), ("normal_perturb", C.c_double),
                 ("photo_threshold", C.c_double), ("value_scale", C.c_double)]
