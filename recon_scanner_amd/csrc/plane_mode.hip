@@ -275,74 +275,78 @@ __device__ float PlaneCost(const PlaneArgs& a, int x, int y, const RefWindow<HAL
           const float fpitch = static_cast<float>(pitch);
           const float obase = static_cast<float>(iv) * fpitch + static_cast<float>(iu);
           const float shift = win.shift;
-          // The window is inside the image when its four corners are (the
-          // homography keeps the convex window convex while the denominator
-          // stays positive on it); a 1e-3 px margin keeps rounded interior
-          // samples inside too.
-          bool fast = win.full;
-          if (fast) {
+          // Classify the warped window by its four corners (the homography
+          // keeps the convex window convex while the denominator stays
+          // positive on it): inside the image (with a 1e-3 px margin so
+          // rounded interior samples stay inside), entirely beyond one image
+          // edge (the view cannot be valid: skip it), or straddling an edge
+          // (per-sample checks).
+          bool inside = true, den_ok = true;
+          bool all_l = true, all_r = true, all_t = true, all_b = true;
 #pragma unroll
-            for (int c = 0; c < 4; ++c) {
-              const float dx = (c & 1) ? HALF : -HALF, dy = (c & 2) ? HALF : -HALF;
-              const float dh0 = fmaf(h00, dx, h01 * dy), dh1 = fmaf(h10, dx, h11 * dy);
-              const float dh2 = fmaf(h20, dx, h21 * dy);
-              const float den = 1.0f + dh2;
-              const float r = __builtin_amdgcn_rcpf(den);
-              const float tx = fmaf(fmaf(-ucf, dh2, dh0), r, fu);
-              const float ty = fmaf(fmaf(-vcf, dh2, dh1), r, fv);
-              fast = fast && den > 0.0f && tx >= lox + 1e-3f && tx <= hix - 1e-3f &&
-                     ty >= loy + 1e-3f && ty <= hiy - 1e-3f;
-            }
+          for (int c = 0; c < 4; ++c) {
+            const float dx = (c & 1) ? HALF : -HALF, dy = (c & 2) ? HALF : -HALF;
+            const float dh0 = fmaf(h00, dx, h01 * dy), dh1 = fmaf(h10, dx, h11 * dy);
+            const float dh2 = fmaf(h20, dx, h21 * dy);
+            const float den = 1.0f + dh2;
+            const float r = __builtin_amdgcn_rcpf(den);
+            const float tx = fmaf(fmaf(-ucf, dh2, dh0), r, fu);
+            const float ty = fmaf(fmaf(-vcf, dh2, dh1), r, fv);
+            den_ok = den_ok && den > 0.0f;
+            inside = inside && tx >= lox + 1e-3f && tx <= hix - 1e-3f && ty >= loy + 1e-3f &&
+                     ty <= hiy - 1e-3f;
+            all_l = all_l && tx < lox;
+            all_r = all_r && tx > hix;
+            all_t = all_t && ty < loy;
+            all_b = all_b && ty > hiy;
           }
+          inside = inside && den_ok && win.full;
+          const bool outside = den_ok && (all_l || all_r || all_t || all_b);
           float ss = 0.0f, sss = 0.0f, srs = 0.0f, sr = win.sum_r, srr = win.sum_r2, cnt = win.n;
-          // Every lane runs the branch-free loop (indices clamped into the
-          // image so lanes that need the checked path still read in
-          // bounds); those lanes then redo the window with per-sample checks.
-          // Each window row is processed in three phases (addresses, all
-          // loads, arithmetic) so the wave keeps 2 x NS loads in flight.
-          const int max_idx = (hgt - 1) * pitch + (w - 1);
+          if (outside) {
+            cnt = 0.0f;
+          } else if (inside) {
+            // Each window row in three phases (addresses, all loads,
+            // arithmetic): the wave keeps 2 x NS loads in flight.
 #pragma unroll 1
-          for (int j = 0; j < NS; ++j) {
-            const float dy = static_cast<float>(-HALF + j * STRIDE);
-            const float b0 = h01 * dy, b1 = h11 * dy, b2 = h21 * dy;
-            int idx[NS];
-            float fxs[NS], fys[NS];
+            for (int j = 0; j < NS; ++j) {
+              const float dy = static_cast<float>(-HALF + j * STRIDE);
+              const float b0 = h01 * dy, b1 = h11 * dy, b2 = h21 * dy;
+              int idx[NS];
+              float fxs[NS], fys[NS];
 #pragma unroll
-            for (int i = 0; i < NS; ++i) {
-              const float dx = static_cast<float>(-HALF + i * STRIDE);
-              const float dh0 = fmaf(h00, dx, b0), dh1 = fmaf(h10, dx, b1);
-              const float dh2 = fmaf(h20, dx, b2);
-              const float r = __builtin_amdgcn_rcpf(1.0f + dh2);
-              const float tx = fmaf(fmaf(-ucf, dh2, dh0), r, fu);
-              const float ty = fmaf(fmaf(-vcf, dh2, dh1), r, fv);
-              const float flx = floorf(tx), fly = floorf(ty);
-              fxs[i] = tx - flx;
-              fys[i] = ty - fly;
-              idx[i] = __builtin_amdgcn_fmed3f(fmaf(fly, fpitch, flx + obase), 0.0f,
-                                               static_cast<float>(max_idx)) ;
-            }
-            float2 p0[NS], p1[NS];
+              for (int i = 0; i < NS; ++i) {
+                const float dx = static_cast<float>(-HALF + i * STRIDE);
+                const float dh0 = fmaf(h00, dx, b0), dh1 = fmaf(h10, dx, b1);
+                const float dh2 = fmaf(h20, dx, b2);
+                const float r = __builtin_amdgcn_rcpf(1.0f + dh2);
+                const float tx = fmaf(fmaf(-ucf, dh2, dh0), r, fu);
+                const float ty = fmaf(fmaf(-vcf, dh2, dh1), r, fv);
+                const float flx = floorf(tx), fly = floorf(ty);
+                fxs[i] = tx - flx;
+                fys[i] = ty - fly;
+                idx[i] = static_cast<int>(fmaf(fly, fpitch, flx + obase));
+              }
+              float2 p0[NS], p1[NS];
 #pragma unroll
-            for (int i = 0; i < NS; ++i) {
-              p0[i] = BufPair<FMT>(rsrc, idx[i] * kEsz, 0);
-              p1[i] = BufPair<FMT>(rsrc, idx[i] * kEsz, row_bytes);
-            }
+              for (int i = 0; i < NS; ++i) {
+                p0[i] = BufPair<FMT>(rsrc, idx[i] * kEsz, 0);
+                p1[i] = BufPair<FMT>(rsrc, idx[i] * kEsz, row_bytes);
+              }
 #pragma unroll
-            for (int i = 0; i < NS; ++i) {
-              const float top = fmaf(fxs[i], p0[i].y - p0[i].x, p0[i].x);
-              const float bot = fmaf(fxs[i], p1[i].y - p1[i].x, p1[i].x);
-              const float s = fmaf(fys[i], bot - top, top) - shift;
-              const float rr = win.at(i, j);
-              ss += s;
-              sss = fmaf(s, s, sss);
-              srs = fmaf(rr, s, srs);
+              for (int i = 0; i < NS; ++i) {
+                const float top = fmaf(fxs[i], p0[i].y - p0[i].x, p0[i].x);
+                const float bot = fmaf(fxs[i], p1[i].y - p1[i].x, p1[i].x);
+                const float s = fmaf(fys[i], bot - top, top) - shift;
+                const float rr = win.at(i, j);
+                ss += s;
+                sss = fmaf(s, s, sss);
+                srs = fmaf(rr, s, srs);
+              }
             }
-          }
-          if (!fast) {
-            // windows touching the image border or leaving it (rare)
-            ss = 0.0f;
-            sss = 0.0f;
-            srs = 0.0f;
+          } else {
+            // straddling windows: the same three phases with per-sample
+            // checks; invalid samples read pixel 0 and are masked out
             sr = 0.0f;
             srr = 0.0f;
             cnt = 0.0f;
@@ -350,7 +354,10 @@ __device__ float PlaneCost(const PlaneArgs& a, int x, int y, const RefWindow<HAL
             for (int j = 0; j < NS; ++j) {
               const float dy = static_cast<float>(-HALF + j * STRIDE);
               const float b0 = h01 * dy, b1 = h11 * dy, b2 = h21 * dy;
-#pragma unroll 1
+              int idx[NS];
+              float fxs[NS], fys[NS];
+              bool oks[NS];
+#pragma unroll
               for (int i = 0; i < NS; ++i) {
                 const float dx = static_cast<float>(-HALF + i * STRIDE);
                 const float dh0 = fmaf(h00, dx, b0), dh1 = fmaf(h10, dx, b1);
@@ -362,22 +369,31 @@ __device__ float PlaneCost(const PlaneArgs& a, int x, int y, const RefWindow<HAL
                 const float rv = win.at(i, j);
                 const bool ok = (rv == rv) && den > 0.0f && tx >= lox && tx <= hix && ty >= loy &&
                                 ty <= hiy;
-                if (ok) {
-                  const float flx = floorf(tx), fly = floorf(ty);
-                  const float fx = tx - flx, fy = ty - fly;
-                  const int idx = static_cast<int>(fmaf(fly, fpitch, flx + obase));
-                  const float2 p0 = BufPair<FMT>(rsrc, idx * kEsz, 0);
-                  const float2 p1 = BufPair<FMT>(rsrc, idx * kEsz, row_bytes);
-                  const float top = fmaf(fx, p0.y - p0.x, p0.x);
-                  const float bot = fmaf(fx, p1.y - p1.x, p1.x);
-                  const float s = fmaf(fy, bot - top, top) - shift;
-                  ss += s;
-                  sss = fmaf(s, s, sss);
-                  srs = fmaf(rv, s, srs);
-                  sr += rv;
-                  srr = fmaf(rv, rv, srr);
-                  cnt += 1.0f;
-                }
+                const float flx = floorf(tx), fly = floorf(ty);
+                fxs[i] = tx - flx;
+                fys[i] = ty - fly;
+                oks[i] = ok;
+                idx[i] = ok ? static_cast<int>(fmaf(fly, fpitch, flx + obase)) : 0;
+              }
+              float2 p0[NS], p1[NS];
+#pragma unroll
+              for (int i = 0; i < NS; ++i) {
+                p0[i] = BufPair<FMT>(rsrc, idx[i] * kEsz, 0);
+                p1[i] = BufPair<FMT>(rsrc, idx[i] * kEsz, row_bytes);
+              }
+#pragma unroll
+              for (int i = 0; i < NS; ++i) {
+                const bool ok = oks[i];
+                const float top = fmaf(fxs[i], p0[i].y - p0[i].x, p0[i].x);
+                const float bot = fmaf(fxs[i], p1[i].y - p1[i].x, p1[i].x);
+                const float s = ok ? fmaf(fys[i], bot - top, top) - shift : 0.0f;
+                const float rr = ok ? win.at(i, j) : 0.0f;
+                ss += s;
+                sss = fmaf(s, s, sss);
+                srs = fmaf(rr, s, srs);
+                sr += rr;
+                srr = fmaf(rr, rr, srr);
+                cnt += ok ? 1.0f : 0.0f;
               }
             }
           }
