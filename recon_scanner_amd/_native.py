@@ -14,7 +14,8 @@ import numpy as np
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_DIR = os.path.join(PKG_DIR, "_lib")
-HIP_LIB = os.path.join(LIB_DIR, "librecon_mvs_hip.so")
+# RECON_MVS_HIP_LIB overrides the library path (deployments that install it elsewhere)
+HIP_LIB = os.environ.get("RECON_MVS_HIP_LIB", os.path.join(LIB_DIR, "librecon_mvs_hip.so"))
 SCENE_LIB = os.path.join(LIB_DIR, "libreconscene.so")
 
 # status codes (recon_c_api.h:135-176)

@@ -46,8 +46,8 @@ def _run(cmd):
     return r
 
 
-def _compile(src, extra):
-    out = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
+def _compile(src, extra, obj_dir=OBJ):
+    out = os.path.join(obj_dir, os.path.splitext(src)[0] + ".o")
     cmd = [HIPCC] + HIP_FLAGS + extra
     if src.endswith(".cpp"):
         cmd += ["-x", "hip"]
@@ -56,15 +56,17 @@ def _compile(src, extra):
     return out
 
 
-def build_hip(jobs=8):
-    os.makedirs(OBJ, exist_ok=True)
-    os.makedirs(LIB, exist_ok=True)
-    srcs = [(s, e) for s, e in HIP_SOURCES if os.path.exists(os.path.join(CSRC, s))]
+def build_hip(jobs=8, defines=(), out_dir=LIB, obj_dir=OBJ):
+    """Compiles every HIP source; `defines` (-D...) are added to each compile."""
+    os.makedirs(obj_dir, exist_ok=True)
+    os.makedirs(out_dir, exist_ok=True)
+    srcs = [(s, list(e) + list(defines)) for s, e in HIP_SOURCES
+            if os.path.exists(os.path.join(CSRC, s))]
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(lambda se: _compile(*se), srcs))
-    tmp = os.path.join(LIB, "librecon_mvs_hip.so.tmp")
+        objs = list(ex.map(lambda se: _compile(se[0], se[1], obj_dir), srcs))
+    tmp = os.path.join(out_dir, "librecon_mvs_hip.so.tmp")
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", tmp])
-    os.replace(tmp, os.path.join(LIB, "librecon_mvs_hip.so"))
+    os.replace(tmp, os.path.join(out_dir, "librecon_mvs_hip.so"))
 
 
 def build_scene():

@@ -275,33 +275,24 @@ __device__ float PlaneCost(const PlaneArgs& a, int x, int y, const RefWindow<HAL
           const float fpitch = static_cast<float>(pitch);
           const float obase = static_cast<float>(iv) * fpitch + static_cast<float>(iu);
           const float shift = win.shift;
-          // Classify the warped window by its four corners (the homography
-          // keeps the convex window convex while the denominator stays
-          // positive on it): inside the image (with a 1e-3 px margin so
-          // rounded interior samples stay inside), entirely beyond one image
-          // edge (the view cannot be valid: skip it), or straddling an edge
-          // (per-sample checks).
-          bool inside = true, den_ok = true;
-          bool all_l = true, all_r = true, all_t = true, all_b = true;
-#pragma unroll
-          for (int c = 0; c < 4; ++c) {
-            const float dx = (c & 1) ? HALF : -HALF, dy = (c & 2) ? HALF : -HALF;
-            const float dh0 = fmaf(h00, dx, h01 * dy), dh1 = fmaf(h10, dx, h11 * dy);
-            const float dh2 = fmaf(h20, dx, h21 * dy);
-            const float den = 1.0f + dh2;
-            const float r = __builtin_amdgcn_rcpf(den);
-            const float tx = fmaf(fmaf(-ucf, dh2, dh0), r, fu);
-            const float ty = fmaf(fmaf(-vcf, dh2, dh1), r, fv);
-            den_ok = den_ok && den > 0.0f;
-            inside = inside && tx >= lox + 1e-3f && tx <= hix - 1e-3f && ty >= loy + 1e-3f &&
-                     ty <= hiy - 1e-3f;
-            all_l = all_l && tx < lox;
-            all_r = all_r && tx > hix;
-            all_t = all_t && ty < loy;
-            all_b = all_b && ty > hiy;
-          }
-          inside = inside && den_ok && win.full;
-          const bool outside = den_ok && (all_l || all_r || all_t || all_b);
+          // Classify the warped window with a bound on its sample offsets
+          // from the centre: |dh| <= HALF (|h.0| + |h.1|) per row of H, so
+          // |du| <= (e0 + |u_c| e2) / (1 - e2) while e2 < 1.  Inside the image
+          // with a 1e-3 px margin (rounded samples stay inside), entirely
+          // beyond one edge (the view cannot be valid: skip it), or
+          // straddling an edge (per-sample checks).
+          const float e0 = HALF * (fabsf(h00) + fabsf(h01)), e1 = HALF * (fabsf(h10) + fabsf(h11));
+          const float e2 = HALF * (fabsf(h20) + fabsf(h21));
+          const float rd = __builtin_amdgcn_rcpf(1.0f - e2);
+          const float bu = fmaf(fabsf(ucf), e2, e0) * rd + 1e-3f;
+          const float bv = fmaf(fabsf(vcf), e2, e1) * rd + 1e-3f;
+          const bool den_ok = e2 < 0.5f;
+          bool inside = den_ok && win.full && fu - bu >= lox && fu + bu <= hix && fv - bv >= loy &&
+                        fv + bv <= hiy;
+          const bool outside = den_ok && (fu + bu < lox || fu - bu > hix || fv + bv < loy || fv - bv > hiy);
+#ifdef RMVS_ALWAYS_MASKED
+          inside = false;
+#endif
           float ss = 0.0f, sss = 0.0f, srs = 0.0f, sr = win.sum_r, srr = win.sum_r2, cnt = win.n;
           if (outside) {
             cnt = 0.0f;
@@ -398,7 +389,7 @@ __device__ float PlaneCost(const PlaneArgs& a, int x, int y, const RefWindow<HAL
             }
           }
           if (cnt >= static_cast<float>(S / 2) && cnt > 0.0f) {
-            const float in = 1.0f / cnt;
+            const float in = __builtin_amdgcn_rcpf(cnt);
             const float mr = sr * in, ms = ss * in;
             const float vr = fmaf(srr, in, -mr * mr);
             const float vs = fmaf(sss, in, -ms * ms);
