@@ -262,7 +262,8 @@ typedef struct recon_mvs_plane_params {
   uint32_t seed;             /* counter-RNG seed */
   int32_t propagation_reach; /* odd steps scanned per direction, 1..16 (1 = 4-neighbour) */
   int32_t refine_iterations; /* golden-section steps of recon_mvs_hip_plane_refine */
-  int32_t tile_schedule;     /* workgroup tile: 0 = 64x8, 1 = 32x16, 2 = 128x4 px */
+  int32_t tile_schedule;     /* workgroup tile: 0 = 64x8, 1 = 32x16, 2 = 128x4 px,
+                                -1 = autotuned (cached per size, views, window, format) */
   double min_depth_m, max_depth_m;
   double depth_perturb;      /* relative inverse-depth radius at iteration 0 */
   double normal_perturb;     /* normal perturbation radius at iteration 0 */
@@ -342,10 +343,51 @@ RECON_MVS_API int32_t recon_mvs_hip_eval_plane_costs(recon_mvs_hip_context* ctx,
                                                      int32_t* valid_views, float* view_ncc);
 
 /*
+ * Autotuner: times random init + two half-sweeps of ref_view under every
+ * compiled tile schedule, caches and returns the fastest (*best_schedule);
+ * ms_per_schedule (3 entries, may be NULL) receives each time (-1 where a
+ * schedule is not compiled for this window / format).  Overwrites the slot.
+ */
+RECON_MVS_API int32_t recon_mvs_hip_autotune(recon_mvs_hip_context* ctx, int32_t ref_view,
+                                             int32_t num_src, const int32_t* src_views,
+                                             const recon_mvs_plane_params* params,
+                                             int32_t* best_schedule, double* ms_per_schedule);
+
+/* One reference view of a batch and its sources (view indices of set_views). */
+typedef struct recon_mvs_batch_job {
+  int32_t ref_view;
+  int32_t num_src;
+  int32_t src_views[16];
+} recon_mvs_batch_job;
+
+typedef struct recon_mvs_hip_batch recon_mvs_hip_batch;
+
+/*
+ * Batched plane-mode PatchMatch of many reference views of any resolution
+ * (no counterpart in the reference, whose StageMvs loops over views,
+ * orchestrator.cpp:988-1040).  The work is split into (view, tile) units
+ * served from a work queue by one persistent launch per phase (random init,
+ * then every half-sweep); all phases are captured once into a HIP graph.
+ * Each job's depth map lands in slot ref_view, bit-identical to
+ * recon_mvs_hip_plane_patchmatch with the same parameters.  tile_schedule
+ * -1 autotunes on the job with the most pixel-views.
+ */
+RECON_MVS_API int32_t recon_mvs_hip_batch_create(recon_mvs_hip_context* ctx, int32_t num_jobs,
+                                                 const recon_mvs_batch_job* jobs,
+                                                 const recon_mvs_plane_params* params,
+                                                 recon_mvs_hip_batch** out);
+/* Enqueues one run of the captured graph on the context's stream. */
+RECON_MVS_API int32_t recon_mvs_hip_batch_run(recon_mvs_hip_context* ctx, recon_mvs_hip_batch* batch);
+/* Tile schedule the batch was built with. */
+RECON_MVS_API int32_t recon_mvs_hip_batch_schedule(const recon_mvs_hip_batch* batch);
+RECON_MVS_API int32_t recon_mvs_hip_batch_destroy(recon_mvs_hip_context* ctx, recon_mvs_hip_batch* batch);
+
+/*
  * Kernel timers.  enable != 0 records HIP events around every launch of the
  * plane-mode kernels on the context's stream; ms / counts (8 entries each,
  * may be NULL) receive the accumulated milliseconds and launch counts per
- * kernel class: 0 sweep, 1 init, 2 refine, 3 consistency, 4 finalize.
+ * kernel class: 0 sweep, 1 init, 2 refine, 3 consistency, 4 score, 5 eval,
+ * 6 batch (one graph run).
  * reset != 0 clears the accumulators.
  */
 RECON_MVS_API int32_t recon_mvs_hip_timing(recon_mvs_hip_context* ctx, int32_t enable,

@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -92,6 +93,8 @@ struct recon_mvs_hip_context {
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
   double timed_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   int64_t timed_count[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  // autotuned tile schedule per (size, views, window, format)
+  std::map<std::string, int> tuned_schedule;
 };
 
 namespace rmvs {

@@ -275,116 +275,96 @@ __device__ float PlaneCost(const PlaneArgs& a, int x, int y, const RefWindow<HAL
           const float fpitch = static_cast<float>(pitch);
           const float obase = static_cast<float>(iv) * fpitch + static_cast<float>(iu);
           const float shift = win.shift;
-          // Classify the warped window with a bound on its sample offsets
-          // from the centre: |dh| <= HALF (|h.0| + |h.1|) per row of H, so
-          // |du| <= (e0 + |u_c| e2) / (1 - e2) while e2 < 1.  Inside the image
-          // with a 1e-3 px margin (rounded samples stay inside), entirely
-          // beyond one edge (the view cannot be valid: skip it), or
-          // straddling an edge (per-sample checks).
+          // A window entirely beyond one image edge cannot give a valid view:
+          // skip it.  Bound on the sample offsets from the centre: |dh| <=
+          // HALF (|h.0| + |h.1|) per row of H, so |du| <= (e0 + |u_c| e2) /
+          // (1 - e2) while e2 < 1.
           const float e0 = HALF * (fabsf(h00) + fabsf(h01)), e1 = HALF * (fabsf(h10) + fabsf(h11));
           const float e2 = HALF * (fabsf(h20) + fabsf(h21));
           const float rd = __builtin_amdgcn_rcpf(1.0f - e2);
           const float bu = fmaf(fabsf(ucf), e2, e0) * rd + 1e-3f;
           const float bv = fmaf(fabsf(vcf), e2, e1) * rd + 1e-3f;
+          // e2 < 0.5 keeps every sample's denominator positive
           const bool den_ok = e2 < 0.5f;
-          bool inside = den_ok && win.full && fu - bu >= lox && fu + bu <= hix && fv - bv >= loy &&
-                        fv + bv <= hiy;
           const bool outside = den_ok && (fu + bu < lox || fu - bu > hix || fv + bv < loy || fv - bv > hiy);
-#ifdef RMVS_ALWAYS_MASKED
-          inside = false;
-#endif
-          float ss = 0.0f, sss = 0.0f, srs = 0.0f, sr = win.sum_r, srr = win.sum_r2, cnt = win.n;
-          if (outside) {
-            cnt = 0.0f;
-          } else if (inside) {
-            // Each window row in three phases (addresses, all loads,
-            // arithmetic): the wave keeps 2 x NS loads in flight.
+          // Rows of the window map to segments (projective maps keep them
+          // convex while the denominator stays positive): a row whose two end
+          // samples are inside the image with a 1e-4 px margin is inside.
+          const bool rows_checkable = den_ok && win.full;
+          const float mlox = lox + 1e-4f, mhix = hix - 1e-4f, mloy = loy + 1e-4f, mhiy = hiy - 1e-4f;
+          float ss = 0.0f, sss = 0.0f, srs = 0.0f, sr = 0.0f, srr = 0.0f, cnt = 0.0f;
+          if (!outside) {
+            // Each window row in three phases (positions, all loads,
+            // arithmetic): a wave keeps 2 x NS loads in flight.  Loads need
+            // no clamping: out-of-range buffer reads return 0 and the
+            // sample is masked out.
 #pragma unroll 1
             for (int j = 0; j < NS; ++j) {
               const float dy = static_cast<float>(-HALF + j * STRIDE);
               const float b0 = h01 * dy, b1 = h11 * dy, b2 = h21 * dy;
-              int idx[NS];
-              float fxs[NS], fys[NS];
+              float txs[NS], tys[NS], dens[NS];
 #pragma unroll
               for (int i = 0; i < NS; ++i) {
                 const float dx = static_cast<float>(-HALF + i * STRIDE);
                 const float dh0 = fmaf(h00, dx, b0), dh1 = fmaf(h10, dx, b1);
                 const float dh2 = fmaf(h20, dx, b2);
-                const float r = __builtin_amdgcn_rcpf(1.0f + dh2);
-                const float tx = fmaf(fmaf(-ucf, dh2, dh0), r, fu);
-                const float ty = fmaf(fmaf(-vcf, dh2, dh1), r, fv);
-                const float flx = floorf(tx), fly = floorf(ty);
-                fxs[i] = tx - flx;
-                fys[i] = ty - fly;
-                idx[i] = static_cast<int>(fmaf(fly, fpitch, flx + obase));
+                dens[i] = 1.0f + dh2;
+                const float r = __builtin_amdgcn_rcpf(dens[i]);
+                txs[i] = fmaf(fmaf(-ucf, dh2, dh0), r, fu);
+                tys[i] = fmaf(fmaf(-vcf, dh2, dh1), r, fv);
               }
-              float2 p0[NS], p1[NS];
-#pragma unroll
-              for (int i = 0; i < NS; ++i) {
-                p0[i] = BufPair<FMT>(rsrc, idx[i] * kEsz, 0);
-                p1[i] = BufPair<FMT>(rsrc, idx[i] * kEsz, row_bytes);
-              }
-#pragma unroll
-              for (int i = 0; i < NS; ++i) {
-                const float top = fmaf(fxs[i], p0[i].y - p0[i].x, p0[i].x);
-                const float bot = fmaf(fxs[i], p1[i].y - p1[i].x, p1[i].x);
-                const float s = fmaf(fys[i], bot - top, top) - shift;
-                const float rr = win.at(i, j);
-                ss += s;
-                sss = fmaf(s, s, sss);
-                srs = fmaf(rr, s, srs);
-              }
-            }
-          } else {
-            // straddling windows: the same three phases with per-sample
-            // checks; invalid samples read pixel 0 and are masked out
-            sr = 0.0f;
-            srr = 0.0f;
-            cnt = 0.0f;
-#pragma unroll 1
-            for (int j = 0; j < NS; ++j) {
-              const float dy = static_cast<float>(-HALF + j * STRIDE);
-              const float b0 = h01 * dy, b1 = h11 * dy, b2 = h21 * dy;
-              int idx[NS];
+              const bool row_ok = rows_checkable && txs[0] >= mlox && txs[0] <= mhix &&
+                                  tys[0] >= mloy && tys[0] <= mhiy && txs[NS - 1] >= mlox &&
+                                  txs[NS - 1] <= mhix && tys[NS - 1] >= mloy && tys[NS - 1] <= mhiy;
+              int off[NS];
               float fxs[NS], fys[NS];
-              bool oks[NS];
 #pragma unroll
               for (int i = 0; i < NS; ++i) {
-                const float dx = static_cast<float>(-HALF + i * STRIDE);
-                const float dh0 = fmaf(h00, dx, b0), dh1 = fmaf(h10, dx, b1);
-                const float dh2 = fmaf(h20, dx, b2);
-                const float den = 1.0f + dh2;
-                const float r = __builtin_amdgcn_rcpf(den);
-                const float tx = fmaf(fmaf(-ucf, dh2, dh0), r, fu);
-                const float ty = fmaf(fmaf(-vcf, dh2, dh1), r, fv);
-                const float rv = win.at(i, j);
-                const bool ok = (rv == rv) && den > 0.0f && tx >= lox && tx <= hix && ty >= loy &&
-                                ty <= hiy;
-                const float flx = floorf(tx), fly = floorf(ty);
-                fxs[i] = tx - flx;
-                fys[i] = ty - fly;
-                oks[i] = ok;
-                idx[i] = ok ? static_cast<int>(fmaf(fly, fpitch, flx + obase)) : 0;
+                const float flx = floorf(txs[i]), fly = floorf(tys[i]);
+                fxs[i] = txs[i] - flx;
+                fys[i] = tys[i] - fly;
+                off[i] = static_cast<int>(fmaf(fly, fpitch, flx + obase)) * kEsz;
               }
               float2 p0[NS], p1[NS];
 #pragma unroll
               for (int i = 0; i < NS; ++i) {
-                p0[i] = BufPair<FMT>(rsrc, idx[i] * kEsz, 0);
-                p1[i] = BufPair<FMT>(rsrc, idx[i] * kEsz, row_bytes);
+                p0[i] = BufPair<FMT>(rsrc, off[i], 0);
+                p1[i] = BufPair<FMT>(rsrc, off[i], row_bytes);
               }
+              float sv[NS];
 #pragma unroll
               for (int i = 0; i < NS; ++i) {
-                const bool ok = oks[i];
                 const float top = fmaf(fxs[i], p0[i].y - p0[i].x, p0[i].x);
                 const float bot = fmaf(fxs[i], p1[i].y - p1[i].x, p1[i].x);
-                const float s = ok ? fmaf(fys[i], bot - top, top) - shift : 0.0f;
-                const float rr = ok ? win.at(i, j) : 0.0f;
-                ss += s;
-                sss = fmaf(s, s, sss);
-                srs = fmaf(rr, s, srs);
-                sr += rr;
-                srr = fmaf(rr, rr, srr);
-                cnt += ok ? 1.0f : 0.0f;
+                sv[i] = fmaf(fys[i], bot - top, top) - shift;
+              }
+              if (__ballot(!row_ok) == 0) {
+                // every lane's row is inside both images
+#pragma unroll
+                for (int i = 0; i < NS; ++i) {
+                  const float rr = win.at(i, j);
+                  ss += sv[i];
+                  sss = fmaf(sv[i], sv[i], sss);
+                  srs = fmaf(rr, sv[i], srs);
+                  sr += rr;
+                  srr = fmaf(rr, rr, srr);
+                }
+                cnt += static_cast<float>(NS);
+              } else {
+#pragma unroll
+                for (int i = 0; i < NS; ++i) {
+                  const float rv = win.at(i, j);
+                  const bool ok = row_ok || ((rv == rv) && dens[i] > 0.0f && txs[i] >= lox &&
+                                             txs[i] <= hix && tys[i] >= loy && tys[i] <= hiy);
+                  const float s = ok ? sv[i] : 0.0f;
+                  const float rr = ok ? rv : 0.0f;
+                  ss += s;
+                  sss = fmaf(s, s, sss);
+                  srs = fmaf(rr, s, srs);
+                  sr += rr;
+                  srr = fmaf(rr, rr, srr);
+                  cnt += ok ? 1.0f : 0.0f;
+                }
               }
             }
           }
@@ -537,12 +517,10 @@ struct RefTile {
   }
 };
 
-// Random init + initial cost of every pixel (both colours).
+// Random init + initial cost of every pixel (both colours) of one tile.
 template <int HALF, int STRIDE, int FMT, int TW, int TH>
-__global__ __launch_bounds__(256) void k_plane_init(PlaneArgs a) {
-  __shared__ RefTile<HALF, FMT, TW, TH> tile;
-  int tx, ty;
-  if (!TileOf(a, &tx, &ty)) return;
+__device__ __forceinline__ void InitTile(const PlaneArgs& a, RefTile<HALF, FMT, TW, TH>& tile,
+                                         int tx, int ty) {
   const int x0 = tx * TW, y0 = ty * TH;
   tile.load(a, x0, y0);
   __syncthreads();
@@ -565,6 +543,14 @@ __global__ __launch_bounds__(256) void k_plane_init(PlaneArgs a) {
   }
 }
 
+template <int HALF, int STRIDE, int FMT, int TW, int TH>
+__global__ __launch_bounds__(256) void k_plane_init(PlaneArgs a) {
+  __shared__ RefTile<HALF, FMT, TW, TH> tile;
+  int tx, ty;
+  if (!TileOf(a, &tx, &ty)) return;
+  InitTile<HALF, STRIDE, FMT, TW, TH>(a, tile, tx, ty);
+}
+
 // Re-expresses the plane of pixel (px, py) at the ray of (x, y).
 __device__ __forceinline__ bool TransferPlane(const PlaneArgs& a, float4 p, int px, int py,
                                               const float* ray, Hyp* out) {
@@ -581,17 +567,15 @@ __device__ __forceinline__ bool TransferPlane(const PlaneArgs& a, float4 p, int 
 
 constexpr int kMaxReach = 16;
 
-// One red-black half-sweep over the pixels of colour a.color.
+// One red-black half-sweep of one tile over the pixels of colour `color`.
 template <int HALF, int STRIDE, int FMT, int TW, int TH>
-__global__ __launch_bounds__(256) void k_plane_sweep(PlaneArgs a) {
-  __shared__ RefTile<HALF, FMT, TW, TH> tile;
-  int tx, ty;
-  if (!TileOf(a, &tx, &ty)) return;
+__device__ __forceinline__ void SweepTile(const PlaneArgs& a, RefTile<HALF, FMT, TW, TH>& tile,
+                                          int tx, int ty, int color, int iter) {
   const int x0 = tx * TW, y0 = ty * TH;
   tile.load(a, x0, y0);
   __syncthreads();
   int x, y, r;
-  tile.pixel(x0, y0, a.color, &x, &y, &r);
+  tile.pixel(x0, y0, color, &x, &y, &r);
   if (x >= a.w || y >= a.h) return;
   RefWindow<HALF, STRIDE> win;
   tile.template window<STRIDE>(win, x, x0, r);
@@ -601,10 +585,10 @@ __global__ __launch_bounds__(256) void k_plane_sweep(PlaneArgs a) {
   float4 cur = a.plane[idx];
   float ccur = a.cost[idx];
   int scur = a.support[idx];
-  const uint32_t key = RngKey(a.seed, static_cast<uint32_t>(1 + 2 * a.iter + a.color),
+  const uint32_t key = RngKey(a.seed, static_cast<uint32_t>(1 + 2 * iter + color),
                               static_cast<uint32_t>(idx));
-  const float dscale = ldexpf(a.depth_perturb, -a.iter);
-  const float nscale = ldexpf(a.normal_perturb, -a.iter);
+  const float dscale = ldexpf(a.depth_perturb, -iter);
+  const float nscale = ldexpf(a.normal_perturb, -iter);
   const float4 base = cur;
   // propagation sources: per direction the lowest-cost other-colour pixel at
   // odd distances 1 .. 2*reach-1 (all reads issued before any comparison)
@@ -675,6 +659,44 @@ __global__ __launch_bounds__(256) void k_plane_sweep(PlaneArgs a) {
   a.plane[idx] = cur;
   a.cost[idx] = ccur;
   a.support[idx] = static_cast<uint8_t>(min(scur, 255));
+}
+
+template <int HALF, int STRIDE, int FMT, int TW, int TH>
+__global__ __launch_bounds__(256) void k_plane_sweep(PlaneArgs a) {
+  __shared__ RefTile<HALF, FMT, TW, TH> tile;
+  int tx, ty;
+  if (!TileOf(a, &tx, &ty)) return;
+  SweepTile<HALF, STRIDE, FMT, TW, TH>(a, tile, tx, ty, a.color, a.iter);
+}
+
+// Batched, persistent form: one launch processes (job, tile) units of many
+// reference views of any size from a work queue.  phase < 0: random init;
+// otherwise the half-sweep of colour phase & 1 of iteration phase >> 1.  Every
+// workgroup takes units until the queue is exhausted, then exits.
+template <int HALF, int STRIDE, int FMT, int TW, int TH>
+__global__ __launch_bounds__(256) void k_plane_batch(const PlaneArgs* __restrict__ args,
+                                                     const int2* __restrict__ units, int num_units,
+                                                     unsigned int* __restrict__ counter, int phase) {
+  __shared__ RefTile<HALF, FMT, TW, TH> tile;
+  __shared__ int s_unit;
+  for (;;) {
+    if (threadIdx.x == 0) s_unit = static_cast<int>(atomicAdd(counter, 1u));
+    __syncthreads();
+    const int u = __builtin_amdgcn_readfirstlane(s_unit);
+    if (u >= num_units) return;
+    const int2 un = units[u];
+    const int job = __builtin_amdgcn_readfirstlane(un.x);
+    const int t = __builtin_amdgcn_readfirstlane(un.y);
+    const PlaneArgs& a = args[job];
+    const int tx = t % a.tiles_x, ty = t / a.tiles_x;
+    if (phase < 0) InitTile<HALF, STRIDE, FMT, TW, TH>(a, tile, tx, ty);
+    else SweepTile<HALF, STRIDE, FMT, TW, TH>(a, tile, tx, ty, phase & 1, phase >> 1);
+    __syncthreads();  // the LDS tile and s_unit are reused by the next unit
+  }
+}
+
+__global__ void k_reset_counter(unsigned int* c) {
+  if (threadIdx.x == 0) *c = 0u;
 }
 
 // Scores a given plane state (both colours).
@@ -944,8 +966,8 @@ int32_t CheckParams(recon_mvs_hip_context* ctx, const recon_mvs_plane_params* p)
   if (p->iterations < 0 || p->propagation_reach < 1 || p->propagation_reach > 64)
     return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "bad iterations / propagation_reach");
   if (!(p->value_scale > 0.0)) return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "value_scale must be > 0");
-  if (p->tile_schedule < 0 || p->tile_schedule >= kNumSchedules)
-    return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "tile_schedule out of range");
+  if (p->tile_schedule < -1 || p->tile_schedule >= kNumSchedules)
+    return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "tile_schedule out of [-1, 2]");
   if (p->propagation_reach > kMaxReach)
     return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "propagation_reach above 16");
   return RECON_MVS_OK;
@@ -1052,11 +1074,46 @@ int32_t EnsureState(recon_mvs_hip_context* ctx, int view) {
   return RECON_MVS_OK;
 }
 
-enum KernelId { kSweep = 0, kInit = 1, kRefine = 2, kGeo = 3, kScore = 4, kEval = 5 };
+enum KernelId { kSweep = 0, kInit = 1, kRefine = 2, kGeo = 3, kScore = 4, kEval = 5, kBatch = 6 };
+
+// Arguments of one persistent batched launch.
+struct BatchLaunch {
+  const PlaneArgs* args;
+  const int2* units;
+  int num_units;
+  unsigned int* counter;
+  int phase;
+};
+
+// Tile geometry of schedule `s` (see kNumSchedules).
+inline void ScheduleTile(int s, int* tw, int* th) {
+  *tw = s == 1 ? 32 : (s == 2 ? 128 : 64);
+  *th = 512 / *tw;
+}
+
+// Workgroups of a persistent launch: as many as fit on the device at once.
+template <int HALF, int STRIDE, int FMT, int TW, int TH>
+int BatchGrid() {
+  static int grid = 0;
+  if (grid == 0) {
+    int dev = 0, nb = 0, cus = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_plane_batch<HALF, STRIDE, FMT, TW, TH>, 256, 0);
+    grid = std::max(1, nb) * std::max(1, cus);
+  }
+  return grid;
+}
 
 template <int HALF, int STRIDE, int FMT, int TW, int TH>
 void LaunchTile(KernelId id, PlaneArgs a, hipStream_t s, float f0, int i0, int n, const float* hyps,
-                float* cost, int* nvalid, float* view_ncc) {
+                float* cost, int* nvalid, float* view_ncc, const BatchLaunch* bl) {
+  if (id == kBatch) {
+    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, s, bl->counter);
+    hipLaunchKernelGGL((k_plane_batch<HALF, STRIDE, FMT, TW, TH>), dim3(BatchGrid<HALF, STRIDE, FMT, TW, TH>()),
+                       dim3(256), 0, s, bl->args, bl->units, bl->num_units, bl->counter, bl->phase);
+    return;
+  }
   a.tiles_x = (a.w + TW - 1) / TW;
   a.tiles_y = (a.h + TH - 1) / TH;
   a.num_tiles = a.tiles_x * a.tiles_y;
@@ -1079,17 +1136,17 @@ void LaunchTile(KernelId id, PlaneArgs a, hipStream_t s, float f0, int i0, int n
 
 template <int HALF, int STRIDE, int FMT>
 bool LaunchT(int schedule, KernelId id, const PlaneArgs& a, hipStream_t s, float f0, int i0, int n,
-             const float* hyps, float* cost, int* nvalid, float* view_ncc) {
+             const float* hyps, float* cost, int* nvalid, float* view_ncc, const BatchLaunch* bl) {
   // the 11x11 stride-2 window is compiled for every tile schedule, the others
   // for the default one
   constexpr bool kAllSchedules = (HALF == 5 && STRIDE == 2);
   switch (schedule) {
-    case 0: LaunchTile<HALF, STRIDE, FMT, 64, 8>(id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc); return true;
+    case 0: LaunchTile<HALF, STRIDE, FMT, 64, 8>(id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc, bl); return true;
     case 1:
-      if constexpr (kAllSchedules) { LaunchTile<HALF, STRIDE, FMT, 32, 16>(id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc); return true; }
+      if constexpr (kAllSchedules) { LaunchTile<HALF, STRIDE, FMT, 32, 16>(id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc, bl); return true; }
       return false;
     case 2:
-      if constexpr (kAllSchedules) { LaunchTile<HALF, STRIDE, FMT, 128, 4>(id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc); return true; }
+      if constexpr (kAllSchedules) { LaunchTile<HALF, STRIDE, FMT, 128, 4>(id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc, bl); return true; }
       return false;
     default: return false;
   }
@@ -1097,16 +1154,16 @@ bool LaunchT(int schedule, KernelId id, const PlaneArgs& a, hipStream_t s, float
 
 template <int HALF, int STRIDE>
 bool LaunchFmt(int fmt, int schedule, KernelId id, const PlaneArgs& a, hipStream_t s, float f0, int i0,
-               int n, const float* hyps, float* cost, int* nvalid, float* view_ncc) {
+               int n, const float* hyps, float* cost, int* nvalid, float* view_ncc, const BatchLaunch* bl) {
   // FP16 / BF16 storage is compiled for the 11x11 stride-2 window only
   constexpr bool kAllFormats = (HALF == 5 && STRIDE == 2);
   switch (fmt) {
-    case RECON_MVS_PIXEL_F32: return LaunchT<HALF, STRIDE, RECON_MVS_PIXEL_F32>(schedule, id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc);
+    case RECON_MVS_PIXEL_F32: return LaunchT<HALF, STRIDE, RECON_MVS_PIXEL_F32>(schedule, id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc, bl);
     case RECON_MVS_PIXEL_F16:
-      if constexpr (kAllFormats) return LaunchT<HALF, STRIDE, RECON_MVS_PIXEL_F16>(schedule, id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc);
+      if constexpr (kAllFormats) return LaunchT<HALF, STRIDE, RECON_MVS_PIXEL_F16>(schedule, id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc, bl);
       return false;
     case RECON_MVS_PIXEL_BF16:
-      if constexpr (kAllFormats) return LaunchT<HALF, STRIDE, RECON_MVS_PIXEL_BF16>(schedule, id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc);
+      if constexpr (kAllFormats) return LaunchT<HALF, STRIDE, RECON_MVS_PIXEL_BF16>(schedule, id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc, bl);
       return false;
     default: return false;
   }
@@ -1116,20 +1173,20 @@ bool LaunchFmt(int fmt, int schedule, KernelId id, const PlaneArgs& a, hipStream
 int32_t Launch(recon_mvs_hip_context* ctx, const recon_mvs_plane_params* p, KernelId id,
                const PlaneArgs& a, float f0 = 0.0f, int i0 = 0, int n = 0,
                const float* hyps = nullptr, float* cost = nullptr, int* nvalid = nullptr,
-               float* view_ncc = nullptr) {
+               float* view_ncc = nullptr, const BatchLaunch* bl = nullptr, int schedule = -1) {
   const int fmt = ctx->storage_format;
-  const int sch = p->tile_schedule;
+  const int sch = schedule >= 0 ? schedule : p->tile_schedule;
   hipEvent_t eb, ee;
   TimerBegin(ctx, static_cast<int>(id), &eb, &ee);
   bool ok = false;
   const int ps = p->patch_size, st = p->window_stride;
-  if (ps == 11 && st == 2) ok = LaunchFmt<5, 2>(fmt, sch, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc);
+  if (ps == 11 && st == 2) ok = LaunchFmt<5, 2>(fmt, sch, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc, bl);
 #ifndef RMVS_BENCH_WINDOW_ONLY  // (development builds compile the 11x11 stride-2 window only)
-  else if (ps == 11 && st == 1) ok = LaunchFmt<5, 1>(fmt, sch, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc);
-  else if (ps == 7 && st == 1) ok = LaunchFmt<3, 1>(fmt, sch, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc);
-  else if (ps == 5 && st == 1) ok = LaunchFmt<2, 1>(fmt, sch, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc);
-  else if (ps == 7 && st == 2) ok = LaunchFmt<3, 2>(fmt, sch, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc);
-  else if (ps == 9 && st == 2) ok = LaunchFmt<4, 2>(fmt, sch, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc);
+  else if (ps == 11 && st == 1) ok = LaunchFmt<5, 1>(fmt, sch, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc, bl);
+  else if (ps == 7 && st == 1) ok = LaunchFmt<3, 1>(fmt, sch, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc, bl);
+  else if (ps == 5 && st == 1) ok = LaunchFmt<2, 1>(fmt, sch, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc, bl);
+  else if (ps == 7 && st == 2) ok = LaunchFmt<3, 2>(fmt, sch, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc, bl);
+  else if (ps == 9 && st == 2) ok = LaunchFmt<4, 2>(fmt, sch, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc, bl);
 #endif
   if (!ok)
     return Fail(ctx, RECON_MVS_ERR_NOT_IMPLEMENTED,
@@ -1145,7 +1202,93 @@ double MeanBaselineViews(recon_mvs_hip_context* ctx, int ref, int nsrc, const in
   return MeanBaseline(ctx->views[ref]->cam, c.data(), nsrc);
 }
 
+std::string TuneKey(const recon_mvs_hip_context* ctx, const PlaneArgs& a, const recon_mvs_plane_params* p) {
+  char buf[128];
+  std::snprintf(buf, sizeof(buf), "%dx%d/%d/%d/%d/%d/%d", a.w, a.h, a.nsrc, a.top_k, p->patch_size,
+                p->window_stride, ctx->storage_format);
+  return buf;
+}
+
+// Times random init + two half-sweeps of view `a` under every compiled tile
+// schedule and returns the fastest (the state of the slot is overwritten).
+int32_t Autotune(recon_mvs_hip_context* ctx, const PlaneArgs& a, const recon_mvs_plane_params* p,
+                 int* best, double* ms_out) {
+  hipEvent_t e0, e1;
+  RMVS_HIP(ctx, hipEventCreate(&e0));
+  RMVS_HIP(ctx, hipEventCreate(&e1));
+  const bool timing = ctx->timing;
+  ctx->timing = false;
+  double best_ms = 1e300;
+  *best = 0;
+  int32_t st = RECON_MVS_OK;
+  for (int s = 0; s < kNumSchedules; ++s) {
+    if (ms_out) ms_out[s] = -1.0;
+    // warm-up launch (code load), then the timed one
+    PlaneArgs b = a;
+    if (Launch(ctx, p, kInit, b, 0.0f, 0, 0, nullptr, nullptr, nullptr, nullptr, nullptr, s) !=
+        RECON_MVS_OK) {
+      ctx->error.clear();
+      continue;
+    }
+    (void)hipEventRecord(e0, ctx->stream);
+    for (int c = 0; c < 2 && st == RECON_MVS_OK; ++c) {
+      b.iter = 0;
+      b.color = c;
+      st = Launch(ctx, p, kSweep, b, 0.0f, 0, 0, nullptr, nullptr, nullptr, nullptr, nullptr, s);
+    }
+    (void)hipEventRecord(e1, ctx->stream);
+    if (st != RECON_MVS_OK) break;
+    float ms = 0.0f;
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+      st = Fail(ctx, RECON_MVS_ERR_GPU_FAILURE, "autotune timing failed");
+      break;
+    }
+    if (ms_out) ms_out[s] = ms;
+    if (ms < best_ms) {
+      best_ms = ms;
+      *best = s;
+    }
+  }
+  ctx->timing = timing;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return st;
+}
+
+// The schedule to use for view `a`: p->tile_schedule, or the autotuned one
+// (cached per size / views / window / format) when it is -1.
+int32_t ResolveSchedule(recon_mvs_hip_context* ctx, const PlaneArgs& a,
+                        const recon_mvs_plane_params* p, int* sched) {
+  if (p->tile_schedule >= 0) {
+    *sched = p->tile_schedule;
+    return RECON_MVS_OK;
+  }
+  const std::string key = TuneKey(ctx, a, p);
+  auto it = ctx->tuned_schedule.find(key);
+  if (it != ctx->tuned_schedule.end()) {
+    *sched = it->second;
+    return RECON_MVS_OK;
+  }
+  int best = 0;
+  const int32_t st = Autotune(ctx, a, p, &best, nullptr);
+  if (st != RECON_MVS_OK) return st;
+  ctx->tuned_schedule[key] = best;
+  *sched = best;
+  return RECON_MVS_OK;
+}
+
 }  // namespace
+
+struct recon_mvs_hip_batch {
+  recon_mvs_plane_params params{};
+  int schedule = 0;
+  std::vector<int> refs;
+  PlaneArgs* d_args = nullptr;
+  int2* d_units = nullptr;
+  int num_units = 0;
+  unsigned int* d_counter = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
 
 void DestroyPlaneState(recon_mvs_hip_context* ctx) {
   for (auto* s : ctx->states) FreeState(s);
@@ -1237,13 +1380,16 @@ RECON_MVS_API int32_t recon_mvs_hip_plane_patchmatch(recon_mvs_hip_context* ctx,
   a.plane = S.plane;
   a.cost = S.cost;
   a.support = S.support;
-  st = Launch(ctx, params, kInit, a);
+  int sched = 0;
+  st = ResolveSchedule(ctx, a, params, &sched);
+  if (st != RECON_MVS_OK) return st;
+  st = Launch(ctx, params, kInit, a, 0.0f, 0, 0, nullptr, nullptr, nullptr, nullptr, nullptr, sched);
   if (st != RECON_MVS_OK) return st;
   for (int it = 0; it < params->iterations; ++it) {
     for (int color = 0; color < 2; ++color) {
       a.iter = it;
       a.color = color;
-      st = Launch(ctx, params, kSweep, a);
+      st = Launch(ctx, params, kSweep, a, 0.0f, 0, 0, nullptr, nullptr, nullptr, nullptr, nullptr, sched);
       if (st != RECON_MVS_OK) return st;
     }
   }
@@ -1268,7 +1414,8 @@ RECON_MVS_API int32_t recon_mvs_hip_plane_refine(recon_mvs_hip_context* ctx, int
   const double focal = 0.5 * (R.cam.fx + R.cam.fy);
   if (!(baseline > 1e-9)) return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "zero baseline");
   const float inv_per_px = static_cast<float>(1.0 / (focal * baseline));
-  return Launch(ctx, params, kRefine, a, inv_per_px, params->refine_iterations);
+  return Launch(ctx, params, kRefine, a, inv_per_px, params->refine_iterations, 0, nullptr, nullptr,
+                nullptr, nullptr, nullptr, std::max(0, params->tile_schedule));
 }
 
 RECON_MVS_API int32_t recon_mvs_hip_set_plane_state(recon_mvs_hip_context* ctx, int32_t view,
@@ -1291,7 +1438,8 @@ RECON_MVS_API int32_t recon_mvs_hip_set_plane_state(recon_mvs_hip_context* ctx, 
   a.plane = S.plane;
   a.cost = S.cost;
   a.support = S.support;
-  st = Launch(ctx, params, kScore, a);
+  st = Launch(ctx, params, kScore, a, 0.0f, 0, 0, nullptr, nullptr, nullptr, nullptr, nullptr,
+              std::max(0, params->tile_schedule));
   if (st != RECON_MVS_OK) return st;
   RMVS_HIP(ctx, hipStreamSynchronize(ctx->stream));
   S.has_geo = false;
@@ -1391,7 +1539,8 @@ RECON_MVS_API int32_t recon_mvs_hip_eval_plane_costs(recon_mvs_hip_context* ctx,
   int* dv = reinterpret_cast<int*>(dc + n);
   float* dn = dc + 2 * static_cast<size_t>(n);
   RMVS_HIP(ctx, hipMemcpyAsync(dh, hyps, sizeof(float) * 6 * n, hipMemcpyHostToDevice, ctx->stream));
-  st = Launch(ctx, params, kEval, a, 0.0f, 0, n, dh, dc, dv, view_ncc ? dn : nullptr);
+  st = Launch(ctx, params, kEval, a, 0.0f, 0, n, dh, dc, dv, view_ncc ? dn : nullptr, nullptr,
+              std::max(0, params->tile_schedule));
   if (st != RECON_MVS_OK) return st;
   RMVS_HIP(ctx, hipMemcpyAsync(cost, dc, sizeof(float) * n, hipMemcpyDeviceToHost, ctx->stream));
   RMVS_HIP(ctx, hipMemcpyAsync(valid_views, dv, sizeof(int) * n, hipMemcpyDeviceToHost, ctx->stream));
@@ -1415,6 +1564,149 @@ RECON_MVS_API int32_t recon_mvs_hip_timing(recon_mvs_hip_context* ctx, int32_t e
       ctx->timed_count[i] = 0;
     }
   ctx->timing = enable != 0;
+  return RECON_MVS_OK;
+}
+
+RECON_MVS_API int32_t recon_mvs_hip_autotune(recon_mvs_hip_context* ctx, int32_t ref_view,
+                                             int32_t num_src, const int32_t* src_views,
+                                             const recon_mvs_plane_params* params,
+                                             int32_t* best_schedule, double* ms_per_schedule) {
+  if (!ctx) return RECON_MVS_ERR_INVALID_ARGUMENT;
+  if (!best_schedule) return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "null best_schedule");
+  PlaneArgs a;
+  int32_t st = MakeArgs(ctx, ref_view, num_src, src_views, params, &a);
+  if (st != RECON_MVS_OK) return st;
+  st = EnsureState(ctx, ref_view);
+  if (st != RECON_MVS_OK) return st;
+  PlaneState& S = *ctx->states[ref_view];
+  a.plane = S.plane;
+  a.cost = S.cost;
+  a.support = S.support;
+  int best = 0;
+  st = Autotune(ctx, a, params, &best, ms_per_schedule);
+  if (st != RECON_MVS_OK) return st;
+  ctx->tuned_schedule[TuneKey(ctx, a, params)] = best;
+  *best_schedule = best;
+  return RECON_MVS_OK;
+}
+
+RECON_MVS_API int32_t recon_mvs_hip_batch_create(recon_mvs_hip_context* ctx, int32_t num_jobs,
+                                                 const recon_mvs_batch_job* jobs,
+                                                 const recon_mvs_plane_params* params,
+                                                 recon_mvs_hip_batch** out) {
+  if (!ctx) return RECON_MVS_ERR_INVALID_ARGUMENT;
+  if (!out || !jobs || !params || num_jobs < 1)
+    return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "batch_create: bad arguments");
+  *out = nullptr;
+  std::vector<PlaneArgs> args(num_jobs);
+  std::vector<int> seen(ctx->views.size(), 0);
+  for (int j = 0; j < num_jobs; ++j) {
+    int32_t st = MakeArgs(ctx, jobs[j].ref_view, jobs[j].num_src, jobs[j].src_views, params, &args[j]);
+    if (st != RECON_MVS_OK) return st;
+    if (seen[jobs[j].ref_view]++) return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "a view is the reference of two jobs");
+    st = EnsureState(ctx, jobs[j].ref_view);
+    if (st != RECON_MVS_OK) return st;
+    PlaneState& S = *ctx->states[jobs[j].ref_view];
+    S.has_geo = false;
+    args[j].plane = S.plane;
+    args[j].cost = S.cost;
+    args[j].support = S.support;
+  }
+  // tile schedule: given, or tuned on the job with the most pixel-views
+  int sched = params->tile_schedule;
+  if (sched < 0) {
+    int big = 0;
+    for (int j = 1; j < num_jobs; ++j)
+      if (static_cast<double>(args[j].w) * args[j].h * args[j].nsrc >
+          static_cast<double>(args[big].w) * args[big].h * args[big].nsrc)
+        big = j;
+    const int32_t st = ResolveSchedule(ctx, args[big], params, &sched);
+    if (st != RECON_MVS_OK) return st;
+  }
+  int tw, th;
+  ScheduleTile(sched, &tw, &th);
+  // work units, most views first (largest units leave the queue first)
+  std::vector<int> order(num_jobs);
+  for (int j = 0; j < num_jobs; ++j) order[j] = j;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int x, int y) { return args[x].nsrc > args[y].nsrc; });
+  std::vector<int2> units;
+  for (int j : order) {
+    PlaneArgs& a = args[j];
+    a.tiles_x = (a.w + tw - 1) / tw;
+    a.tiles_y = (a.h + th - 1) / th;
+    a.num_tiles = a.tiles_x * a.tiles_y;
+    a.tiles_per_xcd = (a.num_tiles + 7) / 8;
+    for (int t = 0; t < a.num_tiles; ++t) units.push_back(make_int2(j, t));
+  }
+  auto* b = new recon_mvs_hip_batch();
+  b->params = *params;
+  b->schedule = sched;
+  for (int j = 0; j < num_jobs; ++j) b->refs.push_back(jobs[j].ref_view);
+  b->num_units = static_cast<int>(units.size());
+  auto cleanup = [&](int32_t code, const char* msg) {
+    if (b->d_args) (void)hipFree(b->d_args);
+    if (b->d_units) (void)hipFree(b->d_units);
+    if (b->d_counter) (void)hipFree(b->d_counter);
+    delete b;
+    return Fail(ctx, code, msg);
+  };
+  if (hipMalloc(&b->d_args, sizeof(PlaneArgs) * num_jobs) != hipSuccess ||
+      hipMalloc(&b->d_units, sizeof(int2) * units.size()) != hipSuccess ||
+      hipMalloc(&b->d_counter, 256) != hipSuccess)
+    return cleanup(RECON_MVS_ERR_OUT_OF_MEMORY, "batch allocation failed");
+  if (hipMemcpy(b->d_args, args.data(), sizeof(PlaneArgs) * num_jobs, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->d_units, units.data(), sizeof(int2) * units.size(), hipMemcpyHostToDevice) != hipSuccess)
+    return cleanup(RECON_MVS_ERR_GPU_FAILURE, "batch upload failed");
+  // capture init + all half-sweeps as one graph
+  const bool timing = ctx->timing;
+  ctx->timing = false;
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess ||
+      hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    ctx->timing = timing;
+    return cleanup(RECON_MVS_ERR_GPU_FAILURE, "graph capture failed to start");
+  }
+  int32_t st = RECON_MVS_OK;
+  for (int phase = -1; phase < 2 * params->iterations && st == RECON_MVS_OK; ++phase) {
+    BatchLaunch bl{b->d_args, b->d_units, b->num_units, b->d_counter, phase};
+    st = Launch(ctx, params, kBatch, args[0], 0.0f, 0, 0, nullptr, nullptr, nullptr, nullptr, &bl, sched);
+  }
+  hipGraph_t graph = nullptr;
+  const hipError_t ec = hipStreamEndCapture(ctx->stream, &graph);
+  ctx->timing = timing;
+  if (st != RECON_MVS_OK || ec != hipSuccess || !graph) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return cleanup(RECON_MVS_ERR_GPU_FAILURE, "graph capture failed");
+  }
+  const hipError_t ei = hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ei != hipSuccess) return cleanup(RECON_MVS_ERR_GPU_FAILURE, "graph instantiation failed");
+  *out = b;
+  return RECON_MVS_OK;
+}
+
+RECON_MVS_API int32_t recon_mvs_hip_batch_run(recon_mvs_hip_context* ctx, recon_mvs_hip_batch* b) {
+  if (!ctx) return RECON_MVS_ERR_INVALID_ARGUMENT;
+  if (!b) return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "null batch");
+  hipEvent_t eb, ee;
+  TimerBegin(ctx, kBatch, &eb, &ee);
+  RMVS_HIP(ctx, hipGraphLaunch(b->exec, ctx->stream));
+  TimerEnd(ctx, kBatch, eb, ee);
+  return RECON_MVS_OK;
+}
+
+RECON_MVS_API int32_t recon_mvs_hip_batch_schedule(const recon_mvs_hip_batch* b) {
+  return b ? b->schedule : -1;
+}
+
+RECON_MVS_API int32_t recon_mvs_hip_batch_destroy(recon_mvs_hip_context* ctx, recon_mvs_hip_batch* b) {
+  if (!b) return RECON_MVS_OK;
+  if (ctx) (void)hipStreamSynchronize(ctx->stream);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->d_args) (void)hipFree(b->d_args);
+  if (b->d_units) (void)hipFree(b->d_units);
+  if (b->d_counter) (void)hipFree(b->d_counter);
+  delete b;
   return RECON_MVS_OK;
 }
 

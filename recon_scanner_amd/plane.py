@@ -20,7 +20,7 @@ from . import _native as N
 from . import mvs
 
 FORMATS = {"f32": N.PIXEL_F32, "f16": N.PIXEL_F16, "bf16": N.PIXEL_BF16}
-KERNEL_CLASSES = ("sweep", "init", "refine", "consistency", "score", "eval")
+KERNEL_CLASSES = ("sweep", "init", "refine", "consistency", "score", "eval", "batch")
 
 
 class PlaneParams(C.Structure):
@@ -39,6 +39,10 @@ class PlaneParams(C.Structure):
                 raise AttributeError(k)
             setattr(p, k, v)
         return p
+
+
+class BatchJob(C.Structure):
+    _fields_ = [("ref_view", C.c_int32), ("num_src", C.c_int32), ("src_views", C.c_int32 * 16)]
 
 
 class View(C.Structure):
@@ -66,6 +70,12 @@ def _lib():
         L.recon_mvs_hip_eval_plane_costs.argtypes = [v, C.c_int32, C.c_int32, v, P(PlaneParams),
                                                      C.c_int32, v, v, v, v]
         L.recon_mvs_hip_timing.argtypes = [v, C.c_int32, C.c_int32, v, v]
+        L.recon_mvs_hip_autotune.argtypes = [v, C.c_int32, C.c_int32, v, P(PlaneParams), v, v]
+        L.recon_mvs_hip_batch_create.argtypes = [v, C.c_int32, P(BatchJob), P(PlaneParams),
+                                                 P(C.c_void_p)]
+        L.recon_mvs_hip_batch_run.argtypes = [v, v]
+        L.recon_mvs_hip_batch_schedule.argtypes = [v]
+        L.recon_mvs_hip_batch_destroy.argtypes = [v, v]
         _declared = True
     return L
 
@@ -164,6 +174,20 @@ class PlaneMVS:
                                                           vn.ctypes.data), "eval_plane_costs")
         return cost, nv, vn
 
+    def autotune(self, ref, srcs, params):
+        """Times every tile schedule on this view; returns (best, [ms per schedule])."""
+        s = _i32(srcs)
+        best = C.c_int32(0)
+        ms = np.zeros(8)
+        self._check(self.L.recon_mvs_hip_autotune(self.ctx, ref, len(s), s.ctypes.data,
+                                                  C.byref(params), C.byref(best), ms.ctypes.data),
+                    "autotune")
+        return int(best.value), ms[:3].tolist()
+
+    def batch(self, jobs, params):
+        """A captured batch over jobs = [(ref, [srcs])]; see Batch."""
+        return Batch(self, jobs, params)
+
     def timing(self, enable=True, reset=False):
         ms = np.zeros(8)
         cnt = np.zeros(8, np.int64)
@@ -182,3 +206,37 @@ def round_storage(img, storage):
     b = a.view(np.uint32).astype(np.uint64)
     r = ((b + 0x7FFF + ((b >> 16) & 1)) >> 16) << 16
     return r.astype(np.uint32).view(np.float32)
+
+
+class Batch:
+    """Many reference views of any size in one persistent launch per phase,
+    captured once into a HIP graph (recon_mvs_hip_batch_*)."""
+
+    def __init__(self, pm: PlaneMVS, jobs, params):
+        self.pm = pm
+        arr = (BatchJob * len(jobs))()
+        for i, (r, srcs) in enumerate(jobs):
+            arr[i].ref_view = int(r)
+            arr[i].num_src = len(srcs)
+            for k, v in enumerate(srcs):
+                arr[i].src_views[k] = int(v)
+        h = C.c_void_p()
+        pm._check(pm.L.recon_mvs_hip_batch_create(pm.ctx, len(jobs), arr, C.byref(params),
+                                                  C.byref(h)), "batch_create")
+        self.h = h
+        self.refs = [int(r) for r, _ in jobs]
+        self.schedule = int(pm.L.recon_mvs_hip_batch_schedule(h))
+
+    def run(self):
+        self.pm._check(self.pm.L.recon_mvs_hip_batch_run(self.pm.ctx, self.h), "batch_run")
+
+    def close(self):
+        if self.h:
+            self.pm.L.recon_mvs_hip_batch_destroy(self.pm.ctx, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -194,10 +194,10 @@ def orbit_batch(num_refs=48, num_cams=64, seed=0, min_res=(640, 480), max_res=(1
     jobs = []
     for r in refs:
         ns = int(rng.integers(src_range[0], src_range[1] + 1))
-        cand = [((r + d) % num_cams) for d in range(-6, 7) if d != 0]
+        cand = sorted({(r + d) % num_cams for d in range(-6, 7)} - {r})
         srcs = sorted(rng.choice(cand, size=min(ns, len(cand)), replace=False).tolist())
         jobs.append((r, srcs))
-    sc = Scene(cams, imgs, deps, radius - 0.15, radius + 0.15, name=f"orbit_batch_{num_refs}",
+    sc = Scene(cams, imgs, deps, radius - 0.15, radius + 0.35, name=f"orbit_batch_{num_refs}",
                meta=dict(seed=seed, num_cams=num_cams))
     return sc, jobs
 

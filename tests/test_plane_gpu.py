@@ -154,3 +154,41 @@ def test_refine_keeps_accuracy():
     say("median before", ea, "after", eb)
     assert np.all(b["cost"][m] <= a["cost"][m])
     assert eb <= ea * 1.05
+
+
+def test_batch_matches_per_view_runs_bit_exact():
+    """Batched persistent launch (uneven resolutions, 4-10 sources, FP16
+    storage) == per-view plane_patchmatch with the same parameters."""
+    say("batch: orbit of 12 cameras, 6 refs, uneven sizes, f16")
+    sc, jobs = scenes.orbit_batch(num_refs=6, num_cams=12, seed=3, min_res=(160, 120),
+                                  max_res=(320, 240), src_range=(4, 6))
+    p = plane.default_params(patch_size=11, window_stride=2, top_k=4, iterations=2,
+                             min_depth_m=sc.min_depth, max_depth_m=sc.max_depth, value_scale=1.0,
+                             tile_schedule=1)
+    pm = plane.PlaneMVS(sc.cams, sc.images, storage="f16")
+    singles = {}
+    for r, srcs in jobs:
+        pm.patchmatch(r, srcs, p)
+        singles[r] = pm.depth_map(r)
+    b = pm.batch(jobs, p)
+    for rep in range(3):   # graph replays, eager work between them
+        b.run()
+        pm.synchronize()
+        for r, _ in jobs:
+            g = pm.depth_map(r)
+            for k in ("depth", "normals", "cost", "support"):
+                assert np.array_equal(np.nan_to_num(g[k], nan=-1), np.nan_to_num(singles[r][k], nan=-1)), (rep, r, k)
+    b.close()
+
+
+def test_autotune_picks_a_compiled_schedule():
+    say("autotune")
+    sc = scenes.sphere_planes(width=480, height=360, focal=625.0, supersample=1)
+    p = plane.default_params(patch_size=11, window_stride=2, top_k=4, iterations=1,
+                             min_depth_m=0.25, max_depth_m=0.60, value_scale=1.0, tile_schedule=-1)
+    pm = plane.PlaneMVS(sc.cams, sc.images)
+    best, ms = pm.autotune(0, list(range(1, 9)), p)
+    say("schedule ms", ms, "best", best)
+    assert best in (0, 1, 2) and ms[best] > 0 and ms[best] == min(m for m in ms if m > 0)
+    pm.patchmatch(0, list(range(1, 9)), p)     # -1 resolves through the cache
+    assert np.isfinite(pm.depth_map(0)["depth"]).mean() > 0.5
