@@ -41,7 +41,7 @@ def parse():
     ap.add_argument("--height", type=int, default=1440)
     ap.add_argument("--iterations", type=int, default=5)
     ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16"])
-    ap.add_argument("--tile-schedule", type=int, default=0)
+    ap.add_argument("--tile-schedule", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target duration of the CPU-baseline sample (0 = skip)")
     return ap.parse_args()

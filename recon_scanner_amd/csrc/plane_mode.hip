@@ -259,9 +259,13 @@ __device__ float PlaneCost(const PlaneArgs& a, int x, int y, const RefWindow<HAL
           const double fsu = floor(su), fsv = floor(sv);
           const int iu = static_cast<int>(fsu), iv = static_cast<int>(fsv);
           const float fu = static_cast<float>(su - fsu), fv = static_cast<float>(sv - fsv);
-          const float ucf = static_cast<float>(uc), vcf = static_cast<float>(vc);
-          const float h00 = static_cast<float>(H[0] * ih), h01 = static_cast<float>(H[1] * ih);
-          const float h10 = static_cast<float>(H[3] * ih), h11 = static_cast<float>(H[4] * ih);
+          // Offset of a sample from the centre projection, for a window offset
+          // (dx, dy): du = (g00 dx + g01 dy) / (1 + h20 dx + h21 dy), and dv
+          // likewise with g10, g11; g = (H_row - u_c H_2) / hc2 in FP64.
+          const float g00 = static_cast<float>((H[0] - uc * H[6]) * ih);
+          const float g01 = static_cast<float>((H[1] - uc * H[7]) * ih);
+          const float g10 = static_cast<float>((H[3] - vc * H[6]) * ih);
+          const float g11 = static_cast<float>((H[4] - vc * H[7]) * ih);
           const float h20 = static_cast<float>(H[6] * ih), h21 = static_cast<float>(H[7] * ih);
           const int w = a.src_w[v], hgt = a.src_h[v], pitch = a.src_pitch[v];
           const float lox = static_cast<float>(-iu), hix = static_cast<float>(w - 1 - iu);
@@ -276,14 +280,13 @@ __device__ float PlaneCost(const PlaneArgs& a, int x, int y, const RefWindow<HAL
           const float obase = static_cast<float>(iv) * fpitch + static_cast<float>(iu);
           const float shift = win.shift;
           // A window entirely beyond one image edge cannot give a valid view:
-          // skip it.  Bound on the sample offsets from the centre: |dh| <=
-          // HALF (|h.0| + |h.1|) per row of H, so |du| <= (e0 + |u_c| e2) /
-          // (1 - e2) while e2 < 1.
-          const float e0 = HALF * (fabsf(h00) + fabsf(h01)), e1 = HALF * (fabsf(h10) + fabsf(h11));
+          // skip it.  Over the window |g0. dx + g0. dy| <= e0 (e1 for v) and
+          // the denominator is >= 1 - e2, so |du| <= e0 / (1 - e2).
+          const float e0 = HALF * (fabsf(g00) + fabsf(g01)), e1 = HALF * (fabsf(g10) + fabsf(g11));
           const float e2 = HALF * (fabsf(h20) + fabsf(h21));
           const float rd = __builtin_amdgcn_rcpf(1.0f - e2);
-          const float bu = fmaf(fabsf(ucf), e2, e0) * rd + 1e-3f;
-          const float bv = fmaf(fabsf(vcf), e2, e1) * rd + 1e-3f;
+          const float bu = e0 * rd + 1e-3f;
+          const float bv = e1 * rd + 1e-3f;
           // e2 < 0.5 keeps every sample's denominator positive
           const bool den_ok = e2 < 0.5f;
           const bool outside = den_ok && (fu + bu < lox || fu - bu > hix || fv + bv < loy || fv - bv > hiy);
@@ -292,7 +295,9 @@ __device__ float PlaneCost(const PlaneArgs& a, int x, int y, const RefWindow<HAL
           // samples are inside the image with a 1e-4 px margin is inside.
           const bool rows_checkable = den_ok && win.full;
           const float mlox = lox + 1e-4f, mhix = hix - 1e-4f, mloy = loy + 1e-4f, mhiy = hiy - 1e-4f;
-          float ss = 0.0f, sss = 0.0f, srs = 0.0f, sr = 0.0f, srr = 0.0f, cnt = 0.0f;
+          // Sums over the source samples; the reference sums start from the
+          // whole window and lose the reference values of excluded samples.
+          float ss = 0.0f, sss = 0.0f, srs = 0.0f, xr = 0.0f, xrr = 0.0f, xcnt = 0.0f;
           if (!outside) {
             // Each window row in three phases (positions, all loads,
             // arithmetic): a wave keeps 2 x NS loads in flight.  Loads need
@@ -301,21 +306,20 @@ __device__ float PlaneCost(const PlaneArgs& a, int x, int y, const RefWindow<HAL
 #pragma unroll 1
             for (int j = 0; j < NS; ++j) {
               const float dy = static_cast<float>(-HALF + j * STRIDE);
-              const float b0 = h01 * dy, b1 = h11 * dy, b2 = h21 * dy;
+              const float b0 = g01 * dy, b1 = g11 * dy, b2 = fmaf(h21, dy, 1.0f);
               float txs[NS], tys[NS], dens[NS];
 #pragma unroll
               for (int i = 0; i < NS; ++i) {
                 const float dx = static_cast<float>(-HALF + i * STRIDE);
-                const float dh0 = fmaf(h00, dx, b0), dh1 = fmaf(h10, dx, b1);
-                const float dh2 = fmaf(h20, dx, b2);
-                dens[i] = 1.0f + dh2;
+                dens[i] = fmaf(h20, dx, b2);
                 const float r = __builtin_amdgcn_rcpf(dens[i]);
-                txs[i] = fmaf(fmaf(-ucf, dh2, dh0), r, fu);
-                tys[i] = fmaf(fmaf(-vcf, dh2, dh1), r, fv);
+                txs[i] = fmaf(fmaf(g00, dx, b0), r, fu);
+                tys[i] = fmaf(fmaf(g10, dx, b1), r, fv);
               }
-              const bool row_ok = rows_checkable && txs[0] >= mlox && txs[0] <= mhix &&
-                                  tys[0] >= mloy && tys[0] <= mhiy && txs[NS - 1] >= mlox &&
-                                  txs[NS - 1] <= mhix && tys[NS - 1] >= mloy && tys[NS - 1] <= mhiy;
+              // (bitwise & on purpose: no short-circuit branches)
+              const bool row_ok = rows_checkable & (txs[0] >= mlox) & (txs[0] <= mhix) &
+                                  (tys[0] >= mloy) & (tys[0] <= mhiy) & (txs[NS - 1] >= mlox) &
+                                  (txs[NS - 1] <= mhix) & (tys[NS - 1] >= mloy) & (tys[NS - 1] <= mhiy);
               int off[NS];
               float fxs[NS], fys[NS];
 #pragma unroll
@@ -342,32 +346,33 @@ __device__ float PlaneCost(const PlaneArgs& a, int x, int y, const RefWindow<HAL
                 // every lane's row is inside both images
 #pragma unroll
                 for (int i = 0; i < NS; ++i) {
-                  const float rr = win.at(i, j);
                   ss += sv[i];
                   sss = fmaf(sv[i], sv[i], sss);
-                  srs = fmaf(rr, sv[i], srs);
-                  sr += rr;
-                  srr = fmaf(rr, rr, srr);
+                  srs = fmaf(win.at(i, j), sv[i], srs);
                 }
-                cnt += static_cast<float>(NS);
               } else {
 #pragma unroll
                 for (int i = 0; i < NS; ++i) {
                   const float rv = win.at(i, j);
-                  const bool ok = row_ok || ((rv == rv) && dens[i] > 0.0f && txs[i] >= lox &&
-                                             txs[i] <= hix && tys[i] >= loy && tys[i] <= hiy);
+                  const bool rin = rv == rv;  // NaN: outside the reference image
+                  const bool ok = row_ok | (rin & (dens[i] > 0.0f) & (txs[i] >= lox) &
+                                            (txs[i] <= hix) & (tys[i] >= loy) & (tys[i] <= hiy));
                   const float s = ok ? sv[i] : 0.0f;
-                  const float rr = ok ? rv : 0.0f;
+                  const float x = (ok || !rin) ? 0.0f : rv;
                   ss += s;
                   sss = fmaf(s, s, sss);
-                  srs = fmaf(rr, s, srs);
-                  sr += rr;
-                  srr = fmaf(rr, rr, srr);
-                  cnt += ok ? 1.0f : 0.0f;
+                  srs = fmaf(ok ? rv : 0.0f, s, srs);
+                  xr += x;
+                  xrr = fmaf(x, x, xrr);
+                  xcnt += (ok || !rin) ? 0.0f : 1.0f;
                 }
               }
             }
+          } else {
+            xcnt = win.n;
           }
+          const float cnt = win.n - xcnt;
+          const float sr = win.sum_r - xr, srr = win.sum_r2 - xrr;
           if (cnt >= static_cast<float>(S / 2) && cnt > 0.0f) {
             const float in = __builtin_amdgcn_rcpf(cnt);
             const float mr = sr * in, ms = ss * in;
