@@ -383,11 +383,45 @@ RECON_MVS_API int32_t recon_mvs_hip_batch_schedule(const recon_mvs_hip_batch* ba
 RECON_MVS_API int32_t recon_mvs_hip_batch_destroy(recon_mvs_hip_context* ctx, recon_mvs_hip_batch* batch);
 
 /*
+ * Adjoint of the plane cost (no counterpart in the reference's CPU code: the
+ * reverse mode its design documents plan for pose refinement,
+ * docs/05_单目管线.md).  For every pixel of the depth-map slot of ref_view,
+ * with the planes it holds (from plane_patchmatch or set_plane_state):
+ *   cost[P]          the aggregated cost (1e30: invalid), as plane_patchmatch
+ *                    scores it, under source poses perturbed by pose_delta;
+ *   grad_plane[P*4]  d(sum_p w_p cost_p) / d(depth, nx, ny, nz) per pixel;
+ *   grad_pose[6*num_src]  d(sum_p w_p cost_p) / d(w, tau) per source, for the
+ *                    left perturbation Y -> Exp(w) Y + tau of the source's
+ *                    world->camera transform (FP64 reduction, deterministic).
+ * pose_delta (6*num_src doubles: w, tau per source) and weights (P floats)
+ * may be NULL (no perturbation / all ones); any output may be NULL.  Invalid
+ * pixels get zero gradient.  Windows: (11, 2) in every storage format, (7, 1)
+ * and (5, 1) in FP32.  Synchronous.
+ */
+RECON_MVS_API int32_t recon_mvs_hip_plane_cost_grad(recon_mvs_hip_context* ctx, int32_t ref_view,
+                                                    int32_t num_src, const int32_t* src_views,
+                                                    const recon_mvs_plane_params* params,
+                                                    const double* pose_delta, const float* weights,
+                                                    float* cost, float* grad_plane,
+                                                    double* grad_pose);
+
+/*
+ * Gradient polish of the depth-map slot of ref_view (no counterpart in the
+ * reference): `steps` steps (10 is the intended count); even steps move each
+ * pixel's inverse depth, odd steps its normal, against the cost gradient by a
+ * per-pixel step length, keeping a move only when the pixel's cost drops
+ * (costs never increase).  Clears the slot's consistency result.  Async.
+ */
+RECON_MVS_API int32_t recon_mvs_hip_plane_polish(recon_mvs_hip_context* ctx, int32_t ref_view,
+                                                 int32_t num_src, const int32_t* src_views,
+                                                 const recon_mvs_plane_params* params, int32_t steps);
+
+/*
  * Kernel timers.  enable != 0 records HIP events around every launch of the
  * plane-mode kernels on the context's stream; ms / counts (8 entries each,
  * may be NULL) receive the accumulated milliseconds and launch counts per
  * kernel class: 0 sweep, 1 init, 2 refine, 3 consistency, 4 score, 5 eval,
- * 6 batch (one graph run).
+ * 6 batch (one graph run), 7 adjoint / polish.
  * reset != 0 clears the accumulators.
  */
 RECON_MVS_API int32_t recon_mvs_hip_timing(recon_mvs_hip_context* ctx, int32_t enable,

@@ -112,6 +112,10 @@ def _declare_plane(lib):
     lib.orc_plane_patchmatch.restype = C.c_int
     lib.orc_geo_consistency.argtypes = [C.c_int, _P(C.c_void_p), _cam_p, C.c_double, _P(C.c_void_p)]
     lib.orc_geo_consistency.restype = None
+    lib.orc_plane_cost_grad.argtypes = [C.c_int, _P(C.c_void_p), _cam_p, C.c_int, C.c_int,
+                                        C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                                        C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.orc_plane_cost_grad.restype = C.c_int
     lib.orc_rng_uniform.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]
     lib.orc_rng_uniform.restype = C.c_float
 
@@ -353,3 +357,54 @@ def geo_consistency(depths, cams, thr):
     outs = [np.zeros(d.shape, np.uint8) for d in ds]
     lib.orc_geo_consistency(len(ds), _ptr_array(ds), _cams(cams), thr, _ptr_array(outs))
     return outs
+
+
+def so3_exp(w):
+    """Rodrigues: rotation matrix of the rotation vector w (FP64)."""
+    w = np.asarray(w, np.float64)
+    th = float(np.linalg.norm(w))
+    K = np.array([[0.0, -w[2], w[1]], [w[2], 0.0, -w[0]], [-w[1], w[0], 0.0]])
+    if th < 1e-12:
+        return np.eye(3) + K + 0.5 * K @ K
+    return np.eye(3) + (np.sin(th) / th) * K + ((1.0 - np.cos(th)) / (th * th)) * (K @ K)
+
+
+def perturb_camera(cam, delta):
+    """Copy of `cam` with its world->camera transform multiplied on the left by
+    Exp(delta[:3]) and translated by delta[3:]: Y -> Exp(w) Y + tau."""
+    import copy
+    c = copy.copy(cam)
+    R = np.array(cam.R[:], np.float64).reshape(3, 3)
+    t = np.array(cam.t[:], np.float64)
+    E = so3_exp(delta[:3])
+    R2 = E @ R
+    t2 = E @ t + np.asarray(delta[3:], np.float64)
+    for i in range(9):
+        c.R[i] = float(R2.reshape(-1)[i])
+    for i in range(3):
+        c.t[i] = float(t2[i])
+    return c
+
+
+def plane_cost_grad(images, cams, ref, srcs, params, xy, planes, weights=None, grad=True,
+                    pose_grad=True):
+    """FP64 point-form cost of planes (n x 4: D, nx, ny, nz, float64) at pixels xy
+    (n x 2 int).  Returns (cost, nvalid, grad_plane n x 4 or None, grad_pose
+    nsrc x 6 or None); gradients are of sum(weights * cost)."""
+    lib = restated()
+    imgs, ptrs, cm = _views(images, cams)
+    srcs = np.ascontiguousarray(srcs, np.int32)
+    xy = np.ascontiguousarray(xy, np.int32).reshape(-1, 2)
+    planes = np.ascontiguousarray(planes, np.float64).reshape(-1, 4)
+    n = xy.shape[0]
+    w = None if weights is None else np.ascontiguousarray(weights, np.float64)
+    cost = np.zeros(n); nv = np.zeros(n, np.int32)
+    gp = np.zeros((n, 4)) if grad else None
+    gs = np.zeros((len(srcs), 6)) if (grad and pose_grad) else None
+    ok = lib.orc_plane_cost_grad(len(imgs), ptrs, cm, ref, len(srcs), srcs.ctypes.data,
+                                 C.byref(params), n, xy.ctypes.data, planes.ctypes.data,
+                                 None if w is None else w.ctypes.data, cost.ctypes.data,
+                                 nv.ctypes.data, None if gp is None else gp.ctypes.data,
+                                 None if gs is None else gs.ctypes.data)
+    assert ok
+    return cost, nv, gp, gs

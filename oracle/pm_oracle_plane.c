@@ -57,6 +57,8 @@ typedef struct {
   const float* src[ORC_MAXV];
   int sw[ORC_MAXV], sh[ORC_MAXV];
   double A[ORC_MAXV][9], b[ORC_MAXV][3];
+  double Rrel[ORC_MAXV][9], trel[ORC_MAXV][3]; /* reference camera -> source camera */
+  double sfx[ORC_MAXV], sfy[ORC_MAXV], scx[ORC_MAXV], scy[ORC_MAXV];
   float min_depth, max_depth, inv_min, inv_max;
   double eps, photo_reject;
   uint32_t seed;
@@ -72,8 +74,7 @@ typedef struct {
 
 /* H = K_s R_rel K_r^-1, b = K_s t_rel with R_rel = R_s R_r^T, t_rel = t_s - R_rel t_r. */
 static void view_pair(const recon_mvs_camera* R, const recon_mvs_camera* S, double A[9],
-                      double b[3]) {
-  double Rrel[9], trel[3];
+                      double b[3], double Rrel[9], double trel[3]) {
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) {
       double s = 0.0;
@@ -134,7 +135,11 @@ static int make_args(plane_args* a, const float* const* imgs, const recon_mvs_ca
     a->src[k] = imgs[srcs[k]];
     a->sw[k] = S->width;
     a->sh[k] = S->height;
-    view_pair(R, S, a->A[k], a->b[k]);
+    view_pair(R, S, a->A[k], a->b[k], a->Rrel[k], a->trel[k]);
+    a->sfx[k] = S->fx;
+    a->sfy[k] = S->fy;
+    a->scx[k] = S->cx;
+    a->scy[k] = S->cy;
   }
   a->min_depth = (float)p->min_depth_m;
   a->max_depth = (float)p->max_depth_m;
@@ -458,6 +463,265 @@ int orc_plane_patchmatch(int nviews, const float* const* imgs, const recon_mvs_c
         support[idx] = (uint8_t)(scur < 255 ? scur : 255);
       }
     }
+  }
+  return 1;
+}
+
+/* ---- cost and adjoint in point form (FP64) ----
+   The sample at window offset (dx, dy) of pixel (x, y) lies on the ray
+   r_i = K_r^-1 (x + 0.5 + dx, y + 0.5 + dy, 1) at X_i = lambda_i r_i, with
+   lambda_i = D (n.r_c) / (n.r_i) (the plane through D r_c with normal n).  In
+   the source camera Y_i = R_rel X_i + t_rel, and u = f_s Y_x / Y_z + c_s.  This
+   is the map of plane_cost's homography, written independently of it.
+   Source poses are perturbed on the left: Y -> Exp(w) Y + tau, so
+   dY/dw = -[Y]x and dY/dtau = I.  Validity rules are those of plane_cost. */
+
+/* Bilinear sample as sample_f, plus its derivatives in u and v (one-sided at
+   the right and bottom edges, where x1 = x0 or y1 = y0). */
+static int sample_fd(const float* img, int w, int h, double u, double v, double* out,
+                     double* du, double* dv) {
+  if (u < 0.0 || v < 0.0 || u > (double)(w - 1) || v > (double)(h - 1)) return 0;
+  const int x0 = (int)floor(u), y0 = (int)floor(v);
+  const int x1 = x0 + 1 < w - 1 ? x0 + 1 : w - 1;
+  const int y1 = y0 + 1 < h - 1 ? y0 + 1 : h - 1;
+  const double fx = u - x0, fy = v - y0;
+  const double p00 = img[y0 * w + x0], p10 = img[y0 * w + x1];
+  const double p01 = img[y1 * w + x0], p11 = img[y1 * w + x1];
+  const double top = p00 * (1.0 - fx) + p10 * fx;
+  const double bot = p01 * (1.0 - fx) + p11 * fx;
+  *out = top * (1.0 - fy) + bot * fy;
+  *du = (x1 > x0 ? 1.0 : 0.0) * ((p10 - p00) * (1.0 - fy) + (p11 - p01) * fy);
+  *dv = (y1 > y0 ? 1.0 : 0.0) * (bot - top);
+  return 1;
+}
+
+typedef struct {
+  double rv, sv, du, dv;  /* reference value, source value, d(source)/du, d(source)/dv */
+  double r[3], nr;        /* ray r_i and n . r_i */
+  double hv[3];           /* Y_i / lambda_i */
+} orc_sample;
+
+/* Samples of view v for the plane (D, n) at pixel (x, y); returns the count,
+   or -1 when the view is rejected before sampling. */
+static int view_samples(const plane_args* a, int v, int x, int y, double D, const double n[3],
+                        double rho, orc_sample* out) {
+  const double* Rr = a->Rrel[v];
+  const double* tr = a->trel[v];
+  const double rc[3] = {(x + 0.5 - a->cx) / a->fx, (y + 0.5 - a->cy) / a->fy, 1.0};
+  /* centre: Y_c / D */
+  double hc[3];
+  for (int i = 0; i < 3; ++i)
+    hc[i] = Rr[3 * i] * rc[0] + Rr[3 * i + 1] * rc[1] + Rr[3 * i + 2] * rc[2] + tr[i] / D;
+  if (!(hc[2] * D > 1e-6)) return -1;
+  const double uc = a->sfx[v] * hc[0] / hc[2] + a->scx[v];
+  const double vc = a->sfy[v] * hc[1] / hc[2] + a->scy[v];
+  if (!(fabs(uc - 0.5) < 1e7 && fabs(vc - 0.5) < 1e7)) return -1;
+  const int S1 = 2 * a->half / a->stride + 1;
+  int cnt = 0;
+  for (int j = 0; j < S1; ++j) {
+    for (int i = 0; i < S1; ++i) {
+      const int dx = -a->half + i * a->stride, dy = -a->half + j * a->stride;
+      const int gx = x + dx, gy = y + dy;
+      if (gx < 0 || gy < 0 || gx >= a->w || gy >= a->h) continue;
+      orc_sample* sm = &out[cnt];
+      sm->r[0] = (x + 0.5 + dx - a->cx) / a->fx;
+      sm->r[1] = (y + 0.5 + dy - a->cy) / a->fy;
+      sm->r[2] = 1.0;
+      sm->nr = n[0] * sm->r[0] + n[1] * sm->r[1] + n[2] * sm->r[2];
+      const double q = sm->nr / rho;
+      for (int k = 0; k < 3; ++k)
+        sm->hv[k] = Rr[3 * k] * sm->r[0] + Rr[3 * k + 1] * sm->r[1] + Rr[3 * k + 2] * sm->r[2] + tr[k] * q;
+      if (!(sm->hv[2] > 0.0)) continue;
+      const double u = a->sfx[v] * sm->hv[0] / sm->hv[2] + a->scx[v];
+      const double vv = a->sfy[v] * sm->hv[1] / sm->hv[2] + a->scy[v];
+      if (!sample_fd(a->src[v], a->sw[v], a->sh[v], u - 0.5, vv - 0.5, &sm->sv, &sm->du, &sm->dv))
+        continue;
+      sm->rv = a->ref[gy * a->w + gx];
+      ++cnt;
+    }
+  }
+  return cnt;
+}
+
+typedef struct {
+  double ncc, raw, mr, ms, sr, ss; /* clamped and raw NCC, means, standard deviations */
+  int n;
+} orc_stats;
+
+static int view_stats(const plane_args* a, const orc_sample* sm, int cnt, orc_stats* st) {
+  const int S1 = 2 * a->half / a->stride + 1;
+  const int S = S1 * S1;
+  if (cnt < S / 2 || cnt == 0) return 0;
+  double sr = 0, ss = 0;
+  for (int k = 0; k < cnt; ++k) {
+    sr += sm[k].rv;
+    ss += sm[k].sv;
+  }
+  const double mr = sr / cnt, ms = ss / cnt;
+  double srr = 0, sss = 0, srs = 0;
+  for (int k = 0; k < cnt; ++k) {
+    const double a1 = sm[k].rv - mr, b1 = sm[k].sv - ms;
+    srr += a1 * a1;
+    sss += b1 * b1;
+    srs += a1 * b1;
+  }
+  const double var_r = srr / cnt, var_s = sss / cnt, cov = srs / cnt;
+  const double denom = sqrt(var_r * var_s);
+  if (!(denom >= a->eps) || !(denom > 0.0)) return 0;
+  st->raw = cov / denom;
+  st->ncc = st->raw < -1.0 ? -1.0 : (st->raw > 1.0 ? 1.0 : st->raw);
+  st->mr = mr;
+  st->ms = ms;
+  st->sr = sqrt(var_r);
+  st->ss = sqrt(var_s);
+  st->n = cnt;
+  return 1;
+}
+
+#define ORC_MAX_SAMPLES 1024
+
+/* Cost of plane (D, n) at (x, y) and, when g_plane is non-NULL, the gradient
+   of wgt * cost w.r.t. (D, n) (into g_plane) and the source poses (added to
+   g_pose[v][0..5] = (w, tau)).  Invalid or rejected: ORC_COST_INF, zero
+   gradient. */
+static double point_cost_grad(const plane_args* a, int x, int y, double D, const double n[3],
+                              double wgt, int* nvalid_out, double g_plane[4], double (*g_pose)[6],
+                              orc_sample* sm) {
+  *nvalid_out = 0;
+  if (g_plane) g_plane[0] = g_plane[1] = g_plane[2] = g_plane[3] = 0.0;
+  if (!(D >= (double)a->min_depth && D <= (double)a->max_depth)) return ORC_COST_INF;
+  const double rc[3] = {(x + 0.5 - a->cx) / a->fx, (y + 0.5 - a->cy) / a->fy, 1.0};
+  const double nd = n[0] * rc[0] + n[1] * rc[1] + n[2] * rc[2];
+  if (!(nd < 0.0)) return ORC_COST_INF;
+  const double rho = D * nd;
+  double cst[ORC_MAXV];
+  int vid[ORC_MAXV];
+  int nvalid = 0;
+  for (int v = 0; v < a->nsrc; ++v) {
+    const int cnt = view_samples(a, v, x, y, D, n, rho, sm);
+    orc_stats st;
+    if (cnt < 0 || !view_stats(a, sm, cnt, &st)) continue;
+    cst[nvalid] = 1.0 - st.ncc;
+    vid[nvalid] = v;
+    ++nvalid;
+  }
+  *nvalid_out = nvalid;
+  if (nvalid == 0) return ORC_COST_INF;
+  /* stable sort of (cost, view) by cost */
+  for (int i = 1; i < nvalid; ++i) {
+    const double t = cst[i];
+    const int tv = vid[i];
+    int j = i - 1;
+    while (j >= 0 && cst[j] > t) {
+      cst[j + 1] = cst[j];
+      vid[j + 1] = vid[j];
+      --j;
+    }
+    cst[j + 1] = t;
+    vid[j + 1] = tv;
+  }
+  const int kk = a->top_k < nvalid ? a->top_k : nvalid;
+  double sum = 0.0;
+  for (int k = 0; k < kk; ++k) sum += cst[k];
+  const double c = sum / kk;
+  if (c > a->photo_reject) return ORC_COST_INF;
+  if (!g_plane) return c;
+  double gD = 0.0, gn[3] = {0.0, 0.0, 0.0};
+  for (int k = 0; k < kk; ++k) {
+    const int v = vid[k];
+    const int cnt = view_samples(a, v, x, y, D, n, rho, sm);
+    orc_stats st;
+    if (cnt < 0 || !view_stats(a, sm, cnt, &st)) continue;
+    if (st.raw < -1.0 || st.raw > 1.0) continue; /* clamped: flat */
+    const double dc = -wgt / kk;                  /* d(wgt cost) / d ncc_v */
+    const double al = 1.0 / (st.n * st.sr * st.ss), be = st.ncc / (st.n * st.ss * st.ss);
+    const double* Rr = a->Rrel[v];
+    for (int q = 0; q < cnt; ++q) {
+      const orc_sample* s = &sm[q];
+      /* d ncc / d s_i = (r~_i / (sr ss) - ncc s~_i / ss^2) / N */
+      const double gs = dc * ((s->rv - st.mr) * al - (s->sv - st.ms) * be);
+      const double gu = gs * s->du, gv = gs * s->dv;
+      const double lam = rho / s->nr;
+      const double Y[3] = {lam * s->hv[0], lam * s->hv[1], lam * s->hv[2]};
+      const double gY[3] = {gu * a->sfx[v] / Y[2], gv * a->sfy[v] / Y[2],
+                            -(gu * a->sfx[v] * Y[0] + gv * a->sfy[v] * Y[1]) / (Y[2] * Y[2])};
+      if (g_pose) {
+        g_pose[v][0] += Y[1] * gY[2] - Y[2] * gY[1];
+        g_pose[v][1] += Y[2] * gY[0] - Y[0] * gY[2];
+        g_pose[v][2] += Y[0] * gY[1] - Y[1] * gY[0];
+        g_pose[v][3] += gY[0];
+        g_pose[v][4] += gY[1];
+        g_pose[v][5] += gY[2];
+      }
+      double gX[3];
+      for (int i = 0; i < 3; ++i) gX[i] = Rr[i] * gY[0] + Rr[3 + i] * gY[1] + Rr[6 + i] * gY[2];
+      const double ai = gX[0] * s->r[0] + gX[1] * s->r[1] + gX[2] * s->r[2];
+      gD += ai * lam / D;
+      const double f = ai / s->nr;
+      for (int i = 0; i < 3; ++i) gn[i] += f * (D * rc[i] - lam * s->r[i]);
+    }
+  }
+  g_plane[0] = gD;
+  g_plane[1] = gn[0];
+  g_plane[2] = gn[1];
+  g_plane[3] = gn[2];
+  return c;
+}
+
+/* Exported: costs of n planes (FP64 inputs) at pixels xy, in point form; with
+   grad_plane non-NULL also the gradients of sum_q weights[q] * cost[q] w.r.t.
+   each plane (n x 4: D, n) and, with grad_pose non-NULL, w.r.t. the source
+   poses (nsrc x 6: rotation w, translation tau; left perturbation).  Invalid
+   planes: cost ORC_COST_INF, zero gradient.  The pose sum is reduced in a
+   fixed order (chunks of 64 planes, chunk partials in chunk order). */
+int orc_plane_cost_grad(int nviews, const float* const* imgs, const recon_mvs_camera* cams,
+                        int ref, int nsrc, const int32_t* srcs, const orc_plane_params* p, int n,
+                        const int32_t* xy, const double* planes, const double* weights,
+                        double* cost, int32_t* nvalid, double* grad_plane, double* grad_pose) {
+  (void)nviews;
+  plane_args a;
+  if (!make_args(&a, imgs, cams, ref, nsrc, srcs, p, 0.5)) return 0;
+  const int S1 = 2 * a.half / a.stride + 1;
+  if (S1 * S1 > ORC_MAX_SAMPLES) return 0;
+  const int chunk = 64;
+  const int nchunks = (n + chunk - 1) / chunk;
+  double(*part)[ORC_MAXV][6] = NULL;
+  if (grad_pose) {
+    part = calloc((size_t)(nchunks > 0 ? nchunks : 1), sizeof(*part));
+    if (!part) return 0;
+  }
+#pragma omp parallel
+  {
+    orc_sample* sm = malloc(sizeof(orc_sample) * ORC_MAX_SAMPLES);
+#pragma omp for schedule(dynamic, 1)
+    for (int ck = 0; ck < nchunks; ++ck) {
+      for (int q = ck * chunk; q < n && q < (ck + 1) * chunk; ++q) {
+        const int x = xy[2 * q], y = xy[2 * q + 1];
+        int nv = 0;
+        double g[4] = {0, 0, 0, 0};
+        if (x < 0 || y < 0 || x >= a.w || y >= a.h || !sm) {
+          cost[q] = ORC_COST_INF;
+        } else {
+          const double nn[3] = {planes[4 * q + 1], planes[4 * q + 2], planes[4 * q + 3]};
+          const double w = weights ? weights[q] : 1.0;
+          cost[q] = point_cost_grad(&a, x, y, planes[4 * q], nn, w, &nv, grad_plane ? g : NULL,
+                                    part ? part[ck] : NULL, sm);
+        }
+        if (nvalid) nvalid[q] = nv;
+        if (grad_plane)
+          for (int k = 0; k < 4; ++k) grad_plane[4 * q + k] = g[k];
+      }
+    }
+    free(sm);
+  }
+  if (grad_pose) {
+    for (int v = 0; v < nsrc; ++v)
+      for (int k = 0; k < 6; ++k) {
+        double s = 0.0;
+        for (int ck = 0; ck < nchunks; ++ck) s += part[ck][v][k];
+        grad_pose[6 * v + k] = s;
+      }
+    free(part);
   }
   return 1;
 }

@@ -34,6 +34,7 @@ HIP_SOURCES = [
     ("ref_mode.hip", ["-ffp-contract=off"]),
     ("fuse.hip", ["-ffp-contract=off"]),
     ("plane_mode.hip", ["-ffp-contract=off"]),
+    ("plane_adjoint.hip", ["-ffp-contract=off"]),
     ("mvs_api.cpp", ["-ffp-contract=off"]),
 ]
 

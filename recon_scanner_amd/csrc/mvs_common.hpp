@@ -63,6 +63,9 @@ enum Slot : int {
   kSlotFuse7,
   kSlotFuse8,
   kSlotFuse9,
+  kSlotAdjIn,
+  kSlotAdjOut,
+  kSlotAdjPart,
   kSlotCount
 };
 

@@ -1,0 +1,530 @@
+// plane_common.hpp — device pieces of plane mode shared by the sweep kernels
+// (plane_mode.hip) and the adjoint kernels (plane_adjoint.hip): the resident
+// view and depth-map slots, the counter RNG, the kernel arguments, pixel
+// access, the LDS reference tile and window, and the FP32 plane cost.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "mvs_common.hpp"
+
+constexpr int kMaxViews = 16;
+constexpr int kMaxTopK = 16;
+// Tile schedules (pixels per workgroup tile, 256 threads): 0 = 64 x 8 (the
+// default), 1 = 32 x 16, 2 = 128 x 4.  A wave covers 64 / (TW / 2) rows.
+constexpr int kNumSchedules = 3;
+constexpr float kCostInf = 1e30f;
+
+struct ResidentView {
+  int w = 0, h = 0, pitch = 0;
+  int format = RECON_MVS_PIXEL_F32;
+  void* pixels = nullptr;   // (h + 1) rows x pitch, zero padded right and below
+  recon_mvs_camera cam{};
+};
+
+struct PlaneState {
+  int w = 0, h = 0;
+  float4* plane = nullptr;  // (depth, nx, ny, nz)
+  float* cost = nullptr;
+  uint8_t* support = nullptr;
+  uint8_t* geo_support = nullptr;
+  bool has_geo = false;
+};
+
+namespace rmvs {
+
+// ---------------------------------------------------------------------------
+// Counter RNG: lowbias32 (shaders/common/gpu_algo.h:150-154, gpu_common.h:239-244)
+// ---------------------------------------------------------------------------
+RMVS_HD uint32_t Lowbias32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352dU;
+  x ^= x >> 15;
+  x *= 0x846ca68bU;
+  x ^= x >> 16;
+  return x;
+}
+RMVS_HD uint32_t RngKey(uint32_t seed, uint32_t pass, uint32_t pixel) {
+  return Lowbias32(Lowbias32(Lowbias32(seed) ^ pass) ^ pixel);
+}
+// draw k of a key, uniform in [0, 1) with 24 random bits (exact in float)
+RMVS_HD float RngUniform(uint32_t key, uint32_t k) {
+  return static_cast<float>(Lowbias32(key ^ ((k + 1u) * 0x9E3779B9u)) >> 8) * (1.0f / 16777216.0f);
+}
+
+// ---------------------------------------------------------------------------
+// Kernel arguments
+// ---------------------------------------------------------------------------
+struct PlaneArgs {
+  int w, h, ref_pitch;
+  const void* ref_img;
+  double fx, fy, cx, cy;        // reference intrinsics
+  float ffx, ffy, fcx, fcy;     // same, FP32
+  int nsrc, top_k;
+  const void* src_img[kMaxViews];
+  int src_pitch[kMaxViews], src_w[kMaxViews], src_h[kMaxViews];
+  double A[kMaxViews][9];       // K_s R_rel K_r^-1 (row-major)
+  double b[kMaxViews][3];       // K_s t_rel
+  float4* plane;
+  float* cost;
+  uint8_t* support;
+  float min_depth, max_depth, inv_min, inv_max;
+  float eps2;                   // (NCC denominator floor)^2
+  float photo_reject;           // 3 x photo_threshold
+  uint32_t seed;
+  int color, iter, reach;
+  float depth_perturb, normal_perturb;
+  int tiles_x, tiles_y, tiles_per_xcd, num_tiles;
+};
+
+template <int FMT>
+__device__ __forceinline__ float Pix(const void* base, int i) {
+  if constexpr (FMT == RECON_MVS_PIXEL_F32) {
+    return static_cast<const float*>(base)[i];
+  } else if constexpr (FMT == RECON_MVS_PIXEL_F16) {
+    return static_cast<float>(static_cast<const _Float16*>(base)[i]);
+  } else {
+    return __uint_as_float(static_cast<uint32_t>(static_cast<const uint16_t*>(base)[i]) << 16);
+  }
+}
+
+// Pixels i and i + 1 of one row in one load (8 bytes for FP32, 4 for FP16 /
+// BF16; unaligned global access is enabled on gfx950 under HSA).
+// Pixels at byte offset `off` and the next one, through a raw buffer
+// descriptor (32-bit offsets; `soff` is a wave-uniform extra byte offset).
+template <int FMT>
+__device__ __forceinline__ float2 BufPair(__amdgpu_buffer_rsrc_t rsrc, int off, int soff) {
+  if constexpr (FMT == RECON_MVS_PIXEL_F32) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b64(rsrc, off, soff, 0);
+    return make_float2(__uint_as_float(v[0]), __uint_as_float(v[1]));
+  } else {
+    const uint32_t u = __builtin_amdgcn_raw_buffer_load_b32(rsrc, off, soff, 0);
+    if constexpr (FMT == RECON_MVS_PIXEL_F16) {
+      const _Float16 lo = __builtin_bit_cast(_Float16, static_cast<uint16_t>(u & 0xFFFFu));
+      const _Float16 hi = __builtin_bit_cast(_Float16, static_cast<uint16_t>(u >> 16));
+      return make_float2(static_cast<float>(lo), static_cast<float>(hi));
+    } else {
+      return make_float2(__uint_as_float(u << 16), __uint_as_float(u & 0xFFFF0000u));
+    }
+  }
+}
+
+template <int FMT>
+__device__ __forceinline__ float2 PixPair(const void* base, int i) {
+  if constexpr (FMT == RECON_MVS_PIXEL_F32) {
+    float2 v;
+    __builtin_memcpy(&v, static_cast<const float*>(base) + i, 8);
+    return v;
+  } else {
+    uint32_t u;
+    __builtin_memcpy(&u, static_cast<const uint16_t*>(base) + i, 4);
+    if constexpr (FMT == RECON_MVS_PIXEL_F16) {
+      const _Float16 lo = __builtin_bit_cast(_Float16, static_cast<uint16_t>(u & 0xFFFFu));
+      const _Float16 hi = __builtin_bit_cast(_Float16, static_cast<uint16_t>(u >> 16));
+      return make_float2(static_cast<float>(lo), static_cast<float>(hi));
+    } else {
+      return make_float2(__uint_as_float(u << 16), __uint_as_float(u & 0xFFFF0000u));
+    }
+  }
+}
+
+// Reference window of one pixel: a pointer to its top-left sample in LDS
+// (row step rs floats, column step STRIDE), NaN marking samples outside the
+// image, plus the mean and the centred sums over the samples inside.
+template <int HALF, int STRIDE>
+struct RefWindow {
+  static constexpr int NS = 2 * HALF / STRIDE + 1;
+  static constexpr int S = NS * NS;
+  const float* base;  // LDS values are pre-shifted by `shift`
+  int rs;
+  float sum_r, sum_r2, n, mean, shift;
+  bool full;
+  __device__ __forceinline__ float at(int i, int j) const { return base[j * rs + i * STRIDE]; }
+};
+
+template <int HALF, int STRIDE>
+__device__ __forceinline__ void MakeRefWindow(RefWindow<HALF, STRIDE>& win, const float* base, int rs,
+                                              float shift) {
+  constexpr int NS = RefWindow<HALF, STRIDE>::NS;
+  win.base = base;
+  win.rs = rs;
+  win.shift = shift;
+  float s = 0.0f, cnt = 0.0f;
+  for (int j = 0; j < NS; ++j) {
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+      const float v = win.at(i, j);
+      const bool ok = (v == v);
+      s += ok ? v : 0.0f;
+      cnt += ok ? 1.0f : 0.0f;
+    }
+  }
+  const float mean = cnt > 0.0f ? s / cnt : 0.0f;
+  float sr = 0.0f, sr2 = 0.0f;
+  for (int j = 0; j < NS; ++j) {
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+      const float v0 = win.at(i, j);
+      const float v = (v0 == v0) ? v0 : 0.0f;
+      sr += v;
+      sr2 = fmaf(v, v, sr2);
+    }
+  }
+  win.sum_r = sr;
+  win.sum_r2 = sr2;
+  win.n = cnt;
+  win.mean = mean;
+  win.full = (cnt == static_cast<float>(RefWindow<HALF, STRIDE>::S));
+}
+
+struct Hyp {
+  float d, nx, ny, nz;
+};
+
+constexpr int kMaxTopKNet = 8;  // top_k below the view count is limited to 8
+
+// Aggregated plane cost of hypothesis h at pixel (x, y).  Returns kCostInf
+// when no view is valid or the hypothesis is rejected.  view_ncc (optional)
+// receives NaN for invalid views.
+template <int HALF, int STRIDE, int FMT>
+__device__ float PlaneCost(const PlaneArgs& a, int x, int y, const RefWindow<HALF, STRIDE>& win,
+                           const Hyp& h, int* valid_views_out, float* view_ncc) {
+  constexpr int NS = RefWindow<HALF, STRIDE>::NS;
+  constexpr int S = RefWindow<HALF, STRIDE>::S;
+  *valid_views_out = 0;
+  if (!(h.d >= a.min_depth && h.d <= a.max_depth)) {
+    if (view_ncc)
+      for (int v = 0; v < a.nsrc; ++v) view_ncc[v] = __builtin_nanf("");
+    return kCostInf;
+  }
+  // FP64: plane -> m = K_r^-T n / rho
+  const double qx = x + 0.5, qy = y + 0.5;
+  const double rx = (qx - a.cx) / a.fx, ry = (qy - a.cy) / a.fy;
+  const double nx = h.nx, ny = h.ny, nz = h.nz, D = h.d;
+  const double nd = nx * rx + ny * ry + nz;
+  const bool use_topk = a.top_k < a.nsrc;
+  float topk[kMaxTopKNet];
+#pragma unroll
+  for (int k = 0; k < kMaxTopKNet; ++k) topk[k] = kCostInf;
+  float csum = 0.0f;
+  int nvalid = 0;
+  if (nd < 0.0) {
+    const double rho = D * nd;
+    const double m0 = nx / a.fx / rho;
+    const double m1 = ny / a.fy / rho;
+    const double m2 = (nz - nx * a.cx / a.fx - ny * a.cy / a.fy) / rho;
+    for (int v = 0; v < a.nsrc; ++v) {
+      float ncc_v = __builtin_nanf("");
+      // H = A + b m^T, centre projection
+      double H[9];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        H[3 * i + 0] = fma(a.b[v][i], m0, a.A[v][3 * i + 0]);
+        H[3 * i + 1] = fma(a.b[v][i], m1, a.A[v][3 * i + 1]);
+        H[3 * i + 2] = fma(a.b[v][i], m2, a.A[v][3 * i + 2]);
+      }
+      const double hc0 = fma(H[0], qx, fma(H[1], qy, H[2]));
+      const double hc1 = fma(H[3], qx, fma(H[4], qy, H[5]));
+      const double hc2 = fma(H[6], qx, fma(H[7], qy, H[8]));
+      if (hc2 * D > 1e-6) {
+        const double ih = 1.0 / hc2;
+        const double uc = hc0 * ih, vc = hc1 * ih;       // continuous source coords
+        const double su = uc - 0.5, sv = vc - 0.5;       // sampler coords (pixel centres at integers)
+        if (fabs(su) < 1e7 && fabs(sv) < 1e7) {
+          const double fsu = floor(su), fsv = floor(sv);
+          const int iu = static_cast<int>(fsu), iv = static_cast<int>(fsv);
+          const float fu = static_cast<float>(su - fsu), fv = static_cast<float>(sv - fsv);
+          // Offset of a sample from the centre projection, for a window offset
+          // (dx, dy): du = (g00 dx + g01 dy) / (1 + h20 dx + h21 dy), and dv
+          // likewise with g10, g11; g = (H_row - u_c H_2) / hc2 in FP64.
+          const float g00 = static_cast<float>((H[0] - uc * H[6]) * ih);
+          const float g01 = static_cast<float>((H[1] - uc * H[7]) * ih);
+          const float g10 = static_cast<float>((H[3] - vc * H[6]) * ih);
+          const float g11 = static_cast<float>((H[4] - vc * H[7]) * ih);
+          const float h20 = static_cast<float>(H[6] * ih), h21 = static_cast<float>(H[7] * ih);
+          const int w = a.src_w[v], hgt = a.src_h[v], pitch = a.src_pitch[v];
+          const float lox = static_cast<float>(-iu), hix = static_cast<float>(w - 1 - iu);
+          const float loy = static_cast<float>(-iv), hiy = static_cast<float>(hgt - 1 - iv);
+          constexpr int kEsz = FMT == RECON_MVS_PIXEL_F32 ? 4 : 2;
+          const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+              const_cast<void*>(a.src_img[v]), 0, pitch * (hgt + 1) * kEsz, 0x00020000);
+          const int row_bytes = pitch * kEsz;
+          // the window origin's row/column in the source, as a float index
+          // offset (exact: image sizes stay below 2^24 pixels)
+          const float fpitch = static_cast<float>(pitch);
+          const float obase = static_cast<float>(iv) * fpitch + static_cast<float>(iu);
+          const float shift = win.shift;
+          // A window entirely beyond one image edge cannot give a valid view:
+          // skip it.  Over the window |g0. dx + g0. dy| <= e0 (e1 for v) and
+          // the denominator is >= 1 - e2, so |du| <= e0 / (1 - e2).
+          const float e0 = HALF * (fabsf(g00) + fabsf(g01)), e1 = HALF * (fabsf(g10) + fabsf(g11));
+          const float e2 = HALF * (fabsf(h20) + fabsf(h21));
+          const float rd = __builtin_amdgcn_rcpf(1.0f - e2);
+          const float bu = e0 * rd + 1e-3f;
+          const float bv = e1 * rd + 1e-3f;
+          // e2 < 0.5 keeps every sample's denominator positive
+          const bool den_ok = e2 < 0.5f;
+          const bool outside = den_ok && (fu + bu < lox || fu - bu > hix || fv + bv < loy || fv - bv > hiy);
+          // Rows of the window map to segments (projective maps keep them
+          // convex while the denominator stays positive): a row whose two end
+          // samples are inside the image with a 1e-4 px margin is inside.
+          const bool rows_checkable = den_ok && win.full;
+          const float mlox = lox + 1e-4f, mhix = hix - 1e-4f, mloy = loy + 1e-4f, mhiy = hiy - 1e-4f;
+          // Sums over the source samples; the reference sums start from the
+          // whole window and lose the reference values of excluded samples.
+          float ss = 0.0f, sss = 0.0f, srs = 0.0f, xr = 0.0f, xrr = 0.0f, xcnt = 0.0f;
+          if (!outside) {
+            // Each window row in three phases (positions, all loads,
+            // arithmetic): a wave keeps 2 x NS loads in flight.  Loads need
+            // no clamping: out-of-range buffer reads return 0 and the
+            // sample is masked out.
+#pragma unroll 1
+            for (int j = 0; j < NS; ++j) {
+              const float dy = static_cast<float>(-HALF + j * STRIDE);
+              const float b0 = g01 * dy, b1 = g11 * dy, b2 = fmaf(h21, dy, 1.0f);
+              float txs[NS], tys[NS], dens[NS];
+#pragma unroll
+              for (int i = 0; i < NS; ++i) {
+                const float dx = static_cast<float>(-HALF + i * STRIDE);
+                dens[i] = fmaf(h20, dx, b2);
+                const float r = __builtin_amdgcn_rcpf(dens[i]);
+                txs[i] = fmaf(fmaf(g00, dx, b0), r, fu);
+                tys[i] = fmaf(fmaf(g10, dx, b1), r, fv);
+              }
+              // (bitwise & on purpose: no short-circuit branches)
+              const bool row_ok = rows_checkable & (txs[0] >= mlox) & (txs[0] <= mhix) &
+                                  (tys[0] >= mloy) & (tys[0] <= mhiy) & (txs[NS - 1] >= mlox) &
+                                  (txs[NS - 1] <= mhix) & (tys[NS - 1] >= mloy) & (tys[NS - 1] <= mhiy);
+              int off[NS];
+              float fxs[NS], fys[NS];
+#pragma unroll
+              for (int i = 0; i < NS; ++i) {
+                const float flx = floorf(txs[i]), fly = floorf(tys[i]);
+                fxs[i] = txs[i] - flx;
+                fys[i] = tys[i] - fly;
+                off[i] = static_cast<int>(fmaf(fly, fpitch, flx + obase)) * kEsz;
+              }
+              float2 p0[NS], p1[NS];
+#pragma unroll
+              for (int i = 0; i < NS; ++i) {
+                p0[i] = BufPair<FMT>(rsrc, off[i], 0);
+                p1[i] = BufPair<FMT>(rsrc, off[i], row_bytes);
+              }
+              float sv[NS];
+#pragma unroll
+              for (int i = 0; i < NS; ++i) {
+                const float top = fmaf(fxs[i], p0[i].y - p0[i].x, p0[i].x);
+                const float bot = fmaf(fxs[i], p1[i].y - p1[i].x, p1[i].x);
+                sv[i] = fmaf(fys[i], bot - top, top) - shift;
+              }
+              if (__ballot(!row_ok) == 0) {
+                // every lane's row is inside both images
+#pragma unroll
+                for (int i = 0; i < NS; ++i) {
+                  ss += sv[i];
+                  sss = fmaf(sv[i], sv[i], sss);
+                  srs = fmaf(win.at(i, j), sv[i], srs);
+                }
+              } else {
+#pragma unroll
+                for (int i = 0; i < NS; ++i) {
+                  const float rv = win.at(i, j);
+                  const bool rin = rv == rv;  // NaN: outside the reference image
+                  const bool ok = row_ok | (rin & (dens[i] > 0.0f) & (txs[i] >= lox) &
+                                            (txs[i] <= hix) & (tys[i] >= loy) & (tys[i] <= hiy));
+                  const float s = ok ? sv[i] : 0.0f;
+                  const float x = (ok || !rin) ? 0.0f : rv;
+                  ss += s;
+                  sss = fmaf(s, s, sss);
+                  srs = fmaf(ok ? rv : 0.0f, s, srs);
+                  xr += x;
+                  xrr = fmaf(x, x, xrr);
+                  xcnt += (ok || !rin) ? 0.0f : 1.0f;
+                }
+              }
+            }
+          } else {
+            xcnt = win.n;
+          }
+          const float cnt = win.n - xcnt;
+          const float sr = win.sum_r - xr, srr = win.sum_r2 - xrr;
+          if (cnt >= static_cast<float>(S / 2) && cnt > 0.0f) {
+            const float in = __builtin_amdgcn_rcpf(cnt);
+            const float mr = sr * in, ms = ss * in;
+            const float vr = fmaf(srr, in, -mr * mr);
+            const float vs = fmaf(sss, in, -ms * ms);
+            const float cv = fmaf(srs, in, -mr * ms);
+            const float d2 = vr * vs;
+            if (d2 >= a.eps2 && d2 > 0.0f) {
+              float ncc = cv * __builtin_amdgcn_rsqf(d2);
+              ncc = fminf(fmaxf(ncc, -1.0f), 1.0f);
+              ncc_v = ncc;
+              float c = 1.0f - ncc;
+              ++nvalid;
+              csum += c;
+              if (use_topk) {
+                // insert into the sorted k smallest
+#pragma unroll
+                for (int k = 0; k < kMaxTopKNet; ++k) {
+                  const float lo = fminf(topk[k], c);
+                  c = fmaxf(topk[k], c);
+                  topk[k] = lo;
+                }
+              }
+            }
+          }
+        }
+      }
+      if (view_ncc) view_ncc[v] = ncc_v;
+    }
+  } else if (view_ncc) {
+    for (int v = 0; v < a.nsrc; ++v) view_ncc[v] = __builtin_nanf("");
+  }
+  *valid_views_out = nvalid;
+  if (nvalid == 0) return kCostInf;
+  float cost;
+  if (use_topk) {
+    const int kk = min(a.top_k, nvalid);
+    float sum = 0.0f;
+#pragma unroll
+    for (int k = 0; k < kMaxTopKNet; ++k) sum += (k < kk) ? topk[k] : 0.0f;
+    cost = sum / static_cast<float>(kk);
+  } else {
+    cost = csum / static_cast<float>(nvalid);
+  }
+  return cost > a.photo_reject ? kCostInf : cost;
+}
+
+// ---------------------------------------------------------------------------
+// Candidate planes (FP32, no contraction)
+// ---------------------------------------------------------------------------
+
+// Ray of pixel (x, y) through its centre, z = 1.
+__device__ __forceinline__ void PixelRay(const PlaneArgs& a, int x, int y, float* r) {
+  r[0] = (static_cast<float>(x) + 0.5f - a.fcx) / a.ffx;
+  r[1] = (static_cast<float>(y) + 0.5f - a.fcy) / a.ffy;
+  r[2] = 1.0f;
+}
+
+__device__ __forceinline__ bool Normalize3(float* n) {
+  const float l2 = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+  if (!(l2 > 1e-12f)) return false;
+  const float l = sqrtf(l2);
+  n[0] = n[0] / l;
+  n[1] = n[1] / l;
+  n[2] = n[2] / l;
+  return true;
+}
+
+// Random plane: inverse depth uniform in [inv_min, inv_max]; normal from a
+// uniform cube sample, normalised and turned towards the camera.
+__device__ Hyp RandomPlane(const PlaneArgs& a, uint32_t key, uint32_t k0, const float* ray) {
+  const float u0 = RngUniform(key, k0);
+  const float inv = a.inv_min + u0 * (a.inv_max - a.inv_min);
+  float n[3] = {2.0f * RngUniform(key, k0 + 1) - 1.0f, 2.0f * RngUniform(key, k0 + 2) - 1.0f,
+                2.0f * RngUniform(key, k0 + 3) - 1.0f};
+  if (!Normalize3(n)) {
+    n[0] = 0.0f;
+    n[1] = 0.0f;
+    n[2] = -1.0f;
+  }
+  const float dp = n[0] * ray[0] + n[1] * ray[1] + n[2] * ray[2];
+  if (dp > 0.0f) {
+    n[0] = -n[0];
+    n[1] = -n[1];
+    n[2] = -n[2];
+  }
+  return Hyp{1.0f / inv, n[0], n[1], n[2]};
+}
+
+}  // namespace rmvs
+
+using namespace rmvs;
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+
+// Tile index -> (tile x, tile y), XCD-aware: workgroups are dealt round-robin
+// to the 8 XCDs, so consecutive spatial tiles are placed on one XCD (shared
+// L2 for their overlapping source footprints).
+__device__ __forceinline__ bool TileOf(const PlaneArgs& a, int* tx, int* ty) {
+  const int b = blockIdx.x;
+  const int t = (b & 7) * a.tiles_per_xcd + (b >> 3);
+  if (t >= a.num_tiles) return false;
+  *tx = t % a.tiles_x;
+  *ty = t / a.tiles_x;
+  return true;
+}
+
+// Reference tile of a workgroup: TW x TH pixels plus a HALF-pixel halo in
+// LDS, every value shifted by the tile's centre pixel (NaN outside the
+// image).  256 threads; a thread owns one pixel of each colour: thread t ->
+// row t / (TW/2), the (t % (TW/2))-th pixel of the colour in that row.
+template <int HALF, int FMT, int TW, int TH>
+struct RefTile {
+  static_assert(TW * TH == 512, "256 threads, two pixels each");
+  static constexpr int LW = TW + 2 * HALF + 1;  // +1: odd row pitch against bank conflicts
+  static constexpr int LH = TH + 2 * HALF;
+  float v[LH][LW];
+  float shift;
+
+  __device__ __forceinline__ void load(const PlaneArgs& a, int x0, int y0) {
+    if (threadIdx.x == 0) {
+      const int cx = min(x0 + TW / 2, a.w - 1), cy = min(y0 + TH / 2, a.h - 1);
+      shift = Pix<FMT>(a.ref_img, cy * a.ref_pitch + cx);
+    }
+    __syncthreads();
+    const float sh = shift;
+    for (int i = threadIdx.x; i < (LW - 1) * LH; i += blockDim.x) {
+      const int ly = i / (LW - 1), lx = i % (LW - 1);
+      const int gx = x0 - HALF + lx, gy = y0 - HALF + ly;
+      float val = __builtin_nanf("");
+      if (gx >= 0 && gy >= 0 && gx < a.w && gy < a.h) val = Pix<FMT>(a.ref_img, gy * a.ref_pitch + gx) - sh;
+      v[ly][lx] = val;
+    }
+  }
+  // pixel of colour `color` owned by this thread
+  __device__ __forceinline__ void pixel(int x0, int y0, int color, int* x, int* y, int* r) const {
+    *r = threadIdx.x / (TW / 2);
+    const int j = threadIdx.x % (TW / 2);
+    *y = y0 + *r;
+    *x = x0 + 2 * j + (((x0 + *y) & 1) ^ color);
+  }
+  template <int STRIDE>
+  __device__ __forceinline__ void window(RefWindow<HALF, STRIDE>& win, int x, int x0, int r) const {
+    MakeRefWindow<HALF, STRIDE>(win, &v[r][x - x0], STRIDE * LW, shift);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Host helpers (defined in plane_mode.hip)
+// ---------------------------------------------------------------------------
+namespace rmvs {
+
+// Reference-to-source rigid motions of one problem: Y = R_rel X + t_rel.
+struct PairGeometry {
+  double Rrel[kMaxViews][9];
+  double trel[kMaxViews][3];
+};
+
+// Rotation matrix Exp([w]x) of a rotation vector (Rodrigues, FP64).
+void So3Exp(const double w[3], double E[9]);
+
+// Kernel arguments for reference view `ref` and its sources.  pose_delta
+// (nsrc x 6, may be null) perturbs each source's world->camera transform on
+// the left, Y -> Exp(w) Y + tau with (w, tau) = pose_delta[6k .. 6k+5];
+// geo (may be null) receives the resulting R_rel, t_rel.
+int32_t PlaneMakeArgs(recon_mvs_hip_context* ctx, int ref, int nsrc, const int32_t* srcs,
+                      const recon_mvs_plane_params* p, PlaneArgs* a,
+                      const double* pose_delta = nullptr, PairGeometry* geo = nullptr);
+// Allocates the depth-map slot of `view` (no-op when it exists with its size).
+int32_t PlaneEnsureState(recon_mvs_hip_context* ctx, int view);
+// Mean reference-to-source baseline (patchmatch_stereo.cpp:472-479).
+double PlaneMeanBaseline(recon_mvs_hip_context* ctx, int ref, int nsrc, const int32_t* srcs);
+
+}  // namespace rmvs

@@ -20,7 +20,7 @@ from . import _native as N
 from . import mvs
 
 FORMATS = {"f32": N.PIXEL_F32, "f16": N.PIXEL_F16, "bf16": N.PIXEL_BF16}
-KERNEL_CLASSES = ("sweep", "init", "refine", "consistency", "score", "eval", "batch")
+KERNEL_CLASSES = ("sweep", "init", "refine", "consistency", "score", "eval", "batch", "adjoint")
 
 
 class PlaneParams(C.Structure):
@@ -76,6 +76,10 @@ def _lib():
         L.recon_mvs_hip_batch_run.argtypes = [v, v]
         L.recon_mvs_hip_batch_schedule.argtypes = [v]
         L.recon_mvs_hip_batch_destroy.argtypes = [v, v]
+        L.recon_mvs_hip_plane_cost_grad.argtypes = [v, C.c_int32, C.c_int32, v, P(PlaneParams),
+                                                    v, v, v, v, v]
+        L.recon_mvs_hip_plane_polish.argtypes = [v, C.c_int32, C.c_int32, v, P(PlaneParams),
+                                                 C.c_int32]
         _declared = True
     return L
 
@@ -173,6 +177,31 @@ class PlaneMVS:
                                                           cost.ctypes.data, nv.ctypes.data,
                                                           vn.ctypes.data), "eval_plane_costs")
         return cost, nv, vn
+
+    def cost_grad(self, ref, srcs, params, pose_delta=None, weights=None, grad=True,
+                  pose_grad=True):
+        """Cost of every pixel of the slot of `ref` (under source poses perturbed
+        by pose_delta, nsrc x 6: w, tau) and the gradients of sum(weights * cost):
+        returns (cost HxW, grad_plane HxWx4 or None, grad_pose nsrc x 6 or None)."""
+        s = _i32(srcs)
+        h, w = self.shapes[ref]
+        pd = None if pose_delta is None else np.ascontiguousarray(pose_delta, np.float64).reshape(len(s), 6)
+        wt = None if weights is None else np.ascontiguousarray(weights, np.float32).reshape(h, w)
+        cost = np.zeros((h, w), np.float32)
+        gp = np.zeros((h, w, 4), np.float32) if grad else None
+        gs = np.zeros((len(s), 6)) if (grad and pose_grad) else None
+        self._check(self.L.recon_mvs_hip_plane_cost_grad(
+            self.ctx, ref, len(s), s.ctypes.data, C.byref(params),
+            None if pd is None else pd.ctypes.data, None if wt is None else wt.ctypes.data,
+            cost.ctypes.data, None if gp is None else gp.ctypes.data,
+            None if gs is None else gs.ctypes.data), "plane_cost_grad")
+        return cost, gp, gs
+
+    def polish(self, ref, srcs, params, steps=10):
+        """Gradient polish of the slot of `ref` (costs never increase)."""
+        s = _i32(srcs)
+        self._check(self.L.recon_mvs_hip_plane_polish(self.ctx, ref, len(s), s.ctypes.data,
+                                                      C.byref(params), int(steps)), "plane_polish")
 
     def autotune(self, ref, srcs, params):
         """Times every tile schedule on this view; returns (best, [ms per schedule])."""

@@ -11,6 +11,8 @@ import oracle
 from recon_scanner_amd import _native as N
 from recon_scanner_amd import scenes
 
+from helpers import sphere_normals
+
 
 @pytest.fixture(scope="module", autouse=True)
 def _built():
@@ -138,21 +140,6 @@ def test_identity_view_gives_perfect_ncc():
                                                              window_stride=2), hyps)
     assert (nv == 1).all()
     assert np.max(np.abs(vn - 1.0)) < 1e-12
-
-
-def sphere_normals(sc, view=0):
-    """Analytic normals of the sphere_arc scene in the camera frame of `view`."""
-    cam = sc.cams[view]
-    h, w = sc.depths[view].shape
-    yy, xx = np.mgrid[0:h, 0:w]
-    ray = np.stack([(xx + 0.5 - cam.cx) / cam.fx, (yy + 0.5 - cam.cy) / cam.fy, np.ones((h, w))], -1)
-    Pc = ray * sc.depths[view][..., None]
-    R, t = cam.R_np(), cam.t_np()
-    Pw = (Pc - t) @ R                       # R^T (p - t)
-    on_sphere = np.abs(np.linalg.norm(Pw - [0, 0, 0.35], axis=-1) - 0.05) < 1e-4
-    nw = np.where(on_sphere[..., None], (Pw - [0, 0, 0.35]) / 0.05, [0.0, 0.0, -1.0])
-    nc = nw @ R.T
-    return nc.astype(np.float32)
 
 
 def test_true_planes_score_best():
