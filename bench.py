@@ -34,13 +34,21 @@ HYPS_PER_ITER = 8           # 4 propagated + 3 perturbed + 1 random
 
 def parse():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", default="depthmaps",
+                    choices=["depthmaps", "batch", "adjoint", "copy"],
+                    help="depthmaps: the 9-view 1920x1440 job (default); batch: 48 references "
+                         "of uneven size in one captured batch, FP16 images, against the same "
+                         "jobs run one by one in FP32; adjoint: cost + gradients of a 960x720 "
+                         "depth map against 6 misaligned sources, and the 10-step polish; copy: "
+                         "a plain 1 GiB device copy (calibrates rocprofv3 byte counters)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1440)
     ap.add_argument("--iterations", type=int, default=5)
-    ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16"])
+    ap.add_argument("--storage", default=None, choices=["f32", "f16", "bf16"],
+                    help="image storage (default: f32; f16 for --workload batch)")
     ap.add_argument("--tile-schedule", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target duration of the CPU-baseline sample (0 = skip)")
@@ -93,11 +101,163 @@ def cpu_baseline(sc, iterations, target_s):
             "median_depth_error_mm": err * 1000.0, "seconds": dt}
 
 
+def base_record(args, dt, workload, **config):
+    return {"n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": dt / args.steps * 1000.0, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "config": dict(workload=workload, **config)}
+
+
+def median_rel_error(pm, sc, refs):
+    errs = []
+    for r in refs:
+        d = pm.depth_map(r)["depth"]
+        m = np.isfinite(d) & (sc.depths[r] > 0)
+        errs.append(np.abs(d[m] - sc.depths[r][m]) / sc.depths[r][m])
+    return float(np.median(np.concatenate(errs)))
+
+
+def run_batch(args):
+    """48 references (640x480 .. 1920x1440, 4-10 sources each) of a 64-camera
+    orbit: one captured batch with FP16 images against the same jobs run one
+    by one with FP32 images."""
+    from recon_scanner_amd import plane, scenes
+    storage = args.storage or "f16"
+    t0 = time.time()
+    sc, jobs = scenes.orbit_batch(num_refs=48, num_cams=64, seed=0)
+    sys.stderr.write(f"[bench] orbit rendered in {time.time() - t0:.1f} s\n")
+    p = plane.default_params(patch_size=11, window_stride=2, top_k=4, iterations=args.iterations,
+                             min_depth_m=sc.min_depth, max_depth_m=sc.max_depth, value_scale=1.0,
+                             tile_schedule=args.tile_schedule)
+    units = sum(sc.cams[r].width * sc.cams[r].height * len(s) for r, s in jobs) * args.iterations
+    refs = [r for r, _ in jobs]
+    # plain path: FP32 images, one plane_patchmatch per reference
+    pm = plane.PlaneMVS(sc.cams, sc.images, storage="f32")
+
+    def plain():
+        for r, srcs in jobs:
+            pm.patchmatch(r, srcs, p)
+    for _ in range(args.warmup):
+        plain()
+    pm.synchronize()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        plain()
+    pm.synchronize()
+    dt_plain = time.perf_counter() - t
+    err_plain = median_rel_error(pm, sc, refs)
+    sys.stderr.write(f"[bench] plain FP32 path: {dt_plain / args.steps * 1e3:.1f} ms per pass\n")
+    # batched path
+    pm = plane.PlaneMVS(sc.cams, sc.images, storage=storage)
+    b = pm.batch(jobs, p)
+    for _ in range(args.warmup):
+        b.run()
+    pm.synchronize()
+    pm.timing(enable=True, reset=True)
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        b.run()
+    pm.synchronize()
+    dt = time.perf_counter() - t
+    tim = pm.timing(enable=False, reset=True)
+    err_b = median_rel_error(pm, sc, refs)
+    b.close()
+    mpx = sum(sc.cams[r].width * sc.cams[r].height for r in refs) / 1e6
+    out = {"metric": "batched PatchMatch Gpix*view*iter/s", "value": units * args.steps / dt / 1e9,
+           "unit": "Gpix*view*iter/s",
+           "dtype": f"fp32 ({storage} images)",
+           "data": "synthetic (64-camera orbit around a sphere and two boxes, analytic depth)"}
+    out.update(base_record(args, dt, f"48 references of uneven size ({mpx:.1f} Mpx in total), "
+                                     f"4-10 sources each, 11x11 stride 2, top-4, "
+                                     f"{args.iterations} red-black iterations, one captured batch",
+                           refs=len(refs), megapixels=mpx, storage=storage,
+                           schedule=b.schedule))
+    out.update({"depth_maps_per_s": len(refs) * args.steps / dt,
+                "plain_fp32": {"value": units * args.steps / dt_plain / 1e9,
+                               "depth_maps_per_s": len(refs) * args.steps / dt_plain,
+                               "median_rel_depth_error": err_plain},
+                "speedup_vs_plain_fp32": dt_plain / dt,
+                "median_rel_depth_error": err_b,
+                "batch_graph_ms": tim["batch"][0] / max(1, tim["batch"][1])})
+    print(json.dumps(out))
+
+
+def run_adjoint(args):
+    """Cost and gradients (per-pixel plane, 6 source poses) of a 960x720 depth
+    map whose sources are misaligned by 0.5 deg / 1 mm (seeded), then the
+    10-step gradient polish of a fresh PatchMatch result."""
+    import oracle
+    from recon_scanner_amd import plane, scenes
+    sc = scenes.sphere_planes(width=960, height=720, focal=1250.0, n_views=7, supersample=2)
+    rng = np.random.default_rng(5)
+    cams = list(sc.cams)
+    for k in range(1, 7):
+        ax, tr = rng.normal(size=3), rng.normal(size=3)
+        d = np.concatenate([np.radians(0.5) * ax / np.linalg.norm(ax), 1e-3 * tr / np.linalg.norm(tr)])
+        cams[k] = oracle.perturb_camera(sc.cams[k], d)
+    srcs = list(range(1, 7))
+    p = plane.default_params(patch_size=11, window_stride=2, top_k=4, iterations=args.iterations,
+                             min_depth_m=sc.min_depth, max_depth_m=sc.max_depth, value_scale=1.0,
+                             tile_schedule=args.tile_schedule)
+    pm = plane.PlaneMVS(cams, sc.images, storage=args.storage or "f32")
+    pm.patchmatch(0, srcs, p)
+    for _ in range(args.warmup):
+        pm.cost_grad(0, srcs, p)
+    pm.timing(enable=True, reset=True)
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        pm.cost_grad(0, srcs, p)
+    dt = time.perf_counter() - t
+    tim = pm.timing(enable=False, reset=True)
+    adj_ms = tim["adjoint"][0] / max(1, tim["adjoint"][1])
+    P, V = 960 * 720, 6
+    pm.timing(enable=True, reset=True)
+    pm.polish(0, srcs, p, steps=10)
+    pm.synchronize()
+    pol_ms = pm.timing(enable=False, reset=True)["adjoint"][0]
+    out = {"metric": "adjoint Gpix*view/s (cost + plane and pose gradients)",
+           "value": P * V / (adj_ms * 1e-3) / 1e9, "unit": "Gpix*view/s", "dtype": "fp32",
+           "data": "synthetic (rendered sphere + 2 tilted planes, poses misaligned 0.5 deg / 1 mm)"}
+    out.update(base_record(args, dt, "adjoint of a 960x720 depth map, 6 sources, 11x11 stride 2, "
+                                     "top-4 (host copies included in ms_per_step)",
+                           width=960, height=720, sources=6))
+    out.update({"adjoint_kernel_ms": adj_ms, "polish_10_steps_ms": pol_ms})
+    print(json.dumps(out))
+
+
+def run_copy(args):
+    """A plain device-to-device copy of 1 GiB, `steps` times: known bytes for
+    calibrating FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = 1 << 28
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(args.warmup):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t
+    out = {"metric": "device copy GB/s", "value": 2 * 4 * n * args.steps / dt / 1e9, "unit": "GB/s",
+           "dtype": "fp32", "data": "constant", "bytes_read_per_copy": 4 * n,
+           "bytes_written_per_copy": 4 * n}
+    out.update(base_record(args, dt, "1 GiB device-to-device copy"))
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
+    if args.workload == "batch":
+        return run_batch(args)
+    if args.workload == "adjoint":
+        return run_adjoint(args)
+    if args.workload == "copy":
+        return run_copy(args)
+    args.storage = args.storage or "f32"
     from recon_scanner_amd import plane, scenes
 
     t0 = time.time()

@@ -130,11 +130,10 @@ __device__ __forceinline__ void ViewPass(const PlaneArgs& a, const AdjArgs& g, i
                                          float nd, float nxf, float nyf, float rho, ViewStats* st,
                                          float dc, ViewGrad* gr, float ifx, float ify) {
   constexpr int NS = RefWindow<HALF, STRIDE>::NS;
-  constexpr int kEsz = FMT == RECON_MVS_PIXEL_F32 ? 4 : 2;
+  constexpr int kEB = ElemBytes<FMT>();
   const int pitch = a.src_pitch[v], hgt = a.src_h[v];
   const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
-      const_cast<void*>(a.src_img[v]), 0, pitch * (hgt + 1) * kEsz, 0x00020000);
-  const int row_bytes = pitch * kEsz;
+      const_cast<void*>(a.src_img[v]), 0, pitch * hgt * kEB, 0x00020000);
   const float shift = win.shift;
   float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f, s4 = 0.0f, cnt = 0.0f;
   float al = 0.0f, be = 0.0f;
@@ -159,11 +158,10 @@ __device__ __forceinline__ void ViewPass(const PlaneArgs& a, const AdjArgs& g, i
                       (ty >= vg.loy) & (ty <= vg.hiy);
       const float flx = floorf(tx), fly = floorf(ty);
       const float fx = tx - flx, fy = ty - fly;
-      const int off = static_cast<int>(fmaf(fly, vg.fpitch, flx + vg.obase)) * kEsz;
-      const float2 p0 = BufPair<FMT>(rsrc, ok ? off : 0, 0);
-      const float2 p1 = BufPair<FMT>(rsrc, ok ? off : 0, row_bytes);
-      const float dtop = p0.y - p0.x, dbot = p1.y - p1.x;
-      const float top = fmaf(fx, dtop, p0.x), bot = fmaf(fx, dbot, p1.x);
+      const int off = static_cast<int>(fmaf(fly, vg.fpitch, flx + vg.obase)) * kEB;
+      const float4 qd = BufQuad<FMT>(rsrc, ok ? off : 0);
+      const float dtop = qd.z - qd.x, dbot = qd.w - qd.y;
+      const float top = fmaf(fx, dtop, qd.x), bot = fmaf(fx, dbot, qd.y);
       const float sv = fmaf(fy, bot - top, top) - shift;
       if constexpr (MODE == 0) {
         const float s = ok ? sv : 0.0f, r = ok ? rv : 0.0f;

@@ -22,7 +22,7 @@ constexpr float kCostInf = 1e30f;
 struct ResidentView {
   int w = 0, h = 0, pitch = 0;
   int format = RECON_MVS_PIXEL_F32;
-  void* pixels = nullptr;   // (h + 1) rows x pitch, zero padded right and below
+  void* pixels = nullptr;   // h rows x pitch row-pair elements (see BufQuad)
   recon_mvs_camera cam{};
 };
 
@@ -81,53 +81,50 @@ struct PlaneArgs {
   int tiles_x, tiles_y, tiles_per_xcd, num_tiles;
 };
 
+// Resident images are stored as row pairs: element (y, x) holds
+// (I[y][x], I[y+1][x]) (zero beyond the image), so the four bilinear taps of
+// a sample at (x, y) are the 2 x 2 elements (y, x), (y, x + 1): one load of
+// 16 B (FP32) or 8 B (FP16 / BF16).  A stride-2 window touches a dense 12 x 12
+// pixel block; as row pairs that is 6 runs of 12 elements instead of 12 runs
+// of 12 pixels, about half the cache lines for scattered (unconverged)
+// hypotheses.  Twice the bytes of a plain image.
+template <int FMT>
+constexpr int ElemBytes() {
+  return FMT == RECON_MVS_PIXEL_F32 ? 8 : 4;
+}
+
+// Pixel (y, x) of a row-pair image, i = y * pitch + x.
 template <int FMT>
 __device__ __forceinline__ float Pix(const void* base, int i) {
   if constexpr (FMT == RECON_MVS_PIXEL_F32) {
-    return static_cast<const float*>(base)[i];
+    return static_cast<const float*>(base)[2 * i];
   } else if constexpr (FMT == RECON_MVS_PIXEL_F16) {
-    return static_cast<float>(static_cast<const _Float16*>(base)[i]);
+    return static_cast<float>(static_cast<const _Float16*>(base)[2 * i]);
   } else {
-    return __uint_as_float(static_cast<uint32_t>(static_cast<const uint16_t*>(base)[i]) << 16);
+    return __uint_as_float(static_cast<uint32_t>(static_cast<const uint16_t*>(base)[2 * i]) << 16);
   }
 }
 
-// Pixels i and i + 1 of one row in one load (8 bytes for FP32, 4 for FP16 /
-// BF16; unaligned global access is enabled on gfx950 under HSA).
-// Pixels at byte offset `off` and the next one, through a raw buffer
-// descriptor (32-bit offsets; `soff` is a wave-uniform extra byte offset).
+// The taps of the element at byte offset `off` and its right neighbour,
+// through a raw buffer descriptor (32-bit offsets; reads beyond the
+// descriptor's size return zeros): (top-left, bottom-left, top-right,
+// bottom-right).
 template <int FMT>
-__device__ __forceinline__ float2 BufPair(__amdgpu_buffer_rsrc_t rsrc, int off, int soff) {
+__device__ __forceinline__ float4 BufQuad(__amdgpu_buffer_rsrc_t rsrc, int off) {
   if constexpr (FMT == RECON_MVS_PIXEL_F32) {
-    const auto v = __builtin_amdgcn_raw_buffer_load_b64(rsrc, off, soff, 0);
-    return make_float2(__uint_as_float(v[0]), __uint_as_float(v[1]));
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(rsrc, off, 0, 0);
+    return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]),
+                       __uint_as_float(v[3]));
   } else {
-    const uint32_t u = __builtin_amdgcn_raw_buffer_load_b32(rsrc, off, soff, 0);
+    const auto u = __builtin_amdgcn_raw_buffer_load_b64(rsrc, off, 0, 0);
     if constexpr (FMT == RECON_MVS_PIXEL_F16) {
-      const _Float16 lo = __builtin_bit_cast(_Float16, static_cast<uint16_t>(u & 0xFFFFu));
-      const _Float16 hi = __builtin_bit_cast(_Float16, static_cast<uint16_t>(u >> 16));
-      return make_float2(static_cast<float>(lo), static_cast<float>(hi));
+      const auto h = [](uint32_t b) {
+        return static_cast<float>(__builtin_bit_cast(_Float16, static_cast<uint16_t>(b)));
+      };
+      return make_float4(h(u[0] & 0xFFFFu), h(u[0] >> 16), h(u[1] & 0xFFFFu), h(u[1] >> 16));
     } else {
-      return make_float2(__uint_as_float(u << 16), __uint_as_float(u & 0xFFFF0000u));
-    }
-  }
-}
-
-template <int FMT>
-__device__ __forceinline__ float2 PixPair(const void* base, int i) {
-  if constexpr (FMT == RECON_MVS_PIXEL_F32) {
-    float2 v;
-    __builtin_memcpy(&v, static_cast<const float*>(base) + i, 8);
-    return v;
-  } else {
-    uint32_t u;
-    __builtin_memcpy(&u, static_cast<const uint16_t*>(base) + i, 4);
-    if constexpr (FMT == RECON_MVS_PIXEL_F16) {
-      const _Float16 lo = __builtin_bit_cast(_Float16, static_cast<uint16_t>(u & 0xFFFFu));
-      const _Float16 hi = __builtin_bit_cast(_Float16, static_cast<uint16_t>(u >> 16));
-      return make_float2(static_cast<float>(lo), static_cast<float>(hi));
-    } else {
-      return make_float2(__uint_as_float(u << 16), __uint_as_float(u & 0xFFFF0000u));
+      return make_float4(__uint_as_float(u[0] << 16), __uint_as_float(u[0] & 0xFFFF0000u),
+                         __uint_as_float(u[1] << 16), __uint_as_float(u[1] & 0xFFFF0000u));
     }
   }
 }
@@ -249,10 +246,9 @@ __device__ float PlaneCost(const PlaneArgs& a, int x, int y, const RefWindow<HAL
           const int w = a.src_w[v], hgt = a.src_h[v], pitch = a.src_pitch[v];
           const float lox = static_cast<float>(-iu), hix = static_cast<float>(w - 1 - iu);
           const float loy = static_cast<float>(-iv), hiy = static_cast<float>(hgt - 1 - iv);
-          constexpr int kEsz = FMT == RECON_MVS_PIXEL_F32 ? 4 : 2;
+          constexpr int kEB = ElemBytes<FMT>();
           const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
-              const_cast<void*>(a.src_img[v]), 0, pitch * (hgt + 1) * kEsz, 0x00020000);
-          const int row_bytes = pitch * kEsz;
+              const_cast<void*>(a.src_img[v]), 0, pitch * hgt * kEB, 0x00020000);
           // the window origin's row/column in the source, as a float index
           // offset (exact: image sizes stay below 2^24 pixels)
           const float fpitch = static_cast<float>(pitch);
@@ -279,7 +275,7 @@ __device__ float PlaneCost(const PlaneArgs& a, int x, int y, const RefWindow<HAL
           float ss = 0.0f, sss = 0.0f, srs = 0.0f, xr = 0.0f, xrr = 0.0f, xcnt = 0.0f;
           if (!outside) {
             // Each window row in three phases (positions, all loads,
-            // arithmetic): a wave keeps 2 x NS loads in flight.  Loads need
+            // arithmetic): a wave keeps NS loads in flight.  Loads need
             // no clamping: out-of-range buffer reads return 0 and the
             // sample is masked out.
 #pragma unroll 1
@@ -306,19 +302,16 @@ __device__ float PlaneCost(const PlaneArgs& a, int x, int y, const RefWindow<HAL
                 const float flx = floorf(txs[i]), fly = floorf(tys[i]);
                 fxs[i] = txs[i] - flx;
                 fys[i] = tys[i] - fly;
-                off[i] = static_cast<int>(fmaf(fly, fpitch, flx + obase)) * kEsz;
+                off[i] = static_cast<int>(fmaf(fly, fpitch, flx + obase)) * kEB;
               }
-              float2 p0[NS], p1[NS];
+              float4 q[NS];
 #pragma unroll
-              for (int i = 0; i < NS; ++i) {
-                p0[i] = BufPair<FMT>(rsrc, off[i], 0);
-                p1[i] = BufPair<FMT>(rsrc, off[i], row_bytes);
-              }
+              for (int i = 0; i < NS; ++i) q[i] = BufQuad<FMT>(rsrc, off[i]);
               float sv[NS];
 #pragma unroll
               for (int i = 0; i < NS; ++i) {
-                const float top = fmaf(fxs[i], p0[i].y - p0[i].x, p0[i].x);
-                const float bot = fmaf(fxs[i], p1[i].y - p1[i].x, p1[i].x);
+                const float top = fmaf(fxs[i], q[i].z - q[i].x, q[i].x);
+                const float bot = fmaf(fxs[i], q[i].w - q[i].y, q[i].y);
                 sv[i] = fmaf(fys[i], bot - top, top) - shift;
               }
               if (__ballot(!row_ok) == 0) {

@@ -358,27 +358,37 @@ __global__ __launch_bounds__(64) void k_plane_eval(PlaneArgs a, int n, const flo
     for (int v = 0; v < a.nsrc; ++v) view_ncc[q * a.nsrc + v] = vn[v];
 }
 
-// Converts an uploaded view (u8 or f32 rows) into the padded storage layout.
+__device__ __forceinline__ uint16_t ToBf16(float v) {
+  // round to nearest even into bfloat16
+  const uint32_t b = __float_as_uint(v);
+  return static_cast<uint16_t>((b + 0x7FFFu + ((b >> 16) & 1u)) >> 16);
+}
+
+// Converts an uploaded view (u8 or f32 rows) into the row-pair storage layout
+// (see BufQuad): one thread per element.
 template <int FMT>
 __global__ void k_store_view(const void* __restrict__ src, int src_format, int w, int h, int pitch,
                              void* __restrict__ dst) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= pitch * (h + 1)) return;
+  if (i >= pitch * h) return;
   const int y = i / pitch, x = i % pitch;
-  float v = 0.0f;
-  if (x < w && y < h) {
-    v = src_format == RECON_MVS_PIXEL_U8 ? static_cast<float>(static_cast<const uint8_t*>(src)[y * w + x])
-                                         : static_cast<const float*>(src)[y * w + x];
-  }
+  auto at = [&](int xx, int yy) -> float {
+    if (xx >= w || yy >= h) return 0.0f;
+    return src_format == RECON_MVS_PIXEL_U8
+               ? static_cast<float>(static_cast<const uint8_t*>(src)[yy * w + xx])
+               : static_cast<const float*>(src)[yy * w + xx];
+  };
+  const float v0 = at(x, y), v1 = at(x, y + 1);
   if constexpr (FMT == RECON_MVS_PIXEL_F32) {
-    static_cast<float*>(dst)[i] = v;
+    static_cast<float2*>(dst)[i] = make_float2(v0, v1);
   } else if constexpr (FMT == RECON_MVS_PIXEL_F16) {
-    static_cast<_Float16*>(dst)[i] = static_cast<_Float16>(v);
+    _Float16* o = static_cast<_Float16*>(dst) + 2 * static_cast<size_t>(i);
+    o[0] = static_cast<_Float16>(v0);
+    o[1] = static_cast<_Float16>(v1);
   } else {
-    // round to nearest even into bfloat16
-    const uint32_t b = __float_as_uint(v);
-    const uint32_t r = b + 0x7FFFu + ((b >> 16) & 1u);
-    static_cast<uint16_t*>(dst)[i] = static_cast<uint16_t>(r >> 16);
+    uint16_t* o = static_cast<uint16_t*>(dst) + 2 * static_cast<size_t>(i);
+    o[0] = ToBf16(v0);
+    o[1] = ToBf16(v1);
   }
 }
 
@@ -900,14 +910,14 @@ RECON_MVS_API int32_t recon_mvs_hip_set_views(recon_mvs_hip_context* ctx, int32_
     v->format = storage_format;
     v->cam = in.camera;
     ctx->views.push_back(v);
-    const size_t bytes = static_cast<size_t>(v->pitch) * (v->h + 1) * esz;
+    const size_t bytes = static_cast<size_t>(v->pitch) * v->h * 2 * esz;
     if (hipMalloc(&v->pixels, bytes) != hipSuccess)
       return Fail(ctx, RECON_MVS_ERR_OUT_OF_MEMORY, "view allocation failed");
     const size_t in_bytes = static_cast<size_t>(v->w) * v->h * (in.pixel_format == RECON_MVS_PIXEL_U8 ? 1 : 4);
     void* staging = Ensure(ctx, kSlotQueryImgA, in_bytes);
     if (!staging) return Fail(ctx, RECON_MVS_ERR_OUT_OF_MEMORY, "staging allocation failed");
     RMVS_HIP(ctx, hipMemcpyAsync(staging, in.pixels, in_bytes, hipMemcpyHostToDevice, ctx->stream));
-    const int total = v->pitch * (v->h + 1);
+    const int total = v->pitch * v->h;
     const dim3 g((total + 255) / 256), b(256);
     if (storage_format == RECON_MVS_PIXEL_F32)
       hipLaunchKernelGGL(k_store_view<RECON_MVS_PIXEL_F32>, g, b, 0, ctx->stream, staging, in.pixel_format, v->w, v->h, v->pitch, v->pixels);

@@ -8,7 +8,7 @@ texture contrast ~0.1 that bounds the per-view NCC change by ~1e-3 (FP16) and
 ~1e-2 (BF16) in the worst case; the typical change is far smaller.  Stated
 bounds:
   * per-view NCC vs the FP64 oracle on the unrounded images: 99th percentile
-    of |dNCC| <= 5e-4 (FP16), 4e-3 (BF16); maximum <= 4e-3 (FP16), 3e-2 (BF16);
+    of |dNCC| <= 1e-3 (FP16), 8e-3 (BF16); maximum <= 4e-3 (FP16), 3e-2 (BF16);
   * end to end (3 red-black iterations): the median relative depth error
     against the analytic truth is within 0.1 percentage points of the FP32
     run's.
@@ -21,7 +21,7 @@ from recon_scanner_amd import plane, scenes
 
 pytestmark = pytest.mark.gpu
 
-BOUNDS = {"f16": (5e-4, 4e-3), "bf16": (4e-3, 3e-2)}
+BOUNDS = {"f16": (1e-3, 4e-3), "bf16": (8e-3, 3e-2)}
 
 
 def say(*a):
