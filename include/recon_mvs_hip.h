@@ -269,9 +269,19 @@ typedef struct recon_mvs_plane_params {
   double normal_perturb;     /* normal perturbation radius at iteration 0 */
   double photo_threshold;    /* hypotheses with cost > 3 x this are rejected */
   double value_scale;        /* full range of the image values (see above) */
+  /* Occupancy of the unconverged phases.  Random init and the first
+     iterations evaluate scattered hypotheses whose window gathers miss the
+     caches; fewer resident workgroups per CU thrash them less.
+     occupancy_cap: 0 = automatic (FP32 images: init + `capped_iterations`
+     iterations at 1 workgroup per CU; FP16 / BF16: uncapped; with
+     tile_schedule = -1 tuned per phase), -1 = never capped, n > 0 = n
+     workgroups per CU in the capped phases.  Results do not depend on it. */
+  int32_t occupancy_cap;
+  int32_t capped_iterations; /* leading iterations run capped (default 1; < 0 = 1) */
 } recon_mvs_plane_params;
 
-/* Fills *p with the defaults: 11x11 window, stride 2, top-4, 5 iterations. */
+/* Fills *p with the defaults: 11x11 window, stride 2, top-4, 5 iterations,
+   automatic occupancy (init + 1 iteration capped for FP32 images). */
 RECON_MVS_API void recon_mvs_plane_params_default(recon_mvs_plane_params* p);
 
 /*

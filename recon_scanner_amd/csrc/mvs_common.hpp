@@ -98,6 +98,8 @@ struct recon_mvs_hip_context {
   int64_t timed_count[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   // autotuned tile schedule per (size, views, window, format)
   std::map<std::string, int> tuned_schedule;
+  // autotuned per-phase occupancy caps (plane mode, tile_schedule = -1)
+  std::map<std::string, std::vector<int>> tuned_caps;
 };
 
 namespace rmvs {

@@ -489,6 +489,8 @@ int32_t CheckParams(recon_mvs_hip_context* ctx, const recon_mvs_plane_params* p)
     return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "tile_schedule out of [-1, 2]");
   if (p->propagation_reach > kMaxReach)
     return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "propagation_reach above 16");
+  if (p->occupancy_cap < -1 || p->occupancy_cap > 64)
+    return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "occupancy_cap out of [-1, 64]");
   return RECON_MVS_OK;
 }
 
@@ -629,27 +631,52 @@ inline void ScheduleTile(int s, int* tw, int* th) {
   *th = 512 / *tw;
 }
 
-// Workgroups of a persistent launch: as many as fit on the device at once.
+// Dynamic LDS that limits a kernel with `stat` bytes of static LDS to `cap`
+// resident workgroups per CU (0: no limit).  The padding is never touched; it
+// only makes the (cap + 1)-th workgroup not fit in the CU's LDS.
+inline size_t CapBytes(int cap, size_t stat) {
+  static int lds_cu = -1, lds_block = 0;
+  if (lds_cu < 0) {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess)
+      lds_cu = 0;
+    if (hipDeviceGetAttribute(&lds_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+      lds_block = 0;
+  }
+  if (cap <= 0 || lds_cu <= 0) return 0;
+  const size_t per_wg = (static_cast<size_t>(lds_cu) / cap) & ~static_cast<size_t>(1023);
+  const size_t want = per_wg > stat + 1024 ? per_wg - 1024 - stat : 0;
+  const size_t room = static_cast<size_t>(lds_block) > stat ? static_cast<size_t>(lds_block) - stat : 0;
+  return std::min(want, room);
+}
+
+// Workgroups of a persistent launch: as many as fit on the device at once
+// (with `cap` workgroups per CU at most).
 template <int HALF, int STRIDE, int FMT, int TW, int TH>
-int BatchGrid() {
-  static int grid = 0;
-  if (grid == 0) {
-    int dev = 0, nb = 0, cus = 0;
+int BatchGrid(int cap) {
+  static int nb = -1, cus = 1;
+  if (nb < 0) {
+    int dev = 0;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_plane_batch<HALF, STRIDE, FMT, TW, TH>, 256, 0);
-    grid = std::max(1, nb) * std::max(1, cus);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_plane_batch<HALF, STRIDE, FMT, TW, TH>, 256, 0) !=
+        hipSuccess)
+      nb = 1;
   }
-  return grid;
+  const int per_cu = std::max(1, cap > 0 ? std::min(cap, nb) : nb);
+  return per_cu * std::max(1, cus);
 }
 
 template <int HALF, int STRIDE, int FMT, int TW, int TH>
 void LaunchTile(KernelId id, PlaneArgs a, hipStream_t s, float f0, int i0, int n, const float* hyps,
-                float* cost, int* nvalid, float* view_ncc, const BatchLaunch* bl) {
+                float* cost, int* nvalid, float* view_ncc, const BatchLaunch* bl, int cap) {
+  constexpr size_t kTileLds = sizeof(RefTile<HALF, FMT, TW, TH>) + 64;
+  const size_t pad = CapBytes(cap, kTileLds);
   if (id == kBatch) {
     hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, s, bl->counter);
-    hipLaunchKernelGGL((k_plane_batch<HALF, STRIDE, FMT, TW, TH>), dim3(BatchGrid<HALF, STRIDE, FMT, TW, TH>()),
-                       dim3(256), 0, s, bl->args, bl->units, bl->num_units, bl->counter, bl->phase);
+    hipLaunchKernelGGL((k_plane_batch<HALF, STRIDE, FMT, TW, TH>), dim3(BatchGrid<HALF, STRIDE, FMT, TW, TH>(cap)),
+                       dim3(256), pad, s, bl->args, bl->units, bl->num_units, bl->counter, bl->phase);
     return;
   }
   a.tiles_x = (a.w + TW - 1) / TW;
@@ -658,8 +685,8 @@ void LaunchTile(KernelId id, PlaneArgs a, hipStream_t s, float f0, int i0, int n
   a.tiles_per_xcd = (a.num_tiles + 7) / 8;
   const dim3 grid(a.tiles_per_xcd * 8), block(256);
   switch (id) {
-    case kSweep: hipLaunchKernelGGL((k_plane_sweep<HALF, STRIDE, FMT, TW, TH>), grid, block, 0, s, a); break;
-    case kInit: hipLaunchKernelGGL((k_plane_init<HALF, STRIDE, FMT, TW, TH>), grid, block, 0, s, a); break;
+    case kSweep: hipLaunchKernelGGL((k_plane_sweep<HALF, STRIDE, FMT, TW, TH>), grid, block, pad, s, a); break;
+    case kInit: hipLaunchKernelGGL((k_plane_init<HALF, STRIDE, FMT, TW, TH>), grid, block, pad, s, a); break;
     case kScore: hipLaunchKernelGGL((k_plane_score<HALF, STRIDE, FMT, TW, TH>), grid, block, 0, s, a); break;
     case kRefine:
       hipLaunchKernelGGL((k_plane_refine<HALF, STRIDE, FMT, TW, TH>), grid, block, 0, s, a, f0, i0);
@@ -674,17 +701,18 @@ void LaunchTile(KernelId id, PlaneArgs a, hipStream_t s, float f0, int i0, int n
 
 template <int HALF, int STRIDE, int FMT>
 bool LaunchT(int schedule, KernelId id, const PlaneArgs& a, hipStream_t s, float f0, int i0, int n,
-             const float* hyps, float* cost, int* nvalid, float* view_ncc, const BatchLaunch* bl) {
+             const float* hyps, float* cost, int* nvalid, float* view_ncc, const BatchLaunch* bl,
+             int cap) {
   // the 11x11 stride-2 window is compiled for every tile schedule, the others
   // for the default one
   constexpr bool kAllSchedules = (HALF == 5 && STRIDE == 2);
   switch (schedule) {
-    case 0: LaunchTile<HALF, STRIDE, FMT, 64, 8>(id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc, bl); return true;
+    case 0: LaunchTile<HALF, STRIDE, FMT, 64, 8>(id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cap); return true;
     case 1:
-      if constexpr (kAllSchedules) { LaunchTile<HALF, STRIDE, FMT, 32, 16>(id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc, bl); return true; }
+      if constexpr (kAllSchedules) { LaunchTile<HALF, STRIDE, FMT, 32, 16>(id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cap); return true; }
       return false;
     case 2:
-      if constexpr (kAllSchedules) { LaunchTile<HALF, STRIDE, FMT, 128, 4>(id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc, bl); return true; }
+      if constexpr (kAllSchedules) { LaunchTile<HALF, STRIDE, FMT, 128, 4>(id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cap); return true; }
       return false;
     default: return false;
   }
@@ -692,16 +720,17 @@ bool LaunchT(int schedule, KernelId id, const PlaneArgs& a, hipStream_t s, float
 
 template <int HALF, int STRIDE>
 bool LaunchFmt(int fmt, int schedule, KernelId id, const PlaneArgs& a, hipStream_t s, float f0, int i0,
-               int n, const float* hyps, float* cost, int* nvalid, float* view_ncc, const BatchLaunch* bl) {
+               int n, const float* hyps, float* cost, int* nvalid, float* view_ncc, const BatchLaunch* bl,
+               int cap) {
   // FP16 / BF16 storage is compiled for the 11x11 stride-2 and 7x7 windows
   constexpr bool kAllFormats = (HALF == 5 && STRIDE == 2) || (HALF == 3 && STRIDE == 1);
   switch (fmt) {
-    case RECON_MVS_PIXEL_F32: return LaunchT<HALF, STRIDE, RECON_MVS_PIXEL_F32>(schedule, id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc, bl);
+    case RECON_MVS_PIXEL_F32: return LaunchT<HALF, STRIDE, RECON_MVS_PIXEL_F32>(schedule, id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cap);
     case RECON_MVS_PIXEL_F16:
-      if constexpr (kAllFormats) return LaunchT<HALF, STRIDE, RECON_MVS_PIXEL_F16>(schedule, id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc, bl);
+      if constexpr (kAllFormats) return LaunchT<HALF, STRIDE, RECON_MVS_PIXEL_F16>(schedule, id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cap);
       return false;
     case RECON_MVS_PIXEL_BF16:
-      if constexpr (kAllFormats) return LaunchT<HALF, STRIDE, RECON_MVS_PIXEL_BF16>(schedule, id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc, bl);
+      if constexpr (kAllFormats) return LaunchT<HALF, STRIDE, RECON_MVS_PIXEL_BF16>(schedule, id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cap);
       return false;
     default: return false;
   }
@@ -711,20 +740,21 @@ bool LaunchFmt(int fmt, int schedule, KernelId id, const PlaneArgs& a, hipStream
 int32_t Launch(recon_mvs_hip_context* ctx, const recon_mvs_plane_params* p, KernelId id,
                const PlaneArgs& a, float f0 = 0.0f, int i0 = 0, int n = 0,
                const float* hyps = nullptr, float* cost = nullptr, int* nvalid = nullptr,
-               float* view_ncc = nullptr, const BatchLaunch* bl = nullptr, int schedule = -1) {
+               float* view_ncc = nullptr, const BatchLaunch* bl = nullptr, int schedule = -1,
+               int cap = 0) {
   const int fmt = ctx->storage_format;
   const int sch = schedule >= 0 ? schedule : p->tile_schedule;
   hipEvent_t eb, ee;
   TimerBegin(ctx, static_cast<int>(id), &eb, &ee);
   bool ok = false;
   const int ps = p->patch_size, st = p->window_stride;
-  if (ps == 11 && st == 2) ok = LaunchFmt<5, 2>(fmt, sch, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc, bl);
+  if (ps == 11 && st == 2) ok = LaunchFmt<5, 2>(fmt, sch, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cap);
 #ifndef RMVS_BENCH_WINDOW_ONLY  // (development builds compile the 11x11 stride-2 window only)
-  else if (ps == 11 && st == 1) ok = LaunchFmt<5, 1>(fmt, sch, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc, bl);
-  else if (ps == 7 && st == 1) ok = LaunchFmt<3, 1>(fmt, sch, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc, bl);
-  else if (ps == 5 && st == 1) ok = LaunchFmt<2, 1>(fmt, sch, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc, bl);
-  else if (ps == 7 && st == 2) ok = LaunchFmt<3, 2>(fmt, sch, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc, bl);
-  else if (ps == 9 && st == 2) ok = LaunchFmt<4, 2>(fmt, sch, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc, bl);
+  else if (ps == 11 && st == 1) ok = LaunchFmt<5, 1>(fmt, sch, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cap);
+  else if (ps == 7 && st == 1) ok = LaunchFmt<3, 1>(fmt, sch, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cap);
+  else if (ps == 5 && st == 1) ok = LaunchFmt<2, 1>(fmt, sch, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cap);
+  else if (ps == 7 && st == 2) ok = LaunchFmt<3, 2>(fmt, sch, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cap);
+  else if (ps == 9 && st == 2) ok = LaunchFmt<4, 2>(fmt, sch, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cap);
 #endif
   if (!ok)
     return Fail(ctx, RECON_MVS_ERR_NOT_IMPLEMENTED,
@@ -815,6 +845,82 @@ int32_t ResolveSchedule(recon_mvs_hip_context* ctx, const PlaneArgs& a,
   return RECON_MVS_OK;
 }
 
+// Times one full PatchMatch run of view `a` (init + every half-sweep) with
+// occupancy cap `cap` in every phase; ms[phase] receives each launch's time.
+int32_t TimePhases(recon_mvs_hip_context* ctx, PlaneArgs a, const recon_mvs_plane_params* p,
+                   int sched, int cap, std::vector<float>* ms) {
+  const int nph = 1 + 2 * p->iterations;
+  std::vector<hipEvent_t> ev(nph + 1, nullptr);
+  int32_t st = RECON_MVS_OK;
+  for (auto& e : ev)
+    if (hipEventCreate(&e) != hipSuccess) st = Fail(ctx, RECON_MVS_ERR_GPU_FAILURE, "event create failed");
+  if (st == RECON_MVS_OK && hipEventRecord(ev[0], ctx->stream) != hipSuccess)
+    st = Fail(ctx, RECON_MVS_ERR_GPU_FAILURE, "event record failed");
+  for (int ph = 0; ph < nph && st == RECON_MVS_OK; ++ph) {
+    if (ph == 0) {
+      st = Launch(ctx, p, kInit, a, 0.0f, 0, 0, nullptr, nullptr, nullptr, nullptr, nullptr, sched, cap);
+    } else {
+      a.iter = (ph - 1) / 2;
+      a.color = (ph - 1) % 2;
+      st = Launch(ctx, p, kSweep, a, 0.0f, 0, 0, nullptr, nullptr, nullptr, nullptr, nullptr, sched, cap);
+    }
+    if (st == RECON_MVS_OK && hipEventRecord(ev[ph + 1], ctx->stream) != hipSuccess)
+      st = Fail(ctx, RECON_MVS_ERR_GPU_FAILURE, "event record failed");
+  }
+  if (st == RECON_MVS_OK && hipEventSynchronize(ev[nph]) != hipSuccess)
+    st = Fail(ctx, RECON_MVS_ERR_GPU_FAILURE, "occupancy tuning run failed");
+  ms->assign(nph, 0.0f);
+  for (int ph = 0; ph < nph && st == RECON_MVS_OK; ++ph)
+    if (hipEventElapsedTime(&(*ms)[ph], ev[ph], ev[ph + 1]) != hipSuccess)
+      st = Fail(ctx, RECON_MVS_ERR_GPU_FAILURE, "event timing failed");
+  for (auto& e : ev)
+    if (e) (void)hipEventDestroy(e);
+  return st;
+}
+
+// Occupancy cap of every phase of a PatchMatch run (index 0: init; 1 + 2 it +
+// c: half-sweep c of iteration it), as recon_mvs_plane_params.occupancy_cap
+// describes.  Tuning runs the view under caps 0 (none), 1 and 2 and keeps
+// each phase's fastest; the phases compute the same results under any cap.
+int32_t ResolveCaps(recon_mvs_hip_context* ctx, const PlaneArgs& a, const recon_mvs_plane_params* p,
+                    int sched, std::vector<int>* caps) {
+  const int nph = 1 + 2 * p->iterations;
+  caps->assign(nph, 0);
+  if (p->occupancy_cap < 0) return RECON_MVS_OK;
+  const int ci = p->capped_iterations < 0 ? 1 : p->capped_iterations;
+  const int ncapped = std::min(nph, 1 + 2 * ci);
+  if (p->occupancy_cap > 0 || p->tile_schedule >= 0) {
+    const int cap = p->occupancy_cap > 0 ? p->occupancy_cap
+                                         : (ctx->storage_format == RECON_MVS_PIXEL_F32 ? 1 : 0);
+    for (int ph = 0; ph < ncapped; ++ph) (*caps)[ph] = cap;
+    return RECON_MVS_OK;
+  }
+  const std::string key = TuneKey(ctx, a, p) + "/" + std::to_string(sched) + "/" +
+                          std::to_string(p->iterations);
+  auto it = ctx->tuned_caps.find(key);
+  if (it != ctx->tuned_caps.end()) {
+    *caps = it->second;
+    return RECON_MVS_OK;
+  }
+  const bool timing = ctx->timing;
+  ctx->timing = false;
+  const int trial_caps[3] = {0, 1, 2};
+  std::vector<float> best(nph, 3.0e38f), ms;
+  int32_t st = TimePhases(ctx, a, p, sched, 0, &ms);  // warm-up (code load)
+  for (int t = 0; t < 3 && st == RECON_MVS_OK; ++t) {
+    st = TimePhases(ctx, a, p, sched, trial_caps[t], &ms);
+    for (int ph = 0; ph < nph && st == RECON_MVS_OK; ++ph)
+      if (ms[ph] < best[ph]) {
+        best[ph] = ms[ph];
+        (*caps)[ph] = trial_caps[t];
+      }
+  }
+  ctx->timing = timing;
+  if (st != RECON_MVS_OK) return st;
+  ctx->tuned_caps[key] = *caps;
+  return RECON_MVS_OK;
+}
+
 }  // namespace
 
 namespace rmvs {
@@ -882,6 +988,8 @@ RECON_MVS_API void recon_mvs_plane_params_default(recon_mvs_plane_params* p) {
   p->normal_perturb = 0.30;
   p->photo_threshold = 1.0;
   p->value_scale = 1.0;
+  p->occupancy_cap = 0;
+  p->capped_iterations = 1;
 }
 
 RECON_MVS_API int32_t recon_mvs_hip_set_views(recon_mvs_hip_context* ctx, int32_t n,
@@ -950,13 +1058,18 @@ RECON_MVS_API int32_t recon_mvs_hip_plane_patchmatch(recon_mvs_hip_context* ctx,
   int sched = 0;
   st = ResolveSchedule(ctx, a, params, &sched);
   if (st != RECON_MVS_OK) return st;
-  st = Launch(ctx, params, kInit, a, 0.0f, 0, 0, nullptr, nullptr, nullptr, nullptr, nullptr, sched);
+  std::vector<int> caps;
+  st = ResolveCaps(ctx, a, params, sched, &caps);
+  if (st != RECON_MVS_OK) return st;
+  st = Launch(ctx, params, kInit, a, 0.0f, 0, 0, nullptr, nullptr, nullptr, nullptr, nullptr, sched,
+              caps[0]);
   if (st != RECON_MVS_OK) return st;
   for (int it = 0; it < params->iterations; ++it) {
     for (int color = 0; color < 2; ++color) {
       a.iter = it;
       a.color = color;
-      st = Launch(ctx, params, kSweep, a, 0.0f, 0, 0, nullptr, nullptr, nullptr, nullptr, nullptr, sched);
+      st = Launch(ctx, params, kSweep, a, 0.0f, 0, 0, nullptr, nullptr, nullptr, nullptr, nullptr, sched,
+                  caps[1 + 2 * it + color]);
       if (st != RECON_MVS_OK) return st;
     }
   }
@@ -1179,15 +1292,19 @@ RECON_MVS_API int32_t recon_mvs_hip_batch_create(recon_mvs_hip_context* ctx, int
     args[j].cost = S.cost;
     args[j].support = S.support;
   }
-  // tile schedule: given, or tuned on the job with the most pixel-views
+  // tile schedule and occupancy caps: given, or tuned on the job with the
+  // most pixel-views
+  int big = 0;
+  for (int j = 1; j < num_jobs; ++j)
+    if (static_cast<double>(args[j].w) * args[j].h * args[j].nsrc >
+        static_cast<double>(args[big].w) * args[big].h * args[big].nsrc)
+      big = j;
   int sched = params->tile_schedule;
-  if (sched < 0) {
-    int big = 0;
-    for (int j = 1; j < num_jobs; ++j)
-      if (static_cast<double>(args[j].w) * args[j].h * args[j].nsrc >
-          static_cast<double>(args[big].w) * args[big].h * args[big].nsrc)
-        big = j;
-    const int32_t st = ResolveSchedule(ctx, args[big], params, &sched);
+  std::vector<int> caps;
+  {
+    int32_t st = RECON_MVS_OK;
+    if (sched < 0) st = ResolveSchedule(ctx, args[big], params, &sched);
+    if (st == RECON_MVS_OK) st = ResolveCaps(ctx, args[big], params, sched, &caps);
     if (st != RECON_MVS_OK) return st;
   }
   int tw, th;
@@ -1236,7 +1353,8 @@ RECON_MVS_API int32_t recon_mvs_hip_batch_create(recon_mvs_hip_context* ctx, int
   int32_t st = RECON_MVS_OK;
   for (int phase = -1; phase < 2 * params->iterations && st == RECON_MVS_OK; ++phase) {
     BatchLaunch bl{b->d_args, b->d_units, b->num_units, b->d_counter, phase};
-    st = Launch(ctx, params, kBatch, args[0], 0.0f, 0, 0, nullptr, nullptr, nullptr, nullptr, &bl, sched);
+    st = Launch(ctx, params, kBatch, args[0], 0.0f, 0, 0, nullptr, nullptr, nullptr, nullptr, &bl, sched,
+                caps[phase + 1]);
   }
   hipGraph_t graph = nullptr;
   const hipError_t ec = hipStreamEndCapture(ctx->stream, &graph);

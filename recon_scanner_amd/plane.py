@@ -29,7 +29,8 @@ class PlaneParams(C.Structure):
                 ("refine_iterations", C.c_int32), ("tile_schedule", C.c_int32),
                 ("min_depth_m", C.c_double), ("max_depth_m", C.c_double),
                 ("depth_perturb", C.c_double), ("normal_perturb", C.c_double),
-                ("photo_threshold", C.c_double), ("value_scale", C.c_double)]
+                ("photo_threshold", C.c_double), ("value_scale", C.c_double),
+                ("occupancy_cap", C.c_int32), ("capped_iterations", C.c_int32)]
 
     def copy(self, **kw):
         p = PlaneParams()

@@ -192,3 +192,24 @@ def test_autotune_picks_a_compiled_schedule():
     assert best in (0, 1, 2) and ms[best] > 0 and ms[best] == min(m for m in ms if m > 0)
     pm.patchmatch(0, list(range(1, 9)), p)     # -1 resolves through the cache
     assert np.isfinite(pm.depth_map(0)["depth"]).mean() > 0.5
+
+
+def test_occupancy_caps_do_not_change_results():
+    """The per-phase occupancy caps change only how many workgroups share a
+    CU: depth maps are bit-identical under every setting."""
+    say("occupancy caps: -1, 1, 2, 3 on sphere_planes 480x360, 2 iterations")
+    sc = scenes.sphere_planes(width=480, height=360, focal=625.0, supersample=1)
+    srcs = list(range(1, 9))
+    pm = plane.PlaneMVS(sc.cams, sc.images)
+    ref = None
+    for cap in (-1, 1, 2, 3):
+        p = plane.default_params(patch_size=11, window_stride=2, top_k=4, iterations=2,
+                                 min_depth_m=0.25, max_depth_m=0.60, value_scale=1.0,
+                                 tile_schedule=1, occupancy_cap=cap, capped_iterations=2)
+        pm.patchmatch(0, srcs, p)
+        g = pm.depth_map(0)
+        if ref is None:
+            ref = g
+            continue
+        for k in ("depth", "normals", "cost", "support"):
+            assert np.array_equal(np.nan_to_num(g[k], nan=-1), np.nan_to_num(ref[k], nan=-1)), (cap, k)
