@@ -187,9 +187,17 @@ constexpr int kMaxTopKNet = 8;  // top_k below the view count is limited to 8
 // Aggregated plane cost of hypothesis h at pixel (x, y).  Returns kCostInf
 // when no view is valid or the hypothesis is rejected.  view_ncc (optional)
 // receives NaN for invalid views.
+//
+// Early exit: after each view the cost is bounded below by its value when
+// every view still to come is valid with NCC = 1 (cost 0) -- adding a valid
+// zero-cost view never raises the mean of the k smallest.  Once that bound
+// exceeds `reject_at` the hypothesis cannot beat it: the remaining views are
+// skipped and kCostInf is returned (callers pass the cost to beat, with a
+// rounding margin, or kCostInf to always score fully).
 template <int HALF, int STRIDE, int FMT>
 __device__ float PlaneCost(const PlaneArgs& a, int x, int y, const RefWindow<HALF, STRIDE>& win,
-                           const Hyp& h, int* valid_views_out, float* view_ncc) {
+                           const Hyp& h, int* valid_views_out, float* view_ncc,
+                           float reject_at = kCostInf) {
   constexpr int NS = RefWindow<HALF, STRIDE>::NS;
   constexpr int S = RefWindow<HALF, STRIDE>::S;
   *valid_views_out = 0;
@@ -373,6 +381,25 @@ __device__ float PlaneCost(const PlaneArgs& a, int x, int y, const RefWindow<HAL
         }
       }
       if (view_ncc) view_ncc[v] = ncc_v;
+      if (reject_at < kCostInf) {
+        const int u = a.nsrc - 1 - v;  // views still to come
+        float lb = 0.0f;
+        if (nvalid + u <= a.top_k) {
+          // every valid view is aggregated: the mean of the seen costs and u zeros
+          if (nvalid > 0) lb = csum / static_cast<float>(nvalid + u);
+        } else {
+          // the k smallest: u zeros and the (k - u) smallest seen costs
+          const int need = a.top_k - u;
+          float sk = 0.0f;
+#pragma unroll
+          for (int k = 0; k < kMaxTopKNet; ++k) sk += (k < need) ? topk[k] : 0.0f;
+          if (need > 0) lb = sk / static_cast<float>(a.top_k);
+        }
+        if (lb > reject_at) {
+          *valid_views_out = 0;
+          return kCostInf;
+        }
+      }
     }
   } else if (view_ncc) {
     for (int v = 0; v < a.nsrc; ++v) view_ncc[v] = __builtin_nanf("");

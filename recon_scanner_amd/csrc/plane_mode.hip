@@ -158,7 +158,10 @@ __device__ __forceinline__ void SweepTile(const PlaneArgs& a, RefTile<HALF, FMT,
     }
     if (!ok) continue;
     int nv = 0;
-    const float cc = PlaneCost<HALF, STRIDE, FMT>(a, x, y, win, h, &nv, nullptr);
+    // candidates that provably cannot beat the current cost stop early (the
+    // margin covers the rounding of the bound against the full sum)
+    const float cc = PlaneCost<HALF, STRIDE, FMT>(a, x, y, win, h, &nv, nullptr,
+                                                  ccur < kCostInf ? fmaf(ccur, 1.0f + 1e-5f, 1e-7f) : kCostInf);
     if (cc < ccur) {
       ccur = cc;
       cur = make_float4(h.d, h.nx, h.ny, h.nz);
