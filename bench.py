@@ -49,7 +49,9 @@ def parse():
     ap.add_argument("--iterations", type=int, default=5)
     ap.add_argument("--storage", default=None, choices=["f32", "f16", "bf16"],
                     help="image storage (default: f32; f16 for --workload batch)")
-    ap.add_argument("--tile-schedule", type=int, default=1)
+    ap.add_argument("--tile-schedule", type=int, default=-1,
+                    help="workgroup tile schedule; -1 = autotuned with the per-phase occupancy "
+                         "(during the warm-up step)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target duration of the CPU-baseline sample (0 = skip)")
     return ap.parse_args()
@@ -277,6 +279,11 @@ def main():
             pm.patchmatch(r, [v for v in views if v != r], p)
         pm.geo_consistency(views, 0.01)
 
+    if args.tile_schedule < 0:
+        # untimed set-up: tile schedule and per-phase occupancy are tuned once
+        # per (size, views, window, format), here on view 0
+        pm.patchmatch(views[0], views[1:], p)
+        pm.synchronize()
     for _ in range(args.warmup):
         step()
     pm.synchronize()
