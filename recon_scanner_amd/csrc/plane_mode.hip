@@ -1007,6 +1007,10 @@ RECON_MVS_API int32_t recon_mvs_hip_set_views(recon_mvs_hip_context* ctx, int32_
     if (!v.pixels || v.camera.width <= 0 || v.camera.height <= 0 ||
         (v.pixel_format != RECON_MVS_PIXEL_U8 && v.pixel_format != RECON_MVS_PIXEL_F32))
       return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "bad view");
+    // sample addresses are formed in FP32: exact below 2^24 elements
+    if (static_cast<int64_t>((v.camera.width + 1 + 63) & ~63) * v.camera.height >= (int64_t{1} << 24))
+      return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT,
+                  "view too large: (width + 1 rounded up to 64) x height must stay below 2^24");
   }
   RMVS_HIP(ctx, hipStreamSynchronize(ctx->stream));
   DestroyPlaneState(ctx);
