@@ -47,7 +47,10 @@ enum {
   RECON_MVS_PIXEL_U8 = 0,   /* 0..255 gray, the reference's only format (mvs.h:176) */
   RECON_MVS_PIXEL_F32 = 1,
   RECON_MVS_PIXEL_F16 = 2,
-  RECON_MVS_PIXEL_BF16 = 3
+  RECON_MVS_PIXEL_BF16 = 3,
+  RECON_MVS_PIXEL_BGRA8 = 4 /* byte order B,G,R,A (the app's camera frames): input only,
+                               reduced to gray on upload as BgraToGray does
+                               (core/src/orchestrator/orchestrator.cpp:53-67) */
 };
 
 /*
@@ -242,11 +245,22 @@ RECON_MVS_API int32_t recon_mvs_hip_fetch_point_cloud(
 
 /* One input view (image + camera) handed to recon_mvs_hip_set_views. */
 typedef struct recon_mvs_view {
-  const void* pixels;   /* host, row-major, stride = camera.width */
-  int32_t pixel_format; /* RECON_MVS_PIXEL_U8 or RECON_MVS_PIXEL_F32 */
-  int32_t reserved0;
+  const void* pixels;   /* host, row-major */
+  int32_t pixel_format; /* RECON_MVS_PIXEL_U8, RECON_MVS_PIXEL_F32 or RECON_MVS_PIXEL_BGRA8 */
+  int32_t row_stride;   /* BGRA8: bytes per row (0 = 4 x width); U8 / F32: 0 (rows packed) */
   recon_mvs_camera camera;
 } recon_mvs_view;
+
+/*
+ * Gray conversion of a BGRA8 frame on the GPU, the reference's BgraToGray
+ * (core/src/orchestrator/orchestrator.cpp:53-67): gray = (77 R + 150 G + 29 B)
+ * >> 8 per pixel, `stride` bytes per input row, `gray` width x height packed.
+ * Host buffers; synchronous.  recon_mvs_hip_set_views applies the same
+ * conversion on upload to views given as RECON_MVS_PIXEL_BGRA8.
+ */
+RECON_MVS_API int32_t recon_mvs_hip_bgra_to_gray(recon_mvs_hip_context* ctx, const uint8_t* bgra,
+                                                 int32_t width, int32_t height, int32_t stride,
+                                                 uint8_t* gray);
 
 /*
  * Plane-mode parameters.  Image values are compared in the units they are

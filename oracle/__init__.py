@@ -408,3 +408,10 @@ def plane_cost_grad(images, cams, ref, srcs, params, xy, planes, weights=None, g
                                  None if gs is None else gs.ctypes.data)
     assert ok
     return cost, nv, gp, gs
+
+
+def bgra_to_gray(bgra):
+    """BgraToGray (core/src/orchestrator/orchestrator.cpp:53-67) restated in
+    numpy: gray = (77 R + 150 G + 29 B) >> 8 on B,G,R,A bytes."""
+    a = np.asarray(bgra, np.int64)
+    return ((77 * a[..., 2] + 150 * a[..., 1] + 29 * a[..., 0]) >> 8).astype(np.uint8)

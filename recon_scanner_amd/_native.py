@@ -28,7 +28,7 @@ ERR_CANCELLED = 5
 ERR_NO_DEPTH = 500
 ERR_GPU_FAILURE = 502
 
-PIXEL_U8, PIXEL_F32, PIXEL_F16, PIXEL_BF16 = 0, 1, 2, 3
+PIXEL_U8, PIXEL_F32, PIXEL_F16, PIXEL_BF16, PIXEL_BGRA8 = 0, 1, 2, 3, 4
 
 
 class Camera(C.Structure):

@@ -369,14 +369,31 @@ __device__ __forceinline__ uint16_t ToBf16(float v) {
 
 // Converts an uploaded view (u8 or f32 rows) into the row-pair storage layout
 // (see BufQuad): one thread per element.
+// BT.601 gray of one B,G,R,A pixel in integers, as BgraToGray
+// (core/src/orchestrator/orchestrator.cpp:53-67); the weights sum to 256.
+__device__ __forceinline__ int GrayOf(const uint8_t* px) {
+  return (77 * static_cast<int>(px[2]) + 150 * static_cast<int>(px[1]) + 29 * static_cast<int>(px[0])) >> 8;
+}
+
+__global__ void k_bgra_to_gray(const uint8_t* __restrict__ bgra, int w, int h, int stride,
+                               uint8_t* __restrict__ gray) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= w * h) return;
+  const int y = i / w, x = i % w;
+  gray[i] = static_cast<uint8_t>(GrayOf(bgra + static_cast<size_t>(y) * stride + 4 * x));
+}
+
 template <int FMT>
-__global__ void k_store_view(const void* __restrict__ src, int src_format, int w, int h, int pitch,
-                             void* __restrict__ dst) {
+__global__ void k_store_view(const void* __restrict__ src, int src_format, int src_stride, int w,
+                             int h, int pitch, void* __restrict__ dst) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= pitch * h) return;
   const int y = i / pitch, x = i % pitch;
   auto at = [&](int xx, int yy) -> float {
     if (xx >= w || yy >= h) return 0.0f;
+    if (src_format == RECON_MVS_PIXEL_BGRA8)
+      return static_cast<float>(
+          GrayOf(static_cast<const uint8_t*>(src) + static_cast<size_t>(yy) * src_stride + 4 * xx));
     return src_format == RECON_MVS_PIXEL_U8
                ? static_cast<float>(static_cast<const uint8_t*>(src)[yy * w + xx])
                : static_cast<const float*>(src)[yy * w + xx];
@@ -1005,8 +1022,12 @@ RECON_MVS_API int32_t recon_mvs_hip_set_views(recon_mvs_hip_context* ctx, int32_
   for (int i = 0; i < n; ++i) {
     const recon_mvs_view& v = views[i];
     if (!v.pixels || v.camera.width <= 0 || v.camera.height <= 0 ||
-        (v.pixel_format != RECON_MVS_PIXEL_U8 && v.pixel_format != RECON_MVS_PIXEL_F32))
+        (v.pixel_format != RECON_MVS_PIXEL_U8 && v.pixel_format != RECON_MVS_PIXEL_F32 &&
+         v.pixel_format != RECON_MVS_PIXEL_BGRA8))
       return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "bad view");
+    if (v.pixel_format == RECON_MVS_PIXEL_BGRA8 ? (v.row_stride != 0 && v.row_stride < 4 * v.camera.width)
+                                                : v.row_stride != 0)
+      return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "bad row_stride");
     // sample addresses are formed in FP32: exact below 2^24 elements
     if (static_cast<int64_t>((v.camera.width + 1 + 63) & ~63) * v.camera.height >= (int64_t{1} << 24))
       return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT,
@@ -1028,23 +1049,46 @@ RECON_MVS_API int32_t recon_mvs_hip_set_views(recon_mvs_hip_context* ctx, int32_
     const size_t bytes = static_cast<size_t>(v->pitch) * v->h * 2 * esz;
     if (hipMalloc(&v->pixels, bytes) != hipSuccess)
       return Fail(ctx, RECON_MVS_ERR_OUT_OF_MEMORY, "view allocation failed");
-    const size_t in_bytes = static_cast<size_t>(v->w) * v->h * (in.pixel_format == RECON_MVS_PIXEL_U8 ? 1 : 4);
+    const int src_stride = in.pixel_format == RECON_MVS_PIXEL_BGRA8
+                               ? (in.row_stride ? in.row_stride : 4 * v->w)
+                               : v->w * (in.pixel_format == RECON_MVS_PIXEL_U8 ? 1 : 4);
+    const size_t in_bytes = static_cast<size_t>(src_stride) * v->h;
     void* staging = Ensure(ctx, kSlotQueryImgA, in_bytes);
     if (!staging) return Fail(ctx, RECON_MVS_ERR_OUT_OF_MEMORY, "staging allocation failed");
     RMVS_HIP(ctx, hipMemcpyAsync(staging, in.pixels, in_bytes, hipMemcpyHostToDevice, ctx->stream));
     const int total = v->pitch * v->h;
     const dim3 g((total + 255) / 256), b(256);
     if (storage_format == RECON_MVS_PIXEL_F32)
-      hipLaunchKernelGGL(k_store_view<RECON_MVS_PIXEL_F32>, g, b, 0, ctx->stream, staging, in.pixel_format, v->w, v->h, v->pitch, v->pixels);
+      hipLaunchKernelGGL(k_store_view<RECON_MVS_PIXEL_F32>, g, b, 0, ctx->stream, staging, in.pixel_format, src_stride, v->w, v->h, v->pitch, v->pixels);
     else if (storage_format == RECON_MVS_PIXEL_F16)
-      hipLaunchKernelGGL(k_store_view<RECON_MVS_PIXEL_F16>, g, b, 0, ctx->stream, staging, in.pixel_format, v->w, v->h, v->pitch, v->pixels);
+      hipLaunchKernelGGL(k_store_view<RECON_MVS_PIXEL_F16>, g, b, 0, ctx->stream, staging, in.pixel_format, src_stride, v->w, v->h, v->pitch, v->pixels);
     else
-      hipLaunchKernelGGL(k_store_view<RECON_MVS_PIXEL_BF16>, g, b, 0, ctx->stream, staging, in.pixel_format, v->w, v->h, v->pitch, v->pixels);
+      hipLaunchKernelGGL(k_store_view<RECON_MVS_PIXEL_BF16>, g, b, 0, ctx->stream, staging, in.pixel_format, src_stride, v->w, v->h, v->pitch, v->pixels);
     RMVS_LAUNCH_CHECK(ctx);
     // staging is reused by the next view: finish this one first
     RMVS_HIP(ctx, hipStreamSynchronize(ctx->stream));
   }
   ctx->states.assign(ctx->views.size(), nullptr);
+  return RECON_MVS_OK;
+}
+
+RECON_MVS_API int32_t recon_mvs_hip_bgra_to_gray(recon_mvs_hip_context* ctx, const uint8_t* bgra,
+                                                 int32_t width, int32_t height, int32_t stride,
+                                                 uint8_t* gray) {
+  if (!ctx) return RECON_MVS_ERR_INVALID_ARGUMENT;
+  if (!bgra || !gray || width <= 0 || height <= 0 || stride < 4 * width)
+    return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "bgra_to_gray: bad arguments");
+  const size_t in_bytes = static_cast<size_t>(stride) * height, out_bytes = static_cast<size_t>(width) * height;
+  uint8_t* d_in = static_cast<uint8_t*>(Ensure(ctx, kSlotQueryImgA, in_bytes));
+  uint8_t* d_out = static_cast<uint8_t*>(Ensure(ctx, kSlotQueryImgB, out_bytes));
+  if (!d_in || !d_out) return Fail(ctx, RECON_MVS_ERR_OUT_OF_MEMORY, "alloc failed");
+  RMVS_HIP(ctx, hipMemcpyAsync(d_in, bgra, in_bytes, hipMemcpyHostToDevice, ctx->stream));
+  const int n = width * height;
+  hipLaunchKernelGGL(k_bgra_to_gray, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, d_in, width, height,
+                     stride, d_out);
+  RMVS_LAUNCH_CHECK(ctx);
+  RMVS_HIP(ctx, hipMemcpyAsync(gray, d_out, out_bytes, hipMemcpyDeviceToHost, ctx->stream));
+  RMVS_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return RECON_MVS_OK;
 }
 

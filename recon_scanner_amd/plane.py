@@ -47,7 +47,7 @@ class BatchJob(C.Structure):
 
 
 class View(C.Structure):
-    _fields_ = [("pixels", C.c_void_p), ("pixel_format", C.c_int32), ("reserved0", C.c_int32),
+    _fields_ = [("pixels", C.c_void_p), ("pixel_format", C.c_int32), ("row_stride", C.c_int32),
                 ("camera", N.Camera)]
 
 
@@ -81,6 +81,7 @@ def _lib():
                                                     v, v, v, v, v]
         L.recon_mvs_hip_plane_polish.argtypes = [v, C.c_int32, C.c_int32, v, P(PlaneParams),
                                                  C.c_int32]
+        L.recon_mvs_hip_bgra_to_gray.argtypes = [v, v, C.c_int32, C.c_int32, C.c_int32, v]
         _declared = True
     return L
 
@@ -107,14 +108,24 @@ class PlaneMVS:
         keep = []
         for i, (c, img) in enumerate(zip(self.cams, images)):
             img = np.asarray(img)
-            if img.dtype == np.uint8:
+            stride = 0
+            if img.ndim == 3:
+                # B,G,R,A frames (rows may be padded: any row stride >= 4 x width)
+                assert img.dtype == np.uint8 and img.shape == (c.height, c.width, 4), img.shape
+                a, fmt = img, N.PIXEL_BGRA8
+                if not (a.strides[2] == 1 and a.strides[1] == 4 and a.strides[0] >= 4 * c.width):
+                    a = np.ascontiguousarray(a)
+                stride = a.strides[0]
+            elif img.dtype == np.uint8:
                 a, fmt = np.ascontiguousarray(img), N.PIXEL_U8
             else:
                 a, fmt = np.ascontiguousarray(img, np.float32), N.PIXEL_F32
-            assert a.shape == (c.height, c.width), (a.shape, c.height, c.width)
+            if img.ndim == 2:
+                assert a.shape == (c.height, c.width), (a.shape, c.height, c.width)
             keep.append(a)
             views[i].pixels = a.ctypes.data
             views[i].pixel_format = fmt
+            views[i].row_stride = stride
             views[i].camera = c
         self._check(self.L.recon_mvs_hip_set_views(self.ctx, len(self.cams), views, FORMATS[storage]),
                     "set_views")
@@ -224,6 +235,20 @@ class PlaneMVS:
         self._check(self.L.recon_mvs_hip_timing(self.ctx, int(enable), int(reset), ms.ctypes.data,
                                                 cnt.ctypes.data), "timing")
         return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(KERNEL_CLASSES)}
+
+
+def bgra_to_gray(bgra):
+    """Gray of a B,G,R,A frame (H x W x 4 uint8, rows may be padded) on the
+    GPU: recon_mvs_hip_bgra_to_gray, the reference's BgraToGray."""
+    a = np.asarray(bgra)
+    assert a.dtype == np.uint8 and a.ndim == 3 and a.shape[2] == 4
+    if not (a.strides[2] == 1 and a.strides[1] == 4):
+        a = np.ascontiguousarray(a)
+    h, w = a.shape[:2]
+    out = np.zeros((h, w), np.uint8)
+    mvs.check(_lib().recon_mvs_hip_bgra_to_gray(mvs.context(), a.ctypes.data, w, h, a.strides[0],
+                                                out.ctypes.data), "bgra_to_gray")
+    return out
 
 
 def round_storage(img, storage):

@@ -192,3 +192,18 @@ def test_camera_math_with_distortion_matches_reference():
     oracle.restated().orc_project_batch(C.byref(cam), 120, b.ctypes.data, qu.ctypes.data, ok2.ctypes.data)
     assert bits_equal(pu, qu) and bits_equal(ok1, ok2)
     assert np.max(np.hypot(*(pu - uv).T)) < 1e-6   # round trip, test_mvs.cpp:247
+
+
+def test_bgra_to_gray_restatement_closed_forms():
+    """BgraToGray (orchestrator.cpp:53-67): the BT.601 integer weights sum to
+    256, so equal channels give the channel back; pure primaries give the
+    weights scaled by 255/256; alpha is ignored; B and R are not swapped."""
+    v = np.arange(256, dtype=np.uint8)
+    eq = np.stack([v, v, v, np.full(256, 7, np.uint8)], -1)[None]
+    assert np.array_equal(oracle.bgra_to_gray(eq)[0], v)
+    px = np.array([[[255, 0, 0, 0], [0, 255, 0, 0], [0, 0, 255, 0], [0, 0, 0, 255]]], np.uint8)
+    assert oracle.bgra_to_gray(px)[0].tolist() == [29 * 255 >> 8, 150 * 255 >> 8, 77 * 255 >> 8, 0]
+    rng = np.random.default_rng(0)
+    a = rng.integers(0, 256, (37, 53, 4), dtype=np.uint8)
+    ref = [[(77 * int(p[2]) + 150 * int(p[1]) + 29 * int(p[0])) // 256 for p in row] for row in a]
+    assert np.array_equal(oracle.bgra_to_gray(a), np.array(ref, np.uint8))

@@ -213,3 +213,35 @@ def test_occupancy_caps_do_not_change_results():
             continue
         for k in ("depth", "normals", "cost", "support"):
             assert np.array_equal(np.nan_to_num(g[k], nan=-1), np.nan_to_num(ref[k], nan=-1)), (cap, k)
+
+
+def test_bgra_frames_gray_conversion_bit_exact():
+    """B,G,R,A camera frames: the GPU conversion equals BgraToGray
+    (orchestrator.cpp:53-67, restated in the oracle), standalone and fused into
+    the upload; depth maps from BGRA views equal those from the gray views."""
+    say("BGRA8 upload, sphere_arc 160x120, padded rows")
+    sc = scenes.sphere_arc(width=160, height=120, focal=250.0)
+    rng = np.random.default_rng(5)
+    frames, grays = [], []
+    for i in range(sc.num_views):
+        g = sc.u8(i).astype(np.int32)
+        tint = rng.integers(-20, 21, size=(3,))
+        bgr = np.clip(g[..., None] + tint[None, None, :] + rng.integers(-3, 4, g.shape + (3,)), 0, 255)
+        buf = np.zeros((120, 160 * 4 + 36), np.uint8)            # rows padded to 676 bytes
+        view = buf[:, :640].reshape(120, 160, 4)
+        view[..., :3] = bgr
+        view[..., 3] = 255
+        frames.append(view)
+        grays.append(oracle.bgra_to_gray(view))
+    assert np.array_equal(plane.bgra_to_gray(frames[1]), grays[1])
+    p = plane.default_params(patch_size=7, window_stride=1, top_k=0, iterations=2,
+                             min_depth_m=0.25, max_depth_m=0.45, value_scale=255.0, tile_schedule=0)
+    a = plane.PlaneMVS(sc.cams, frames)
+    a.patchmatch(0, [1, 2, 3, 4], p)
+    da = a.depth_map(0)
+    b = plane.PlaneMVS(sc.cams, grays)
+    b.patchmatch(0, [1, 2, 3, 4], p)
+    db = b.depth_map(0)
+    for k in ("depth", "normals", "cost"):
+        assert np.array_equal(np.nan_to_num(da[k], nan=-1), np.nan_to_num(db[k], nan=-1)), k
+    assert np.isfinite(da["depth"]).mean() > 0.9
