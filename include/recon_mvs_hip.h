@@ -441,6 +441,38 @@ RECON_MVS_API int32_t recon_mvs_hip_plane_polish(recon_mvs_hip_context* ctx, int
                                                  const recon_mvs_plane_params* params, int32_t steps);
 
 /*
+ * The MVS stage of a reconstruction over keyframe arrays, the body of
+ * Orchestrator::StageMvs (core/src/orchestrator/orchestrator.cpp:942-1071).
+ * For every keyframe r with an image (gray[r] != NULL, cams[r].width x height
+ * uint8): SelectSourceViews with r's intrinsics for every pose, minus r and
+ * keyframes without an image, at most num_views; then
+ *   RECON_MVS_STAGE_REFERENCE: PatchMatchStereo + RefineSubpixel per keyframe
+ *     (the reference's algorithm, bit-exact for the same num_threads);
+ *   RECON_MVS_STAGE_PLANE: every keyframe in one plane-mode batch (`plane`,
+ *     NULL = parameters derived from cfg: its window (stride 2 from 9 px),
+ *     iterations, depth range and photometric threshold), sub-pixel
+ *     refinement and the 10-step gradient polish of every map, then geometric
+ *     consistency of each map against its sources (geo_consistency_threshold;
+ *     pixels seen consistently by fewer than min_view_consistency maps
+ *     dropped); replaces the views resident in ctx;
+ * and FuseDepthMaps over the produced maps in keyframe order; fetch the cloud
+ * with recon_mvs_hip_fetch_point_cloud (*num_points points).  depth,
+ * support, confidence (n pointers each, may be NULL or hold NULLs) receive
+ * each keyframe's map (DepthMap fields, mvs.h:85-109; NaN / 0 where invalid
+ * or not produced).  *produced = depth maps produced (0: the caller's
+ * RECON_ERR_MVS_NO_DEPTH, orchestrator.cpp:1041).
+ */
+enum { RECON_MVS_STAGE_REFERENCE = 0, RECON_MVS_STAGE_PLANE = 1 };
+RECON_MVS_API int32_t recon_mvs_hip_run_mvs_stage(recon_mvs_hip_context* ctx, int32_t n,
+                                                  const uint8_t* const* gray,
+                                                  const recon_mvs_camera* cams,
+                                                  const recon_mvs_config* cfg, int32_t mode,
+                                                  const recon_mvs_plane_params* plane,
+                                                  float* const* depth, uint8_t* const* support,
+                                                  float* const* confidence, int32_t* produced,
+                                                  int64_t* num_points);
+
+/*
  * Kernel timers.  enable != 0 records HIP events around every launch of the
  * plane-mode kernels on the context's stream; ms / counts (8 entries each,
  * may be NULL) receive the accumulated milliseconds and launch counts per

@@ -36,6 +36,7 @@ HIP_SOURCES = [
     ("plane_mode.hip", ["-ffp-contract=off"]),
     ("plane_adjoint.hip", ["-ffp-contract=off"]),
     ("mvs_api.cpp", ["-ffp-contract=off"]),
+    ("mvs_stage.cpp", ["-ffp-contract=off"]),
 ]
 
 

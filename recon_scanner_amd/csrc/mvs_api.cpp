@@ -5,6 +5,7 @@
 // the compute runs on HIP device 0.  Also the camera methods of
 // include/recon/camera.h (core/src/util/camera.cpp:33-162).
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -12,6 +13,7 @@
 #include "mvs_common.hpp"
 #include "recon/camera.h"
 #include "recon/mvs.h"
+#include "recon/mvs_stage.h"
 #include "recon_mvs_hip.h"
 
 namespace recon {
@@ -282,6 +284,55 @@ PointCloud FuseDepthMaps(const std::vector<const DepthMap*>& depth_maps,
     cloud.normals[i] = Vec3d(nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]);
   }
   return cloud;
+}
+
+uint32_t RunMvsStage(const std::vector<const uint8_t*>& gray, const std::vector<CameraPose>& poses,
+                     const std::vector<CameraIntrinsics>& intrinsics, const MvsConfig& config,
+                     MvsStageMode mode, MvsStageResult* out) {
+  if (!out) return 0;
+  *out = MvsStageResult();
+  const size_t n = gray.size();
+  if (poses.size() != n || intrinsics.size() != n) return 0;
+  std::vector<recon_mvs_camera> cams(n);
+  for (size_t k = 0; k < n; ++k) cams[k] = ToCam(poses[k], intrinsics[k]);
+  out->depth_maps.resize(n);
+  std::vector<float*> dptr(n, nullptr), cptr(n, nullptr);
+  std::vector<uint8_t*> sptr(n, nullptr);
+  for (size_t k = 0; k < n; ++k) {
+    if (!gray[k]) continue;
+    out->depth_maps[k].Allocate(intrinsics[k].width, intrinsics[k].height);
+    dptr[k] = out->depth_maps[k].depth.data();
+    sptr[k] = out->depth_maps[k].support_count.data();
+    cptr[k] = out->depth_maps[k].confidence.data();
+  }
+  const recon_mvs_config cfg = ToConfig(config);
+  int32_t produced = 0;
+  int64_t npts = 0;
+  std::lock_guard<std::mutex> lock(g_mu);
+  Check(recon_mvs_hip_run_mvs_stage(Ctx(), static_cast<int32_t>(n), gray.data(), cams.data(), &cfg,
+                                    static_cast<int32_t>(mode), nullptr, dptr.data(), sptr.data(), cptr.data(),
+                                    &produced, &npts),
+        "RunMvsStage");
+  for (size_t k = 0; k < n; ++k) {
+    DepthMap& d = out->depth_maps[k];
+    bool any = false;
+    for (float z : d.depth) any = any || std::isfinite(z);
+    if (!any) d = DepthMap();
+  }
+  std::vector<double> pos(3 * npts), nrm(3 * npts);
+  out->cloud.confidences.resize(npts);
+  out->cloud.colors.assign(3 * npts, 0.0f);
+  Check(recon_mvs_hip_fetch_point_cloud(Ctx(), npts, pos.data(), nrm.data(), out->cloud.colors.data(),
+                                        out->cloud.confidences.data(), nullptr),
+        "RunMvsStage");
+  out->cloud.positions.resize(npts);
+  out->cloud.normals.resize(npts);
+  for (int64_t i = 0; i < npts; ++i) {
+    out->cloud.positions[i] = Vec3d(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+    out->cloud.normals[i] = Vec3d(nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]);
+  }
+  out->produced = static_cast<uint32_t>(produced);
+  return out->produced;
 }
 
 }  // namespace recon

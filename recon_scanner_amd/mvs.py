@@ -145,6 +145,10 @@ def lib():
                                                     P(N.Camera), P(N.Config), P(C.c_int64)]
         L.recon_mvs_hip_fetch_point_cloud.argtypes = [ctxp, C.c_int64, C.c_void_p, C.c_void_p,
                                                       C.c_void_p, C.c_void_p, C.c_void_p]
+        L.recon_mvs_hip_run_mvs_stage.argtypes = [ctxp, C.c_int32, C.c_void_p, C.c_void_p,
+                                                  P(N.Config), C.c_int32, C.c_void_p, C.c_void_p,
+                                                  C.c_void_p, C.c_void_p, P(C.c_int32),
+                                                  P(C.c_int64)]
         _lib = L
     return _lib
 
@@ -294,6 +298,53 @@ def FuseDepthMaps(depth_maps, poses, intrinsics, config) -> PointCloud:
                                                     cloud.confidences.ctypes.data,
                                                     cloud.voxel_keys.ctypes.data), "fetch")
     return cloud
+
+
+STAGE_REFERENCE, STAGE_PLANE = 0, 1
+
+
+def RunMvsStage(gray, cams, config, mode=STAGE_REFERENCE, plane_params=None):
+    """The MVS stage over keyframes (Orchestrator::StageMvs, orchestrator.cpp:
+    942-1071; recon_mvs_hip_run_mvs_stage).  gray: list of uint8 images or
+    None; cams: list of N.Camera (pose + intrinsics).  Returns (produced,
+    [DepthMap or None per keyframe], PointCloud)."""
+    n = len(gray)
+    imgs = [None if g is None else np.ascontiguousarray(g, np.uint8) for g in gray]
+    gp = (C.c_void_p * max(n, 1))(*[None if g is None else g.ctypes.data for g in imgs])
+    cams_arr = (N.Camera * max(n, 1))(*cams)
+    maps = []
+    dp = (C.c_void_p * max(n, 1))()
+    sp = (C.c_void_p * max(n, 1))()
+    cp = (C.c_void_p * max(n, 1))()
+    for k in range(n):
+        if imgs[k] is None:
+            maps.append(None)
+            continue
+        dm = DepthMap()
+        dm.Allocate(cams[k].width, cams[k].height)
+        maps.append(dm)
+        dp[k], sp[k], cp[k] = dm.depth.ctypes.data, dm.support_count.ctypes.data, dm.confidence.ctypes.data
+    produced = C.c_int32(0)
+    npts = C.c_int64(0)
+    check(lib().recon_mvs_hip_run_mvs_stage(context(), n, gp, cams_arr, C.byref(config), mode,
+                                            None if plane_params is None else C.byref(plane_params),
+                                            dp, sp, cp, C.byref(produced), C.byref(npts)),
+          "RunMvsStage")
+    m = int(npts.value)
+    cloud = PointCloud()
+    cloud.positions = np.zeros((m, 3))
+    cloud.normals = np.zeros((m, 3))
+    cloud.colors = np.zeros(3 * m, np.float32)
+    cloud.confidences = np.zeros(m, np.float32)
+    cloud.voxel_keys = np.zeros(m, np.int64)
+    if m:
+        check(lib().recon_mvs_hip_fetch_point_cloud(context(), m, cloud.positions.ctypes.data,
+                                                    cloud.normals.ctypes.data,
+                                                    cloud.colors.ctypes.data,
+                                                    cloud.confidences.ctypes.data,
+                                                    cloud.voxel_keys.ctypes.data), "fetch")
+    maps = [None if (d is None or not np.isfinite(d.depth).any()) else d for d in maps]
+    return int(produced.value), maps, cloud
 
 
 def EvaluateCandidates(ref_gray, src_gray, ref_cam: N.Camera, src_cams, config, u_v_inv):

@@ -1,0 +1,104 @@
+"""The MVS stage over keyframes (recon_mvs_hip_run_mvs_stage, the body of
+Orchestrator::StageMvs, core/src/orchestrator/orchestrator.cpp:942-1071).
+
+* reference mode: depth maps and the fused cloud equal, bit for bit, the same
+  composition of the reference's own SelectSourceViews, PatchMatchStereo,
+  RefineSubpixel and FuseDepthMaps (compiled in place from the reference's
+  sources), with StageMvs's selection rules (self and image-less keyframes
+  removed, at most num_views sources);
+* plane mode (one batch + geometric consistency): the fused points lie on the
+  analytic surfaces (median distance below 1 mm).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from recon_scanner_amd import mvs, scenes
+
+from helpers import sort_cloud
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def keyframes():
+    sc = scenes.sphere_planes(width=160, height=120, focal=208.0, n_views=5, supersample=1)
+    gray = [sc.u8(i) for i in range(sc.num_views)]
+    gray[3] = None   # a keyframe without an image is skipped and never a source
+    cfg = mvs.MvsConfig(patch_size=7, iterations=2, num_threads=3, min_depth_m=0.25,
+                        max_depth_m=0.60, num_views=3)
+    return sc, gray, cfg
+
+
+def test_reference_mode_equals_reference_composition(keyframes):
+    sc, gray, cfg = keyframes
+    if not oracle.reference_available():
+        pytest.skip("the reference's sources were not compiled here")
+    say("stage, reference mode, 5 keyframes 160x120")
+    produced, maps, cloud = mvs.RunMvsStage(gray, sc.cams, cfg, mvs.STAGE_REFERENCE)
+    ref_maps, ref_cams, made = [], [], []
+    for r in range(len(gray)):
+        if gray[r] is None:
+            continue
+        sel = oracle.select_views("ref", sc.cams[r], sc.cams, cfg)
+        kept = [s for s in sel if s != r and gray[s] is not None][:cfg.num_views]
+        if not kept:
+            continue
+        f, d, s, c = oracle.ref_patchmatch(gray[r], [gray[k] for k in kept], sc.cams[r], sc.cams,
+                                           sc.cams, kept, cfg)
+        if f == 0:
+            continue
+        _, d, c = oracle.ref_refine(gray[r], [gray[k] for k in kept], sc.cams[r], sc.cams, sc.cams,
+                                    kept, cfg, d, s, c)
+        ref_maps.append((d, s, c))
+        ref_cams.append(sc.cams[r])
+        made.append(r)
+    say("produced", produced, "reference", len(made), "points", cloud.size())
+    assert produced == len(made) > 0
+    for r, (d, s, c) in zip(made, ref_maps):
+        assert maps[r] is not None
+        assert maps[r].depth.reshape(d.shape).tobytes() == d.tobytes(), r
+        assert maps[r].confidence.reshape(c.shape).tobytes() == c.tobytes(), r
+        assert np.array_equal(maps[r].support_count.reshape(s.shape), s), r
+    assert maps[3] is None
+    rc = oracle.fuse("ref", ref_maps, ref_cams, cfg)
+    a = sort_cloud(cloud.positions, cloud.normals, cloud.confidences)
+    b = sort_cloud(rc["positions"], rc["normals"], rc["confidences"])
+    for x, y in zip(a, b):
+        assert np.asarray(x).tobytes() == np.asarray(y).tobytes()
+
+
+def surface_distance(p):
+    """Distance of world points to the sphere_planes scene (sphere r = 50 mm
+    at z = 0.35 m; two tilted planes through (0, 0, 0.47))."""
+    na = np.array([0.42, 0.10, -0.90]); na /= np.linalg.norm(na)
+    nb = np.array([-0.38, -0.15, -0.90]); nb /= np.linalg.norm(nb)
+    c = np.array([0.0, 0.0, 0.35])
+    ds = np.abs(np.linalg.norm(p - c, axis=1) - 0.05)
+    da = np.abs((p - [0, 0, 0.47]) @ na)
+    db = np.abs((p - [0, 0, 0.47]) @ nb)
+    return np.minimum(ds, np.minimum(da, db))
+
+
+def test_plane_mode_cloud_on_the_surfaces(keyframes):
+    """Plane mode's cloud is at least as close to the surfaces as reference
+    mode's on the same keyframes, and within one pixel footprint (Z / f ~
+    1.9 mm here) at the median."""
+    sc, gray, cfg = keyframes
+    say("stage, plane mode (batch + consistency), 5 keyframes 160x120")
+    produced, maps, cloud = mvs.RunMvsStage(gray, sc.cams, cfg, mvs.STAGE_PLANE)
+    _, _, ref_cloud = mvs.RunMvsStage(gray, sc.cams, cfg, mvs.STAGE_REFERENCE)
+    dist = surface_distance(cloud.positions)
+    dref = surface_distance(ref_cloud.positions)
+    say("produced", produced, "points", cloud.size(), "median distance mm",
+        float(np.median(dist)) * 1e3, "reference mode", float(np.median(dref)) * 1e3,
+        "points", ref_cloud.size())
+    assert produced == 4
+    assert maps[3] is None
+    assert cloud.size() > 2000
+    assert np.median(dist) <= np.median(dref)
+    assert np.median(dist) < 0.4 / 208.0
