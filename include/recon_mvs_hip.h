@@ -235,6 +235,53 @@ RECON_MVS_API int32_t recon_mvs_hip_fetch_point_cloud(
     recon_mvs_hip_context* ctx, int64_t capacity, double* positions,
     double* normals, float* colors, float* confidences, int64_t* voxel_keys);
 
+/*
+ * The same cloud as Poisson input (OrientedPointCloud, poisson_mesh.h:25-38),
+ * converted as MVS output reaches meshing: PointCloud -> UnifiedPointCloud
+ * (orchestrator.cpp:1056-1063) -> UnifiedPointCloud::ToOrientedCloud
+ * (orchestrator.cpp:181-192): positions, normals and colors as fetched,
+ * weights[i] = max(0, confidence[i]).  Any pointer may be NULL;
+ * *num_points (may be NULL) receives the cloud's size, and with the four
+ * buffers NULL the call is only that size query.
+ */
+RECON_MVS_API int32_t recon_mvs_hip_fetch_oriented_cloud(
+    recon_mvs_hip_context* ctx, int64_t capacity, double* positions,
+    double* normals, float* colors, float* weights, int64_t* num_points);
+
+/*
+ * Depth-map spill to the session cache (docs/13_内存与降级.md §3.3.2-3.3.4:
+ * `<cache>/mvs_depth/ref_%05d.f32` "with confidence and support counts",
+ * written to *.tmp then renamed, recorded in `<cache>/cache_manifest.json`
+ * with path, bytes, sha256 and stage).  Host only, no context.
+ *
+ * File: 32-byte header {"RMVD", uint32 version 1, int32 width, height,
+ * ref_index, uint32 planes 3, 2 x uint32 0}, then float32 planes of
+ * width x height: depth (NaN = invalid), confidence, support count.
+ * Manifest: {"items": [{"path": "mvs_depth/ref_00007.f32", "bytes": B,
+ * "sha256": "<hex>", "stage": "mvs"}, ...]}; items of other paths are kept.
+ *
+ * spill: creates cache_dir and cache_dir/mvs_depth as needed; replaces an
+ * earlier map of the same ref_index; sha256_hex (65 bytes, may be NULL)
+ * receives the file's digest.  ref_index in [0, 99999].
+ */
+RECON_MVS_API int32_t recon_mvs_spill_depth_map(const char* cache_dir, int32_t ref_index,
+                                                int32_t width, int32_t height, const float* depth,
+                                                const uint8_t* support, const float* confidence,
+                                                char* sha256_hex);
+
+/*
+ * Reads a spilled map back.  The file's sha256 must match its manifest item
+ * and the file must be well formed; otherwise the item (file and manifest
+ * entry) is deleted and RECON_MVS_ERR_NO_DEPTH returned: the caller
+ * recomputes the map (§3.3.4).  A missing file also returns
+ * RECON_MVS_ERR_NO_DEPTH.  With depth, support and confidence all NULL it
+ * only reports *width / *height; otherwise `capacity` (pixels each non-NULL
+ * buffer holds) must be at least width x height.
+ */
+RECON_MVS_API int32_t recon_mvs_load_depth_map(const char* cache_dir, int32_t ref_index,
+                                               int64_t capacity, int32_t* width, int32_t* height,
+                                               float* depth, uint8_t* support, float* confidence);
+
 /* ------------------------------------------------------------------------- */
 /* Plane mode: slanted-plane PatchMatch with homography warps, FP32 on the    */
 /* GPU, views resident in HBM.  It has no CPU counterpart in the reference:  */
@@ -316,6 +363,23 @@ RECON_MVS_API int32_t recon_mvs_hip_plane_patchmatch(recon_mvs_hip_context* ctx,
                                                      int32_t ref_view, int32_t num_src,
                                                      const int32_t* src_views,
                                                      const recon_mvs_plane_params* params);
+
+/*
+ * The same with sparse-point seeding, PatchMatchStereo's seed_points
+ * (mvs.h:193-206; patchmatch_stereo.cpp:499-533; docs/05_单目管线.md §5.3.2):
+ * num_seeds rows of (u, v, depth_m) in ref_view's pixels.  Seeds outside the
+ * image or with depth <= 0 are dropped; each 32 x 32 pixel cell holding seeds
+ * takes their mean depth (FP64, seed order), and its pixels start on the
+ * fronto-parallel plane at that depth (the reference's depth-only
+ * hypothesis) instead of a random plane; other pixels start random.  Then
+ * `iterations` sweeps as above.  num_seeds = 0 is the unseeded call.
+ */
+RECON_MVS_API int32_t recon_mvs_hip_plane_patchmatch_seeded(recon_mvs_hip_context* ctx,
+                                                            int32_t ref_view, int32_t num_src,
+                                                            const int32_t* src_views,
+                                                            const recon_mvs_plane_params* params,
+                                                            const double* seed_points,
+                                                            int32_t num_seeds);
 
 /*
  * Plane-mode sub-pixel refinement of slot ref_view (the RefineSubpixel role,

@@ -351,6 +351,43 @@ def plane_patchmatch(images, cams, ref, srcs, params, init_planes=None):
     return planes, cost, sup
 
 
+def seed_cells(w, h, seeds, grid=32):
+    """Sparse-point seeding of PatchMatchStereo (core/src/mvs/patchmatch_stereo.cpp:
+    499-533): seeds (u, v, depth) outside the image or with depth <= 0 are
+    dropped; the rest fall into grid x grid pixel cells (column
+    min(gx - 1, int(u) // grid)); a cell's depth is the mean of its seeds,
+    summed in seed order.  Returns (gy, gx) float64, NaN for empty cells."""
+    gx, gy = (w + grid - 1) // grid, (h + grid - 1) // grid
+    sums = [[0.0] * gx for _ in range(gy)]
+    counts = [[0] * gx for _ in range(gy)]
+    for su, sv, sd in np.asarray(seeds, np.float64).reshape(-1, 3).tolist():
+        if su < 0 or sv < 0 or su >= w or sv >= h or sd <= 0.0:
+            continue
+        cx, cy = min(gx - 1, int(su) // grid), min(gy - 1, int(sv) // grid)
+        sums[cy][cx] += sd
+        counts[cy][cx] += 1
+    out = np.full((gy, gx), np.nan)
+    for cy in range(gy):
+        for cx in range(gx):
+            if counts[cy][cx]:
+                out[cy, cx] = sums[cy][cx] / counts[cy][cx]
+    return out
+
+
+def seeded_init_planes(w, h, seeds, grid=32):
+    """Plane-mode seeded init: a pixel of a seeded cell starts on the
+    fronto-parallel plane (normal (0, 0, -1)) at the cell's depth (float32),
+    the reference's depth-only hypothesis; other rows are NaN (random init)."""
+    cells = seed_cells(w, h, seeds, grid)
+    ys, xs = np.mgrid[0:h, 0:w]
+    d = cells[np.minimum(ys // grid, cells.shape[0] - 1), np.minimum(xs // grid, cells.shape[1] - 1)]
+    planes = np.full((h, w, 4), np.nan, np.float32)
+    ok = np.isfinite(d)
+    planes[ok] = np.stack([d[ok].astype(np.float32), np.zeros(ok.sum(), np.float32),
+                           np.zeros(ok.sum(), np.float32), np.full(ok.sum(), -1.0, np.float32)], 1)
+    return planes
+
+
 def geo_consistency(depths, cams, thr):
     lib = restated()
     ds = [np.ascontiguousarray(d, np.float32) for d in depths]
@@ -415,3 +452,61 @@ def bgra_to_gray(bgra):
     numpy: gray = (77 R + 150 G + 29 B) >> 8 on B,G,R,A bytes."""
     a = np.asarray(bgra, np.int64)
     return ((77 * a[..., 2] + 150 * a[..., 1] + 29 * a[..., 0]) >> 8).astype(np.uint8)
+
+
+# ---------------------------------------------------------------------------
+# Downstream formats (host restatements; the reference has them as a spec or
+# as plain C++ over std::vector, no kernels)
+# ---------------------------------------------------------------------------
+
+
+def to_oriented_cloud(positions, normals, colors, confidences):
+    """UnifiedPointCloud::ToOrientedCloud (core/src/orchestrator/orchestrator.cpp:
+    181-192): positions / normals / colors copied, weight = max(0, confidence),
+    1.0 for points past the end of the confidence array."""
+    n = len(positions)
+    w = np.ones(n, np.float32)
+    m = min(n, len(confidences))
+    w[:m] = np.maximum(np.float32(0.0), np.asarray(confidences, np.float32)[:m])
+    return (np.array(positions, np.float64), np.array(normals, np.float64),
+            np.array(colors, np.float32), w)
+
+
+SPILL_MAGIC = b"RMVD"
+
+
+def spill_encode(ref_index, width, height, depth, support, confidence):
+    """Bytes of mvs_depth/ref_%05d.f32 (docs/13_内存与降级.md §3.3.2: the depth
+    map "with confidence and support counts"): a 32-byte header
+    (magic, version 1, width, height, ref_index, 3 planes, 2 reserved) and the
+    float32 planes depth, confidence, support."""
+    hdr = SPILL_MAGIC + np.array([1], "<u4").tobytes() + \
+        np.array([width, height, ref_index], "<i4").tobytes() + np.array([3, 0, 0], "<u4").tobytes()
+    planes = np.concatenate([np.asarray(depth, "<f4").reshape(-1),
+                             np.asarray(confidence, "<f4").reshape(-1),
+                             np.asarray(support, np.uint8).reshape(-1).astype("<f4")])
+    return hdr + planes.tobytes()
+
+
+def spill_decode(blob):
+    """Inverse of spill_encode: (ref_index, width, height, depth, support,
+    confidence) or None when the bytes are not a well-formed map."""
+    if len(blob) < 32 or blob[:4] != SPILL_MAGIC:
+        return None
+    ver = int(np.frombuffer(blob, "<u4", 1, 4)[0])
+    w, h, ref = (int(x) for x in np.frombuffer(blob, "<i4", 3, 8))
+    planes = int(np.frombuffer(blob, "<u4", 1, 20)[0])
+    if ver != 1 or planes != 3 or w <= 0 or h <= 0 or len(blob) != 32 + 12 * w * h:
+        return None
+    f = np.frombuffer(blob, "<f4", 3 * w * h, 32)
+    sup = f[2 * w * h:]
+    if not np.all((sup >= 0) & (sup <= 255) & (sup == np.floor(sup))):
+        return None
+    return ref, w, h, f[:w * h].copy(), sup.astype(np.uint8), f[w * h:2 * w * h].copy()
+
+
+def spill_manifest_item(ref_index, blob):
+    """The manifest entry the spill records (§3.3.3: path, bytes, sha256, stage)."""
+    import hashlib
+    return {"path": "mvs_depth/ref_%05d.f32" % ref_index, "bytes": len(blob),
+            "sha256": hashlib.sha256(blob).hexdigest(), "stage": "mvs"}

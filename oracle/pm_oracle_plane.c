@@ -351,7 +351,9 @@ int orc_plane_eval(int nviews, const float* const* imgs, const recon_mvs_camera*
 
 /* Exported: full plane-mode PatchMatch.  planes: P x 4 floats (d, nx, ny, nz);
    cost: P doubles; support: P bytes.  If init_planes is non-NULL the random
-   init is replaced by those planes (then scored) and `iterations` sweeps run. */
+   init is replaced by those planes (then scored) and `iterations` sweeps run;
+   rows whose depth is NaN keep the random plane (seeded init: only pixels
+   with a seed are given). */
 int orc_plane_patchmatch(int nviews, const float* const* imgs, const recon_mvs_camera* cams,
                          int ref, int nsrc, const int32_t* srcs, const orc_plane_params* p,
                          const float* init_planes, float* planes, double* cost,
@@ -367,7 +369,7 @@ int orc_plane_patchmatch(int nviews, const float* const* imgs, const recon_mvs_c
     float ray[3];
     pixel_ray(&a, x, y, ray);
     hyp h;
-    if (init_planes) {
+    if (init_planes && init_planes[4 * idx] == init_planes[4 * idx]) {
       h.d = init_planes[4 * idx];
       h.nx = init_planes[4 * idx + 1];
       h.ny = init_planes[4 * idx + 2];

@@ -37,6 +37,7 @@ HIP_SOURCES = [
     ("plane_adjoint.hip", ["-ffp-contract=off"]),
     ("mvs_api.cpp", ["-ffp-contract=off"]),
     ("mvs_stage.cpp", ["-ffp-contract=off"]),
+    ("spill.cpp", []),
 ]
 
 

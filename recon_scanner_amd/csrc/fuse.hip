@@ -272,4 +272,23 @@ RECON_MVS_API int32_t recon_mvs_hip_fetch_point_cloud(recon_mvs_hip_context* ctx
   return RECON_MVS_OK;
 }
 
+RECON_MVS_API int32_t recon_mvs_hip_fetch_oriented_cloud(recon_mvs_hip_context* ctx, int64_t capacity,
+                                                         double* positions, double* normals,
+                                                         float* colors, float* weights,
+                                                         int64_t* num_points) {
+  if (!ctx) return RECON_MVS_ERR_INVALID_ARGUMENT;
+  const int64_t n = static_cast<int64_t>(ctx->cloud_conf.size());
+  if (num_points) *num_points = n;
+  if (!positions && !normals && !colors && !weights) return RECON_MVS_OK;  // size query
+  if (capacity < n) return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "fetch: capacity below cloud size");
+  if (positions) std::memcpy(positions, ctx->cloud_pos.data(), 24 * n);
+  if (normals) std::memcpy(normals, ctx->cloud_nrm.data(), 24 * n);
+  if (colors) std::memset(colors, 0, 12 * n);
+  // the Poisson point weight is the confidence clamped at 0
+  // (UnifiedPointCloud::ToOrientedCloud, orchestrator.cpp:181-192)
+  if (weights)
+    for (int64_t i = 0; i < n; ++i) weights[i] = std::max(0.0f, ctx->cloud_conf[i]);
+  return RECON_MVS_OK;
+}
+
 }  // extern "C"

@@ -13,6 +13,7 @@
 #include "mvs_common.hpp"
 #include "recon/camera.h"
 #include "recon/mvs.h"
+#include "recon/mvs_cache.h"
 #include "recon/mvs_stage.h"
 #include "recon_mvs_hip.h"
 
@@ -333,6 +334,43 @@ uint32_t RunMvsStage(const std::vector<const uint8_t*>& gray, const std::vector<
   }
   out->produced = static_cast<uint32_t>(produced);
   return out->produced;
+}
+
+OrientedPointCloud ToOrientedCloud(const PointCloud& cloud) {
+  OrientedPointCloud out;
+  out.positions = cloud.positions;
+  out.normals = cloud.normals;
+  out.colors = cloud.colors;
+  out.weights.assign(cloud.positions.size(), 1.0f);
+  for (size_t i = 0; i < cloud.positions.size() && i < cloud.confidences.size(); ++i)
+    out.weights[i] = std::max(0.0f, cloud.confidences[i]);
+  return out;
+}
+
+std::string SpillDepthMap(const std::string& cache_dir, int ref_index, const DepthMap& map) {
+  const size_t n = static_cast<size_t>(std::max(map.width, 0)) * std::max(map.height, 0);
+  if (n == 0 || map.depth.size() != n || map.support_count.size() != n || map.confidence.size() != n)
+    return "";
+  char hex[65] = {0};
+  if (recon_mvs_spill_depth_map(cache_dir.c_str(), ref_index, map.width, map.height, map.depth.data(),
+                                map.support_count.data(), map.confidence.data(), hex) != RECON_MVS_OK)
+    return "";
+  return hex;
+}
+
+bool LoadDepthMap(const std::string& cache_dir, int ref_index, DepthMap* map) {
+  if (!map) return false;
+  int32_t w = 0, h = 0;
+  if (recon_mvs_load_depth_map(cache_dir.c_str(), ref_index, 0, &w, &h, nullptr, nullptr, nullptr) !=
+      RECON_MVS_OK)
+    return false;
+  DepthMap m;
+  m.Allocate(w, h);
+  if (recon_mvs_load_depth_map(cache_dir.c_str(), ref_index, static_cast<int64_t>(w) * h, &w, &h,
+                               m.depth.data(), m.support_count.data(), m.confidence.data()) != RECON_MVS_OK)
+    return false;
+  *map = std::move(m);
+  return true;
 }
 
 }  // namespace recon

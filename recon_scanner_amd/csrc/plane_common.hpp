@@ -33,6 +33,11 @@ struct PlaneState {
   uint8_t* support = nullptr;
   uint8_t* geo_support = nullptr;
   bool has_geo = false;
+  // sparse-point seeding: per 32 x 32 pixel cell the mean seed depth (0 = no
+  // seed), device copy and its host staging (kept until the copy has run)
+  float* seed_cells = nullptr;
+  int seed_capacity = 0;
+  std::vector<float> seed_host;
 };
 
 namespace rmvs {
@@ -72,6 +77,8 @@ struct PlaneArgs {
   float4* plane;
   float* cost;
   uint8_t* support;
+  const float* seed_cells;      // init: mean seed depth per 32 x 32 cell (0 = none), or null
+  int seed_gx, seed_gy;
   float min_depth, max_depth, inv_min, inv_max;
   float eps2;                   // (NCC denominator floor)^2
   float photo_reject;           // 3 x photo_threshold

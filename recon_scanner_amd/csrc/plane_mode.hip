@@ -43,7 +43,12 @@ __device__ __forceinline__ void InitTile(const PlaneArgs& a, RefTile<HALF, FMT, 
     PixelRay(a, x, y, ray);
     const int idx = y * a.w + x;
     const uint32_t key = RngKey(a.seed, 0u, static_cast<uint32_t>(idx));
-    const Hyp h = RandomPlane(a, key, 0, ray);
+    Hyp h = RandomPlane(a, key, 0, ray);
+    if (a.seed_cells) {
+      // a seeded cell starts on the fronto-parallel plane at its mean depth
+      const float sd = a.seed_cells[min(y >> 5, a.seed_gy - 1) * a.seed_gx + min(x >> 5, a.seed_gx - 1)];
+      if (sd > 0.0f) h = Hyp{sd, 0.0f, 0.0f, -1.0f};
+    }
     int nv = 0;
     const float c = PlaneCost<HALF, STRIDE, FMT>(a, x, y, win, h, &nv, nullptr);
     a.plane[idx] = make_float4(h.d, h.nx, h.ny, h.nz);
@@ -493,6 +498,7 @@ void FreeState(PlaneState* s) {
   if (s->cost) (void)hipFree(s->cost);
   if (s->support) (void)hipFree(s->support);
   if (s->geo_support) (void)hipFree(s->geo_support);
+  if (s->seed_cells) (void)hipFree(s->seed_cells);
   delete s;
 }
 
@@ -990,6 +996,93 @@ void DestroyPlaneState(recon_mvs_hip_context* ctx) {
   ctx->views.clear();
 }
 
+namespace {
+
+// Sparse-point seeding (patchmatch_stereo.cpp:499-533): seeds (u, v, depth)
+// outside the image or with depth <= 0 are dropped, the rest fall into
+// 32 x 32 pixel cells (column min(gx - 1, int(u) / 32)), and a cell's depth
+// is the mean of its seeds, summed in seed order in FP64 as the reference
+// does; stored as float, 0 for a cell without seeds.
+void SeedCells(int w, int h, const double* seeds, int n, std::vector<float>* cells, int* gx, int* gy) {
+  constexpr int kGrid = 32;
+  *gx = (w + kGrid - 1) / kGrid;
+  *gy = (h + kGrid - 1) / kGrid;
+  const size_t nc = static_cast<size_t>(*gx) * *gy;
+  std::vector<double> sum(nc, 0.0);
+  std::vector<int> count(nc, 0);
+  for (int i = 0; i < n; ++i) {
+    const double su = seeds[3 * i], sv = seeds[3 * i + 1], sd = seeds[3 * i + 2];
+    if (su < 0 || sv < 0 || su >= w || sv >= h) continue;
+    if (sd <= 0.0) continue;
+    const int cx = std::min(*gx - 1, static_cast<int>(su) / kGrid);
+    const int cy = std::min(*gy - 1, static_cast<int>(sv) / kGrid);
+    sum[static_cast<size_t>(cy) * *gx + cx] += sd;
+    ++count[static_cast<size_t>(cy) * *gx + cx];
+  }
+  cells->assign(nc, 0.0f);
+  for (size_t c = 0; c < nc; ++c)
+    if (count[c] > 0) (*cells)[c] = static_cast<float>(sum[c] / static_cast<double>(count[c]));
+}
+
+int32_t RunPlanePatchmatch(recon_mvs_hip_context* ctx, int32_t ref_view, int32_t num_src,
+                           const int32_t* src_views, const recon_mvs_plane_params* params,
+                           const double* seed_points, int32_t num_seeds) {
+  if (!ctx) return RECON_MVS_ERR_INVALID_ARGUMENT;
+  if (num_seeds < 0 || (num_seeds > 0 && !seed_points))
+    return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "bad seed points");
+  PlaneArgs a;
+  int32_t st = MakeArgs(ctx, ref_view, num_src, src_views, params, &a);
+  if (st != RECON_MVS_OK) return st;
+  st = EnsureState(ctx, ref_view);
+  if (st != RECON_MVS_OK) return st;
+  PlaneState& S = *ctx->states[ref_view];
+  S.has_geo = false;
+  a.plane = S.plane;
+  a.cost = S.cost;
+  a.support = S.support;
+  if (num_seeds > 0) {
+    // the staging vector of an earlier seeded call may still feed its copy
+    if (!S.seed_host.empty()) RMVS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    int gx = 0, gy = 0;
+    SeedCells(S.w, S.h, seed_points, num_seeds, &S.seed_host, &gx, &gy);
+    const int nc = gx * gy;
+    if (S.seed_capacity < nc) {
+      if (S.seed_cells) (void)hipFree(S.seed_cells);
+      S.seed_cells = nullptr;
+      S.seed_capacity = 0;
+      if (hipMalloc(&S.seed_cells, nc * sizeof(float)) != hipSuccess)
+        return Fail(ctx, RECON_MVS_ERR_OUT_OF_MEMORY, "seed cell allocation failed");
+      S.seed_capacity = nc;
+    }
+    RMVS_HIP(ctx, hipMemcpyAsync(S.seed_cells, S.seed_host.data(), nc * sizeof(float),
+                                 hipMemcpyHostToDevice, ctx->stream));
+    a.seed_cells = S.seed_cells;
+    a.seed_gx = gx;
+    a.seed_gy = gy;
+  }
+  int sched = 0;
+  st = ResolveSchedule(ctx, a, params, &sched);
+  if (st != RECON_MVS_OK) return st;
+  std::vector<int> caps;
+  st = ResolveCaps(ctx, a, params, sched, &caps);
+  if (st != RECON_MVS_OK) return st;
+  st = Launch(ctx, params, kInit, a, 0.0f, 0, 0, nullptr, nullptr, nullptr, nullptr, nullptr, sched,
+              caps[0]);
+  if (st != RECON_MVS_OK) return st;
+  for (int it = 0; it < params->iterations; ++it) {
+    for (int color = 0; color < 2; ++color) {
+      a.iter = it;
+      a.color = color;
+      st = Launch(ctx, params, kSweep, a, 0.0f, 0, 0, nullptr, nullptr, nullptr, nullptr, nullptr, sched,
+                  caps[1 + 2 * it + color]);
+      if (st != RECON_MVS_OK) return st;
+    }
+  }
+  return RECON_MVS_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 RECON_MVS_API void recon_mvs_plane_params_default(recon_mvs_plane_params* p) {
@@ -1095,36 +1188,14 @@ RECON_MVS_API int32_t recon_mvs_hip_bgra_to_gray(recon_mvs_hip_context* ctx, con
 RECON_MVS_API int32_t recon_mvs_hip_plane_patchmatch(recon_mvs_hip_context* ctx, int32_t ref_view,
                                                      int32_t num_src, const int32_t* src_views,
                                                      const recon_mvs_plane_params* params) {
-  if (!ctx) return RECON_MVS_ERR_INVALID_ARGUMENT;
-  PlaneArgs a;
-  int32_t st = MakeArgs(ctx, ref_view, num_src, src_views, params, &a);
-  if (st != RECON_MVS_OK) return st;
-  st = EnsureState(ctx, ref_view);
-  if (st != RECON_MVS_OK) return st;
-  PlaneState& S = *ctx->states[ref_view];
-  S.has_geo = false;
-  a.plane = S.plane;
-  a.cost = S.cost;
-  a.support = S.support;
-  int sched = 0;
-  st = ResolveSchedule(ctx, a, params, &sched);
-  if (st != RECON_MVS_OK) return st;
-  std::vector<int> caps;
-  st = ResolveCaps(ctx, a, params, sched, &caps);
-  if (st != RECON_MVS_OK) return st;
-  st = Launch(ctx, params, kInit, a, 0.0f, 0, 0, nullptr, nullptr, nullptr, nullptr, nullptr, sched,
-              caps[0]);
-  if (st != RECON_MVS_OK) return st;
-  for (int it = 0; it < params->iterations; ++it) {
-    for (int color = 0; color < 2; ++color) {
-      a.iter = it;
-      a.color = color;
-      st = Launch(ctx, params, kSweep, a, 0.0f, 0, 0, nullptr, nullptr, nullptr, nullptr, nullptr, sched,
-                  caps[1 + 2 * it + color]);
-      if (st != RECON_MVS_OK) return st;
-    }
-  }
-  return RECON_MVS_OK;
+  return RunPlanePatchmatch(ctx, ref_view, num_src, src_views, params, nullptr, 0);
+}
+
+RECON_MVS_API int32_t recon_mvs_hip_plane_patchmatch_seeded(recon_mvs_hip_context* ctx, int32_t ref_view,
+                                                            int32_t num_src, const int32_t* src_views,
+                                                            const recon_mvs_plane_params* params,
+                                                            const double* seed_points, int32_t num_seeds) {
+  return RunPlanePatchmatch(ctx, ref_view, num_src, src_views, params, seed_points, num_seeds);
 }
 
 RECON_MVS_API int32_t recon_mvs_hip_plane_refine(recon_mvs_hip_context* ctx, int32_t ref_view,

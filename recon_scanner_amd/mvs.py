@@ -149,6 +149,12 @@ def lib():
                                                   P(N.Config), C.c_int32, C.c_void_p, C.c_void_p,
                                                   C.c_void_p, C.c_void_p, P(C.c_int32),
                                                   P(C.c_int64)]
+        L.recon_mvs_hip_fetch_oriented_cloud.argtypes = [ctxp, C.c_int64, C.c_void_p, C.c_void_p,
+                                                         C.c_void_p, C.c_void_p, P(C.c_int64)]
+        L.recon_mvs_spill_depth_map.argtypes = [C.c_char_p, C.c_int32, C.c_int32, C.c_int32,
+                                                C.c_void_p, C.c_void_p, C.c_void_p, C.c_char_p]
+        L.recon_mvs_load_depth_map.argtypes = [C.c_char_p, C.c_int32, C.c_int64, P(C.c_int32),
+                                               P(C.c_int32), C.c_void_p, C.c_void_p, C.c_void_p]
         _lib = L
     return _lib
 
@@ -345,6 +351,76 @@ def RunMvsStage(gray, cams, config, mode=STAGE_REFERENCE, plane_params=None):
                                                     cloud.voxel_keys.ctypes.data), "fetch")
     maps = [None if (d is None or not np.isfinite(d.depth).any()) else d for d in maps]
     return int(produced.value), maps, cloud
+
+
+class OrientedPointCloud:
+    """Poisson input (OrientedPointCloud, poisson_mesh.h:25-38)."""
+
+    def __init__(self, n=0):
+        self.positions = np.zeros((n, 3))
+        self.normals = np.zeros((n, 3))
+        self.colors = np.zeros(3 * n, np.float32)
+        self.weights = np.zeros(n, np.float32)
+
+    def size(self):
+        return int(self.positions.shape[0])
+
+
+def FetchOrientedCloud() -> OrientedPointCloud:
+    """The cloud of the last FuseDepthMaps / RunMvsStage on the context in
+    Poisson input form (UnifiedPointCloud::ToOrientedCloud, orchestrator.cpp:
+    181-192): weights = max(0, confidence)."""
+    n = C.c_int64(0)
+    check(lib().recon_mvs_hip_fetch_oriented_cloud(context(), 0, None, None, None, None, C.byref(n)),
+          "FetchOrientedCloud")
+    out = OrientedPointCloud(int(n.value))
+    if out.size():
+        check(lib().recon_mvs_hip_fetch_oriented_cloud(context(), out.size(), out.positions.ctypes.data,
+                                                       out.normals.ctypes.data, out.colors.ctypes.data,
+                                                       out.weights.ctypes.data, None),
+              "FetchOrientedCloud")
+    return out
+
+
+def SpillDepthMap(cache_dir, ref_index, dm: DepthMap) -> str:
+    """Writes dm as <cache_dir>/mvs_depth/ref_%05d.f32 and records it in
+    <cache_dir>/cache_manifest.json (docs/13_内存与降级.md §3.3.2-3.3.4);
+    returns the file's sha256."""
+    d = np.ascontiguousarray(dm.depth, np.float32).reshape(-1)
+    s_ = np.ascontiguousarray(dm.support_count, np.uint8).reshape(-1)
+    c = np.ascontiguousarray(dm.confidence, np.float32).reshape(-1)
+    n = dm.width * dm.height
+    if n <= 0 or d.size != n or s_.size != n or c.size != n:
+        raise ValueError("SpillDepthMap: map buffers do not match width x height")
+    hexd = C.create_string_buffer(65)
+    st = lib().recon_mvs_spill_depth_map(os.fsencode(cache_dir), int(ref_index), dm.width, dm.height,
+                                         d.ctypes.data, s_.ctypes.data, c.ctypes.data, hexd)
+    if st != N.OK:
+        raise RuntimeError(f"SpillDepthMap failed with status {st}")
+    return hexd.value.decode()
+
+
+def LoadDepthMap(cache_dir, ref_index):
+    """Reads a spilled map back; None when it is missing or fails verification
+    (the failed item is deleted from the cache: recompute it)."""
+    w, h = C.c_int32(0), C.c_int32(0)
+    path = os.fsencode(cache_dir)
+    st = lib().recon_mvs_load_depth_map(path, int(ref_index), 0, C.byref(w), C.byref(h), None, None,
+                                        None)
+    if st == N.ERR_NO_DEPTH:
+        return None
+    if st != N.OK:
+        raise RuntimeError(f"LoadDepthMap failed with status {st}")
+    dm = DepthMap()
+    dm.Allocate(w.value, h.value)
+    st = lib().recon_mvs_load_depth_map(path, int(ref_index), w.value * h.value, C.byref(w),
+                                        C.byref(h), dm.depth.ctypes.data,
+                                        dm.support_count.ctypes.data, dm.confidence.ctypes.data)
+    if st == N.ERR_NO_DEPTH:
+        return None
+    if st != N.OK:
+        raise RuntimeError(f"LoadDepthMap failed with status {st}")
+    return dm
 
 
 def EvaluateCandidates(ref_gray, src_gray, ref_cam: N.Camera, src_cams, config, u_v_inv):

@@ -64,6 +64,8 @@ def _lib():
         L.recon_mvs_plane_params_default.restype = None
         L.recon_mvs_hip_set_views.argtypes = [v, C.c_int32, P(View), C.c_int32]
         L.recon_mvs_hip_plane_patchmatch.argtypes = [v, C.c_int32, C.c_int32, v, P(PlaneParams)]
+        L.recon_mvs_hip_plane_patchmatch_seeded.argtypes = [v, C.c_int32, C.c_int32, v,
+                                                            P(PlaneParams), v, C.c_int32]
         L.recon_mvs_hip_plane_refine.argtypes = [v, C.c_int32, C.c_int32, v, P(PlaneParams)]
         L.recon_mvs_hip_geo_consistency.argtypes = [v, C.c_int32, v, C.c_double, C.c_int32]
         L.recon_mvs_hip_get_depth_map.argtypes = [v, C.c_int32, C.c_int32, v, v, v, v, v, v]
@@ -134,10 +136,19 @@ class PlaneMVS:
     def _check(self, st, what):
         mvs.check(st, what)
 
-    def patchmatch(self, ref, srcs, params):
+    def patchmatch(self, ref, srcs, params, seed_points=None):
+        """Plane-mode PatchMatch of view ref; seed_points: optional (n, 3)
+        rows (u, v, depth_m), the sparse-point seeding of PatchMatchStereo
+        (patchmatch_stereo.cpp:499-533)."""
         s = _i32(srcs)
-        self._check(self.L.recon_mvs_hip_plane_patchmatch(self.ctx, ref, len(s), s.ctypes.data,
-                                                          C.byref(params)), "plane_patchmatch")
+        if seed_points is None:
+            self._check(self.L.recon_mvs_hip_plane_patchmatch(self.ctx, ref, len(s), s.ctypes.data,
+                                                              C.byref(params)), "plane_patchmatch")
+            return
+        sp = np.ascontiguousarray(seed_points, np.float64).reshape(-1, 3)
+        self._check(self.L.recon_mvs_hip_plane_patchmatch_seeded(
+            self.ctx, ref, len(s), s.ctypes.data, C.byref(params), sp.ctypes.data, sp.shape[0]),
+            "plane_patchmatch_seeded")
 
     def refine(self, ref, srcs, params):
         s = _i32(srcs)

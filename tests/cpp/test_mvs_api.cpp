@@ -4,13 +4,19 @@
 // It runs the reference's unit-test scene through PatchMatchStereo ->
 // RefineSubpixel -> FuseDepthMaps on the GPU and applies the thresholds of
 // tests/unit/test_mvs.cpp:344-465.  Exit code 0 = all checks passed.
+#include <unistd.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
 #include <vector>
 
 #include "recon/camera.h"
 #include "recon/mvs.h"
+#include "recon/mvs_cache.h"
 #include "recon_mvs_hip.h"
 
 extern "C" int32_t recon_scene_reference_plane(int32_t num_views, double baseline_m,
@@ -129,6 +135,44 @@ int main() {
   const recon::PointCloud cloud = recon::FuseDepthMaps({&depth}, {poses[0]}, {intr}, cfg);
   std::printf("  fused %zu points\n", cloud.size());
   Expect(cloud.size() > 1000, "FuseDepthMaps produces > 1000 points");
+  // the cloud in Poisson input form: weights are the confidences clamped at 0
+  {
+    const recon::OrientedPointCloud oc = recon::ToOrientedCloud(cloud);
+    bool ok = oc.size() == cloud.size() && oc.weights.size() == cloud.size() &&
+              oc.colors.size() == cloud.colors.size();
+    for (size_t i = 0; ok && i < cloud.size(); ++i)
+      ok = oc.weights[i] == std::max(0.0f, cloud.confidences[i]) &&
+           oc.positions[i](0) == cloud.positions[i](0) && oc.normals[i](2) == cloud.normals[i](2);
+    Expect(ok, "ToOrientedCloud: weights = max(0, confidence)");
+  }
+  // depth-map spill: written, read back bit-exact, a corrupted file rejected and deleted
+  {
+    char tmpl[] = "/tmp/recon_spill_XXXXXX";
+    const char* dir = mkdtemp(tmpl);
+    const std::string sha = dir ? recon::SpillDepthMap(dir, 7, depth) : std::string();
+    Expect(sha.size() == 64, "SpillDepthMap returns a sha256");
+    recon::DepthMap back;
+    const bool loaded = dir && recon::LoadDepthMap(dir, 7, &back);
+    Expect(loaded && back.width == W && back.height == H &&
+               std::memcmp(back.depth.data(), depth.depth.data(), 4 * depth.depth.size()) == 0 &&
+               back.support_count == depth.support_count && back.confidence == depth.confidence,
+           "LoadDepthMap round trip is bit-exact");
+    if (dir) {
+      const std::string f = std::string(dir) + "/mvs_depth/ref_00007.f32";
+      FILE* fp = std::fopen(f.c_str(), "r+b");
+      if (fp) {
+        std::fseek(fp, 100, SEEK_SET);
+        std::fputc(0x5a, fp);
+        std::fclose(fp);
+      }
+      recon::DepthMap bad;
+      Expect(!recon::LoadDepthMap(dir, 7, &bad) && std::fopen(f.c_str(), "rb") == nullptr,
+             "corrupted spill rejected and deleted");
+      std::remove((std::string(dir) + "/cache_manifest.json").c_str());
+      rmdir((std::string(dir) + "/mvs_depth").c_str());
+      rmdir(dir);
+    }
+  }
   // index contract: ambiguous intrinsics length returns 0
   {
     recon::DepthMap d2;

@@ -207,3 +207,59 @@ def test_bgra_to_gray_restatement_closed_forms():
     a = rng.integers(0, 256, (37, 53, 4), dtype=np.uint8)
     ref = [[(77 * int(p[2]) + 150 * int(p[1]) + 29 * int(p[0])) // 256 for p in row] for row in a]
     assert np.array_equal(oracle.bgra_to_gray(a), np.array(ref, np.uint8))
+
+
+def _seed_points(w, h, rng):
+    """Seeds over a w x h image: a few per cell in most cells, one cell left
+    empty, points on cell borders and the image edge, and points the
+    reference drops (outside the image, depth <= 0)."""
+    pts = []
+    for cy in range(0, h, 32):
+        for cx in range(0, w, 32):
+            if (cx, cy) == (32, 32):
+                continue  # an empty cell: random init there
+            for _ in range(int(rng.integers(1, 4))):
+                pts.append((cx + rng.uniform(0, min(32, w - cx)), cy + rng.uniform(0, min(32, h - cy)),
+                            rng.uniform(0.28, 0.42)))
+    pts += [(31.999, 5.0, 0.30), (32.0, 5.0, 0.41), (w - 1e-9, h - 1e-9, 0.33), (-0.5, 3.0, 0.3),
+            (w, 3.0, 0.3), (4.0, h + 1.0, 0.3), (10.0, 10.0, 0.0), (12.0, 12.0, -1.0)]
+    return np.array(pts, np.float64)
+
+
+@pytest.mark.skipif(not oracle.reference_available(), reason="reference not compiled here")
+def test_seed_cells_match_reference_initialisation():
+    """The oracle's restatement of the sparse-point seeding (patchmatch_stereo.cpp:
+    499-533) against the reference itself: with 0 iterations the reference
+    outputs its initial depth 1 / (1 / cell mean) wherever the initial cost is
+    finite, so every seeded pixel's output pins the cell assignment, the drop
+    rules and the mean."""
+    w, h = 100, 70
+    sc = scenes.sphere_arc(width=w, height=h, focal=150.0, n_src=4, seed=5)
+    u8 = [sc.u8(i) for i in range(5)]
+    cfg = N.config_default()
+    cfg.patch_size, cfg.iterations, cfg.num_threads = 5, 0, 1
+    cfg.min_depth_m, cfg.max_depth_m = 0.25, 0.45
+    seeds = _seed_points(w, h, np.random.default_rng(11))
+    _, depth, _, _ = oracle.ref_patchmatch(u8[0], u8[1:], sc.cams[0], sc.cams, sc.cams, [1, 2, 3, 4],
+                                           cfg, seeds=seeds)
+    cells = oracle.seed_cells(w, h, seeds)
+    assert cells.shape == (3, 4) and np.isnan(cells[1, 1]) and np.isfinite(cells).sum() == 11
+    ys, xs = np.mgrid[0:h, 0:w]
+    cd = cells[np.minimum(ys // 32, 2), np.minimum(xs // 32, 3)]
+    seeded = np.isfinite(cd) & np.isfinite(depth)
+    want = (1.0 / (1.0 / cd[seeded])).astype(np.float32)
+    assert np.array_equal(depth[seeded], want)
+    # every seeded cell is covered by the comparison
+    for cy in range(3):
+        for cx in range(4):
+            if np.isfinite(cells[cy, cx]):
+                block = seeded[cy * 32:(cy + 1) * 32, cx * 32:(cx + 1) * 32]
+                assert block.sum() > block.size // 2
+    # the empty cell holds the random init, not a constant
+    empty = depth[32:64, 32:64]
+    assert np.unique(empty[np.isfinite(empty)]).size > 100
+    # the plane-mode init rows: fronto-parallel at the float32 cell depth, NaN elsewhere
+    planes = oracle.seeded_init_planes(w, h, seeds)
+    assert np.array_equal(planes[..., 0][np.isfinite(cd)], cd[np.isfinite(cd)].astype(np.float32))
+    assert np.all(planes[np.isfinite(cd)][:, 1:] == [0.0, 0.0, -1.0])
+    assert np.all(np.isnan(planes[~np.isfinite(cd)]))

@@ -245,3 +245,60 @@ def test_bgra_frames_gray_conversion_bit_exact():
     for k in ("depth", "normals", "cost"):
         assert np.array_equal(np.nan_to_num(da[k], nan=-1), np.nan_to_num(db[k], nan=-1)), k
     assert np.isfinite(da["depth"]).mean() > 0.9
+
+
+def test_seeded_init_bit_exact_and_converges_faster():
+    """Sparse-point seeding (patchmatch_stereo.cpp:499-533): init planes
+    bit-exact with the oracle's (seeded cells fronto-parallel at the FP64 cell
+    mean, the rest the random init), sweeps from there within 0.1 % of the
+    oracle's on >= 99 % of the pixels, and seeds from 2 %-noisy sparse depths
+    reach after one iteration an accuracy the random init does not."""
+    say("plane patchmatch seeded sphere_arc 320x240 7x7")
+    sc = scenes.sphere_arc()
+    srcs = [1, 2, 3, 4]
+    h, w = sc.depths[0].shape
+    rng = np.random.default_rng(4)
+    n = 300
+    us, vs = rng.uniform(0, w, n), rng.uniform(0, h, n)
+    t = sc.depths[0][vs.astype(int), us.astype(int)]
+    seeds = np.stack([us, vs, t * rng.uniform(0.98, 1.02, n)], 1)
+    seeds = seeds[t > 0]
+    p0 = plane.default_params(patch_size=7, window_stride=1, top_k=0, iterations=0,
+                              min_depth_m=0.25, max_depth_m=0.45, value_scale=1.0)
+    pm = plane.PlaneMVS(sc.cams, sc.images)
+    pm.patchmatch(0, srcs, p0, seed_points=seeds)
+    g0 = pm.depth_map(0)
+    init = oracle.seeded_init_planes(w, h, seeds)
+    op, oc, _ = oracle.plane_patchmatch(sc.images, sc.cams, 0, srcs, oparams(p0), init_planes=init)
+    fin = np.isfinite(g0["depth"])
+    assert np.array_equal(g0["normals"], op[..., 1:])
+    assert np.array_equal(g0["depth"][fin], op[..., 0][fin])
+    seeded = np.isfinite(init[..., 0])
+    assert seeded.mean() > 0.9 and np.all(g0["normals"][seeded] == [0.0, 0.0, -1.0])
+    p = p0.copy(iterations=2)
+    pm.patchmatch(0, srcs, p, seed_points=seeds)
+    g = pm.depth_map(0)
+    op, oc, _ = oracle.plane_patchmatch(sc.images, sc.cams, 0, srcs, oparams(p), init_planes=init)
+    od = np.where(oc < 1e29, op[..., 0], np.nan)
+    both = np.isfinite(g["depth"]) & np.isfinite(od)
+    rel = np.abs(g["depth"][both] - od[both]) / od[both]
+    say("seeded 2 it: within 0.1%:", np.mean(rel <= 1e-3), "valid", both.sum())
+    assert np.mean(rel <= 1e-3) >= 0.99
+    # after one iteration, fraction of pixels within 0.5 mm of the truth
+    truth = sc.depths[0]
+
+    def good(d):
+        m = truth > 0
+        return float(np.mean(np.abs(np.nan_to_num(d[m], nan=0.0) - truth[m]) < 5e-4))
+
+    p1 = p0.copy(iterations=1)
+    pm.patchmatch(0, srcs, p1, seed_points=seeds)
+    gs = good(pm.depth_map(0)["depth"])
+    pm.patchmatch(0, srcs, p1)
+    dr = pm.depth_map(0)["depth"]
+    gr = good(dr)
+    say("within 0.5 mm after 1 iteration: seeded", gs, "random", gr)
+    assert gs > gr
+    # an empty seed list is the unseeded call
+    pm.patchmatch(0, srcs, p1, seed_points=np.zeros((0, 3)))
+    assert np.array_equal(pm.depth_map(0)["depth"], dr, equal_nan=True)

@@ -119,6 +119,13 @@ def test_refine_and_fuse_golden():
     for x, y in zip(a, b):
         assert bits_equal(x, y)
     assert np.all(np.diff(cloud.voxel_keys) > 0)
+    # the same cloud in Poisson input form (UnifiedPointCloud::ToOrientedCloud)
+    oc = mvs.FetchOrientedCloud()
+    pos, nrm, col, w = oracle.to_oriented_cloud(cloud.positions, cloud.normals, cloud.colors,
+                                                cloud.confidences)
+    assert oc.size() == cloud.size()
+    assert bits_equal(oc.positions, pos) and bits_equal(oc.normals, nrm)
+    assert bits_equal(oc.colors, col) and bits_equal(oc.weights, w)
 
 
 def test_index_contract_and_early_returns():
