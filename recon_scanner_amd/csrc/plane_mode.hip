@@ -179,7 +179,12 @@ __device__ __forceinline__ void SweepTile(const PlaneArgs& a, RefTile<HALF, FMT,
 }
 
 template <int HALF, int STRIDE, int FMT, int TW, int TH>
-__global__ __launch_bounds__(256) void k_plane_sweep(PlaneArgs a) {
+// Stride-2 windows: 3 waves per SIMD (at most 168 VGPRs; without the limit
+// the compiler takes ~172): converged half-sweeps are VALU bound and gain from
+// the third wave; the unconverged ones are held at lower occupancy by the
+// per-phase caps.  Stride-1 windows would spill under the limit.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STRIDE == 2 ? 3 : 1)))
+void k_plane_sweep(PlaneArgs a) {
   __shared__ RefTile<HALF, FMT, TW, TH> tile;
   int tx, ty;
   if (!TileOf(a, &tx, &ty)) return;
@@ -191,7 +196,8 @@ __global__ __launch_bounds__(256) void k_plane_sweep(PlaneArgs a) {
 // otherwise the half-sweep of colour phase & 1 of iteration phase >> 1.  Every
 // workgroup takes units until the queue is exhausted, then exits.
 template <int HALF, int STRIDE, int FMT, int TW, int TH>
-__global__ __launch_bounds__(256) void k_plane_batch(const PlaneArgs* __restrict__ args,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STRIDE == 2 ? 3 : 1)))
+void k_plane_batch(const PlaneArgs* __restrict__ args,
                                                      const int2* __restrict__ units, int num_units,
                                                      unsigned int* __restrict__ counter, int phase) {
   __shared__ RefTile<HALF, FMT, TW, TH> tile;
