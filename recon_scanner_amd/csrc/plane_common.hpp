@@ -280,71 +280,87 @@ __device__ float PlaneCost(const PlaneArgs& a, int x, int y, const RefWindow<HAL
           // e2 < 0.5 keeps every sample's denominator positive
           const bool den_ok = e2 < 0.5f;
           const bool outside = den_ok && (fu + bu < lox || fu - bu > hix || fv + bv < loy || fv - bv > hiy);
-          // Rows of the window map to segments (projective maps keep them
-          // convex while the denominator stays positive): a row whose two end
-          // samples are inside the image with a 1e-4 px margin is inside.
-          const bool rows_checkable = den_ok && win.full;
-          const float mlox = lox + 1e-4f, mhix = hix - 1e-4f, mloy = loy + 1e-4f, mhiy = hiy - 1e-4f;
           // Sums over the source samples; the reference sums start from the
           // whole window and lose the reference values of excluded samples.
           float ss = 0.0f, sss = 0.0f, srs = 0.0f, xr = 0.0f, xrr = 0.0f, xcnt = 0.0f;
+          // A window whose projection lies inside the source image with a
+          // 1e-4 px margin by the bound above (and whose reference window is
+          // whole) has every sample valid: such windows skip the per-sample
+          // masks.  The test is per window, so the branch below is taken once
+          // per (hypothesis, view), not per row.
+          const float mlox = lox + 1e-4f, mhix = hix - 1e-4f, mloy = loy + 1e-4f, mhiy = hiy - 1e-4f;
+          const bool win_in = den_ok & win.full & (fu - bu >= mlox) & (fu + bu <= mhix) &
+                              (fv - bv >= mloy) & (fv + bv <= mhiy);
           if (!outside) {
-            // Each window row in three phases (positions, all loads,
-            // arithmetic): a wave keeps NS loads in flight.  Loads need
-            // no clamping: out-of-range buffer reads return 0 and the
-            // sample is masked out.
+            if (__ballot(!win_in) == 0) {
+              // every lane's window is inside: no per-sample masks
 #pragma unroll 1
-            for (int j = 0; j < NS; ++j) {
-              const float dy = static_cast<float>(-HALF + j * STRIDE);
-              const float b0 = g01 * dy, b1 = g11 * dy, b2 = fmaf(h21, dy, 1.0f);
-              float txs[NS], tys[NS], dens[NS];
-#pragma unroll
-              for (int i = 0; i < NS; ++i) {
-                const float dx = static_cast<float>(-HALF + i * STRIDE);
-                dens[i] = fmaf(h20, dx, b2);
-                const float r = __builtin_amdgcn_rcpf(dens[i]);
-                txs[i] = fmaf(fmaf(g00, dx, b0), r, fu);
-                tys[i] = fmaf(fmaf(g10, dx, b1), r, fv);
-              }
-              // (bitwise & on purpose: no short-circuit branches)
-              const bool row_ok = rows_checkable & (txs[0] >= mlox) & (txs[0] <= mhix) &
-                                  (tys[0] >= mloy) & (tys[0] <= mhiy) & (txs[NS - 1] >= mlox) &
-                                  (txs[NS - 1] <= mhix) & (tys[NS - 1] >= mloy) & (tys[NS - 1] <= mhiy);
-              int off[NS];
-              float fxs[NS], fys[NS];
-#pragma unroll
-              for (int i = 0; i < NS; ++i) {
-                const float flx = floorf(txs[i]), fly = floorf(tys[i]);
-                fxs[i] = txs[i] - flx;
-                fys[i] = tys[i] - fly;
-                off[i] = static_cast<int>(fmaf(fly, fpitch, flx + obase)) * kEB;
-              }
-              float4 q[NS];
-#pragma unroll
-              for (int i = 0; i < NS; ++i) q[i] = BufQuad<FMT>(rsrc, off[i]);
-              float sv[NS];
-#pragma unroll
-              for (int i = 0; i < NS; ++i) {
-                const float top = fmaf(fxs[i], q[i].z - q[i].x, q[i].x);
-                const float bot = fmaf(fxs[i], q[i].w - q[i].y, q[i].y);
-                sv[i] = fmaf(fys[i], bot - top, top) - shift;
-              }
-              if (__ballot(!row_ok) == 0) {
-                // every lane's row is inside both images
+              for (int j = 0; j < NS; ++j) {
+                const float dy = static_cast<float>(-HALF + j * STRIDE);
+                const float b0 = g01 * dy, b1 = g11 * dy, b2 = fmaf(h21, dy, 1.0f);
+                int off[NS];
+                float fxs[NS], fys[NS];
 #pragma unroll
                 for (int i = 0; i < NS; ++i) {
-                  ss += sv[i];
-                  sss = fmaf(sv[i], sv[i], sss);
-                  srs = fmaf(win.at(i, j), sv[i], srs);
+                  const float dx = static_cast<float>(-HALF + i * STRIDE);
+                  const float r = __builtin_amdgcn_rcpf(fmaf(h20, dx, b2));
+                  const float tx = fmaf(fmaf(g00, dx, b0), r, fu);
+                  const float ty = fmaf(fmaf(g10, dx, b1), r, fv);
+                  const float flx = floorf(tx), fly = floorf(ty);
+                  fxs[i] = tx - flx;
+                  fys[i] = ty - fly;
+                  off[i] = static_cast<int>(fmaf(fly, fpitch, flx + obase)) * kEB;
                 }
-              } else {
+                float4 q[NS];
+#pragma unroll
+                for (int i = 0; i < NS; ++i) q[i] = BufQuad<FMT>(rsrc, off[i]);
 #pragma unroll
                 for (int i = 0; i < NS; ++i) {
+                  const float top = fmaf(fxs[i], q[i].z - q[i].x, q[i].x);
+                  const float bot = fmaf(fxs[i], q[i].w - q[i].y, q[i].y);
+                  const float s = fmaf(fys[i], bot - top, top) - shift;
+                  ss += s;
+                  sss = fmaf(s, s, sss);
+                  srs = fmaf(win.at(i, j), s, srs);
+                }
+              }
+            } else {
+              // some window crosses an edge: each sample is checked
+#pragma unroll 1
+              for (int j = 0; j < NS; ++j) {
+                const float dy = static_cast<float>(-HALF + j * STRIDE);
+                const float b0 = g01 * dy, b1 = g11 * dy, b2 = fmaf(h21, dy, 1.0f);
+                float txs[NS], tys[NS], dens[NS];
+#pragma unroll
+                for (int i = 0; i < NS; ++i) {
+                  const float dx = static_cast<float>(-HALF + i * STRIDE);
+                  dens[i] = fmaf(h20, dx, b2);
+                  const float r = __builtin_amdgcn_rcpf(dens[i]);
+                  txs[i] = fmaf(fmaf(g00, dx, b0), r, fu);
+                  tys[i] = fmaf(fmaf(g10, dx, b1), r, fv);
+                }
+                int off[NS];
+                float fxs[NS], fys[NS];
+#pragma unroll
+                for (int i = 0; i < NS; ++i) {
+                  const float flx = floorf(txs[i]), fly = floorf(tys[i]);
+                  fxs[i] = txs[i] - flx;
+                  fys[i] = tys[i] - fly;
+                  off[i] = static_cast<int>(fmaf(fly, fpitch, flx + obase)) * kEB;
+                }
+                float4 q[NS];
+#pragma unroll
+                for (int i = 0; i < NS; ++i) q[i] = BufQuad<FMT>(rsrc, off[i]);
+#pragma unroll
+                for (int i = 0; i < NS; ++i) {
+                  const float top = fmaf(fxs[i], q[i].z - q[i].x, q[i].x);
+                  const float bot = fmaf(fxs[i], q[i].w - q[i].y, q[i].y);
+                  const float sv = fmaf(fys[i], bot - top, top) - shift;
                   const float rv = win.at(i, j);
                   const bool rin = rv == rv;  // NaN: outside the reference image
-                  const bool ok = row_ok | (rin & (dens[i] > 0.0f) & (txs[i] >= lox) &
+                  const bool ok = win_in | (rin & (dens[i] > 0.0f) & (txs[i] >= lox) &
                                             (txs[i] <= hix) & (tys[i] >= loy) & (tys[i] <= hiy));
-                  const float s = ok ? sv[i] : 0.0f;
+                  const float s = ok ? sv : 0.0f;
                   const float x = (ok || !rin) ? 0.0f : rv;
                   ss += s;
                   sss = fmaf(s, s, sss);
@@ -477,14 +493,20 @@ using namespace rmvs;
 // ---------------------------------------------------------------------------
 
 // Tile index -> (tile x, tile y), XCD-aware: workgroups are dealt round-robin
-// to the 8 XCDs, so consecutive spatial tiles are placed on one XCD (shared
-// L2 for their overlapping source footprints).
+// to the 8 XCDs, so consecutive tile indices are placed on one XCD (shared
+// L2 for their overlapping source footprints), and tile indices run down the
+// columns: each XCD holds a vertical band of the image, and the tiles in
+// flight on it at one time form a roughly square block (about 240 x 200 px at
+// 1920 x 1440).  A hypothesis at an unknown depth samples a source along its
+// epipolar line; a square block keeps the union of those segments compact
+// for every baseline direction, where a band of rows does so only for
+// sources displaced along the rows.
 __device__ __forceinline__ bool TileOf(const PlaneArgs& a, int* tx, int* ty) {
   const int b = blockIdx.x;
   const int t = (b & 7) * a.tiles_per_xcd + (b >> 3);
   if (t >= a.num_tiles) return false;
-  *tx = t % a.tiles_x;
-  *ty = t / a.tiles_x;
+  *tx = t / a.tiles_y;
+  *ty = t % a.tiles_y;
   return true;
 }
 

@@ -58,7 +58,8 @@ __device__ __forceinline__ void InitTile(const PlaneArgs& a, RefTile<HALF, FMT, 
 }
 
 template <int HALF, int STRIDE, int FMT, int TW, int TH>
-__global__ __launch_bounds__(256) void k_plane_init(PlaneArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STRIDE == 2 ? 3 : 1)))
+void k_plane_init(PlaneArgs a) {
   __shared__ RefTile<HALF, FMT, TW, TH> tile;
   int tx, ty;
   if (!TileOf(a, &tx, &ty)) return;
