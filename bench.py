@@ -49,6 +49,9 @@ def parse():
     ap.add_argument("--iterations", type=int, default=5)
     ap.add_argument("--storage", default=None, choices=["f32", "f16", "bf16"],
                     help="image storage (default: f32; f16 for --workload batch)")
+    ap.add_argument("--view-phased", type=int, default=-1,
+                    help="leading iterations run view-phased (one launch per source view); "
+                         "-1: the library's automatic choice")
     ap.add_argument("--tile-schedule", type=int, default=-1,
                     help="workgroup tile schedule; -1 = autotuned with the per-phase occupancy "
                          "(during the warm-up step)")
@@ -271,7 +274,8 @@ def main():
     views = list(range(nv))
     p = plane.default_params(patch_size=11, window_stride=2, top_k=4, iterations=args.iterations,
                              min_depth_m=sc.min_depth, max_depth_m=sc.max_depth, value_scale=1.0,
-                             tile_schedule=args.tile_schedule)
+                             tile_schedule=args.tile_schedule,
+                             view_phased_iterations=args.view_phased)
     pm = plane.PlaneMVS(sc.cams, sc.images, storage=args.storage)
 
     def step():
@@ -350,7 +354,8 @@ def main():
                                f"{args.iterations} red-black iterations + geometric-consistency "
                                f"filter over 9 depth maps",
                    "views": nv, "width": args.width, "height": args.height,
-                   "iterations": args.iterations, "storage": args.storage},
+                   "iterations": args.iterations, "storage": args.storage,
+                   "view_phased_iterations": args.view_phased},
         "hypothesis_view_evals_per_s": hve * args.steps / dt,
         "median_depth_error_mm": med,
         "kept_pixels": kept,

@@ -324,7 +324,8 @@ typedef struct recon_mvs_plane_params {
   int32_t propagation_reach; /* odd steps scanned per direction, 1..16 (1 = 4-neighbour) */
   int32_t refine_iterations; /* golden-section steps of recon_mvs_hip_plane_refine */
   int32_t tile_schedule;     /* workgroup tile: 0 = 64x8, 1 = 32x16, 2 = 128x4 px,
-                                -1 = autotuned (cached per size, views, window, format) */
+                                -1 = autotuned (cached per size, views, window, format);
+                                results do not depend on it */
   double min_depth_m, max_depth_m;
   double depth_perturb;      /* relative inverse-depth radius at iteration 0 */
   double normal_perturb;     /* normal perturbation radius at iteration 0 */
@@ -339,6 +340,14 @@ typedef struct recon_mvs_plane_params {
      workgroups per CU in the capped phases.  Results do not depend on it. */
   int32_t occupancy_cap;
   int32_t capped_iterations; /* leading iterations run capped (default 1; < 0 = 1) */
+  /* Leading iterations whose half-sweeps run view-phased: one launch per
+     source view instead of one for all, each visiting the tiles in strips
+     along that view's epipolar lines, so the unconverged hypotheses' window
+     gathers touch one source image at a time and stay in the L2 (11x11
+     stride-2 window only; other windows ignore it).  -1 = automatic (the
+     default: the first iteration for FP32 images, none for FP16 / BF16),
+     0 = none.  Results do not depend on it. */
+  int32_t view_phased_iterations;
 } recon_mvs_plane_params;
 
 /* Fills *p with the defaults: 11x11 window, stride 2, top-4, 5 iterations,

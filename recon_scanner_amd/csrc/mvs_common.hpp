@@ -66,6 +66,7 @@ enum Slot : int {
   kSlotAdjIn,
   kSlotAdjOut,
   kSlotAdjPart,
+  kSlotPhase,
   kSlotCount
 };
 
@@ -100,6 +101,8 @@ struct recon_mvs_hip_context {
   std::map<std::string, int> tuned_schedule;
   // autotuned per-phase occupancy caps (plane mode, tile_schedule = -1)
   std::map<std::string, std::vector<int>> tuned_caps;
+  // tile orders of view-phased half-sweeps, per (reference, sources, tile)
+  std::map<std::string, int*> phase_orders;
 };
 
 namespace rmvs {

@@ -201,245 +201,295 @@ constexpr int kMaxTopKNet = 8;  // top_k below the view count is limited to 8
 // exceeds `reject_at` the hypothesis cannot beat it: the remaining views are
 // skipped and kCostInf is returned (callers pass the cost to beat, with a
 // rounding margin, or kCostInf to always score fully).
+// Per-hypothesis constants of PlaneCost: the pixel's centre and the plane as
+// m = K_r^-T n / rho (FP64).  front = false: the plane does not face the
+// camera (no view can be valid).
+struct HypGeom {
+  double qx, qy, D, m0, m1, m2;
+  bool front;
+};
+
+__device__ __forceinline__ HypGeom MakeHypGeom(const PlaneArgs& a, int x, int y, const Hyp& h) {
+  HypGeom g;
+  g.qx = x + 0.5;
+  g.qy = y + 0.5;
+  const double rx = (g.qx - a.cx) / a.fx, ry = (g.qy - a.cy) / a.fy;
+  const double nx = h.nx, ny = h.ny, nz = h.nz;
+  g.D = h.d;
+  const double nd = nx * rx + ny * ry + nz;
+  g.front = nd < 0.0;
+  const double rho = g.D * nd;
+  g.m0 = nx / a.fx / rho;
+  g.m1 = ny / a.fy / rho;
+  g.m2 = (nz - nx * a.cx / a.fx - ny * a.cy / a.fy) / rho;
+  return g;
+}
+
+// Costs of the views seen so far: count, sum and the sorted k smallest.
+struct CostAcc {
+  float topk[kMaxTopKNet];
+  float csum;
+  int nvalid;
+};
+
+__device__ __forceinline__ void ResetCostAcc(CostAcc& acc) {
+#pragma unroll
+  for (int k = 0; k < kMaxTopKNet; ++k) acc.topk[k] = kCostInf;
+  acc.csum = 0.0f;
+  acc.nvalid = 0;
+}
+
+// View v's NCC of the hypothesis (NaN when the view is invalid); a valid
+// view's cost 1 - NCC is added to `acc`.
+template <int HALF, int STRIDE, int FMT>
+__device__ __forceinline__ float ViewNcc(const PlaneArgs& a, int v, const HypGeom& hg,
+                                         const RefWindow<HALF, STRIDE>& win, bool use_topk,
+                                         CostAcc& acc) {
+  constexpr int NS = RefWindow<HALF, STRIDE>::NS;
+  constexpr int S = RefWindow<HALF, STRIDE>::S;
+  const double qx = hg.qx, qy = hg.qy, D = hg.D, m0 = hg.m0, m1 = hg.m1, m2 = hg.m2;
+  float* topk = acc.topk;
+  float& csum = acc.csum;
+  int& nvalid = acc.nvalid;
+  float ncc_v = __builtin_nanf("");
+  // H = A + b m^T, centre projection
+  double H[9];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    H[3 * i + 0] = fma(a.b[v][i], m0, a.A[v][3 * i + 0]);
+    H[3 * i + 1] = fma(a.b[v][i], m1, a.A[v][3 * i + 1]);
+    H[3 * i + 2] = fma(a.b[v][i], m2, a.A[v][3 * i + 2]);
+  }
+  const double hc0 = fma(H[0], qx, fma(H[1], qy, H[2]));
+  const double hc1 = fma(H[3], qx, fma(H[4], qy, H[5]));
+  const double hc2 = fma(H[6], qx, fma(H[7], qy, H[8]));
+  if (hc2 * D > 1e-6) {
+    const double ih = 1.0 / hc2;
+    const double uc = hc0 * ih, vc = hc1 * ih;       // continuous source coords
+    const double su = uc - 0.5, sv = vc - 0.5;       // sampler coords (pixel centres at integers)
+    if (fabs(su) < 1e7 && fabs(sv) < 1e7) {
+      const double fsu = floor(su), fsv = floor(sv);
+      const int iu = static_cast<int>(fsu), iv = static_cast<int>(fsv);
+      const float fu = static_cast<float>(su - fsu), fv = static_cast<float>(sv - fsv);
+      // Offset of a sample from the centre projection, for a window offset
+      // (dx, dy): du = (g00 dx + g01 dy) / (1 + h20 dx + h21 dy), and dv
+      // likewise with g10, g11; g = (H_row - u_c H_2) / hc2 in FP64.
+      const float g00 = static_cast<float>((H[0] - uc * H[6]) * ih);
+      const float g01 = static_cast<float>((H[1] - uc * H[7]) * ih);
+      const float g10 = static_cast<float>((H[3] - vc * H[6]) * ih);
+      const float g11 = static_cast<float>((H[4] - vc * H[7]) * ih);
+      const float h20 = static_cast<float>(H[6] * ih), h21 = static_cast<float>(H[7] * ih);
+      const int w = a.src_w[v], hgt = a.src_h[v], pitch = a.src_pitch[v];
+      const float lox = static_cast<float>(-iu), hix = static_cast<float>(w - 1 - iu);
+      const float loy = static_cast<float>(-iv), hiy = static_cast<float>(hgt - 1 - iv);
+      constexpr int kEB = ElemBytes<FMT>();
+      const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+          const_cast<void*>(a.src_img[v]), 0, pitch * hgt * kEB, 0x00020000);
+      // the window origin's row/column in the source, as a float index
+      // offset (exact: image sizes stay below 2^24 pixels)
+      const float fpitch = static_cast<float>(pitch);
+      const float obase = static_cast<float>(iv) * fpitch + static_cast<float>(iu);
+      const float shift = win.shift;
+      // A window entirely beyond one image edge cannot give a valid view:
+      // skip it.  Over the window |g0. dx + g0. dy| <= e0 (e1 for v) and
+      // the denominator is >= 1 - e2, so |du| <= e0 / (1 - e2).
+      const float e0 = HALF * (fabsf(g00) + fabsf(g01)), e1 = HALF * (fabsf(g10) + fabsf(g11));
+      const float e2 = HALF * (fabsf(h20) + fabsf(h21));
+      const float rd = __builtin_amdgcn_rcpf(1.0f - e2);
+      const float bu = e0 * rd + 1e-3f;
+      const float bv = e1 * rd + 1e-3f;
+      // e2 < 0.5 keeps every sample's denominator positive
+      const bool den_ok = e2 < 0.5f;
+      const bool outside = den_ok && (fu + bu < lox || fu - bu > hix || fv + bv < loy || fv - bv > hiy);
+      // Sums over the source samples; the reference sums start from the
+      // whole window and lose the reference values of excluded samples.
+      float ss = 0.0f, sss = 0.0f, srs = 0.0f, xr = 0.0f, xrr = 0.0f, xcnt = 0.0f;
+      // A window whose projection lies inside the source image with a
+      // 1e-4 px margin by the bound above (and whose reference window is
+      // whole) has every sample valid: such windows skip the per-sample
+      // masks.  The test is per window, so the branch below is taken once
+      // per (hypothesis, view), not per row.
+      const float mlox = lox + 1e-4f, mhix = hix - 1e-4f, mloy = loy + 1e-4f, mhiy = hiy - 1e-4f;
+      const bool win_in = den_ok & win.full & (fu - bu >= mlox) & (fu + bu <= mhix) &
+                          (fv - bv >= mloy) & (fv + bv <= mhiy);
+      if (!outside) {
+        if (__ballot(!win_in) == 0) {
+          // every lane's window is inside: no per-sample masks
+#pragma unroll 1
+          for (int j = 0; j < NS; ++j) {
+            const float dy = static_cast<float>(-HALF + j * STRIDE);
+            const float b0 = g01 * dy, b1 = g11 * dy, b2 = fmaf(h21, dy, 1.0f);
+            int off[NS];
+            float fxs[NS], fys[NS];
+#pragma unroll
+            for (int i = 0; i < NS; ++i) {
+              const float dx = static_cast<float>(-HALF + i * STRIDE);
+              const float r = __builtin_amdgcn_rcpf(fmaf(h20, dx, b2));
+              const float tx = fmaf(fmaf(g00, dx, b0), r, fu);
+              const float ty = fmaf(fmaf(g10, dx, b1), r, fv);
+              const float flx = floorf(tx), fly = floorf(ty);
+              fxs[i] = tx - flx;
+              fys[i] = ty - fly;
+              off[i] = static_cast<int>(fmaf(fly, fpitch, flx + obase)) * kEB;
+            }
+            float4 q[NS];
+#pragma unroll
+            for (int i = 0; i < NS; ++i) q[i] = BufQuad<FMT>(rsrc, off[i]);
+#pragma unroll
+            for (int i = 0; i < NS; ++i) {
+              const float top = fmaf(fxs[i], q[i].z - q[i].x, q[i].x);
+              const float bot = fmaf(fxs[i], q[i].w - q[i].y, q[i].y);
+              const float s = fmaf(fys[i], bot - top, top) - shift;
+              ss += s;
+              sss = fmaf(s, s, sss);
+              srs = fmaf(win.at(i, j), s, srs);
+            }
+          }
+        } else {
+          // some window crosses an edge: each sample is checked
+#pragma unroll 1
+          for (int j = 0; j < NS; ++j) {
+            const float dy = static_cast<float>(-HALF + j * STRIDE);
+            const float b0 = g01 * dy, b1 = g11 * dy, b2 = fmaf(h21, dy, 1.0f);
+            float txs[NS], tys[NS], dens[NS];
+#pragma unroll
+            for (int i = 0; i < NS; ++i) {
+              const float dx = static_cast<float>(-HALF + i * STRIDE);
+              dens[i] = fmaf(h20, dx, b2);
+              const float r = __builtin_amdgcn_rcpf(dens[i]);
+              txs[i] = fmaf(fmaf(g00, dx, b0), r, fu);
+              tys[i] = fmaf(fmaf(g10, dx, b1), r, fv);
+            }
+            int off[NS];
+            float fxs[NS], fys[NS];
+#pragma unroll
+            for (int i = 0; i < NS; ++i) {
+              const float flx = floorf(txs[i]), fly = floorf(tys[i]);
+              fxs[i] = txs[i] - flx;
+              fys[i] = tys[i] - fly;
+              off[i] = static_cast<int>(fmaf(fly, fpitch, flx + obase)) * kEB;
+            }
+            float4 q[NS];
+#pragma unroll
+            for (int i = 0; i < NS; ++i) q[i] = BufQuad<FMT>(rsrc, off[i]);
+#pragma unroll
+            for (int i = 0; i < NS; ++i) {
+              const float top = fmaf(fxs[i], q[i].z - q[i].x, q[i].x);
+              const float bot = fmaf(fxs[i], q[i].w - q[i].y, q[i].y);
+              const float sv = fmaf(fys[i], bot - top, top) - shift;
+              const float rv = win.at(i, j);
+              const bool rin = rv == rv;  // NaN: outside the reference image
+              const bool ok = win_in | (rin & (dens[i] > 0.0f) & (txs[i] >= lox) &
+                                        (txs[i] <= hix) & (tys[i] >= loy) & (tys[i] <= hiy));
+              const float s = ok ? sv : 0.0f;
+              const float x = (ok || !rin) ? 0.0f : rv;
+              ss += s;
+              sss = fmaf(s, s, sss);
+              srs = fmaf(ok ? rv : 0.0f, s, srs);
+              xr += x;
+              xrr = fmaf(x, x, xrr);
+              xcnt += (ok || !rin) ? 0.0f : 1.0f;
+            }
+          }
+        }
+      } else {
+        xcnt = win.n;
+      }
+      const float cnt = win.n - xcnt;
+      const float sr = win.sum_r - xr, srr = win.sum_r2 - xrr;
+      if (cnt >= static_cast<float>(S / 2) && cnt > 0.0f) {
+        const float in = __builtin_amdgcn_rcpf(cnt);
+        const float mr = sr * in, ms = ss * in;
+        const float vr = fmaf(srr, in, -mr * mr);
+        const float vs = fmaf(sss, in, -ms * ms);
+        const float cv = fmaf(srs, in, -mr * ms);
+        const float d2 = vr * vs;
+        if (d2 >= a.eps2 && d2 > 0.0f) {
+          float ncc = cv * __builtin_amdgcn_rsqf(d2);
+          ncc = fminf(fmaxf(ncc, -1.0f), 1.0f);
+          ncc_v = ncc;
+          float c = 1.0f - ncc;
+          ++nvalid;
+          csum += c;
+          if (use_topk) {
+            // insert into the sorted k smallest
+#pragma unroll
+            for (int k = 0; k < kMaxTopKNet; ++k) {
+              const float lo = fminf(topk[k], c);
+              c = fmaxf(topk[k], c);
+              topk[k] = lo;
+            }
+          }
+        }
+      }
+    }
+  }
+  return ncc_v;
+}
+
+// Lower bound of the final cost after view v of a.nsrc, every view still to
+// come taken as valid with cost 0 (adding such a view never raises the mean
+// of the k smallest).
+__device__ __forceinline__ float CostLowerBound(const PlaneArgs& a, int v, const CostAcc& acc) {
+  const int u = a.nsrc - 1 - v;  // views still to come
+  float lb = 0.0f;
+  if (acc.nvalid + u <= a.top_k) {
+    // every valid view is aggregated: the mean of the seen costs and u zeros
+    if (acc.nvalid > 0) lb = acc.csum / static_cast<float>(acc.nvalid + u);
+  } else {
+    // the k smallest: u zeros and the (k - u) smallest seen costs
+    const int need = a.top_k - u;
+    float sk = 0.0f;
+#pragma unroll
+    for (int k = 0; k < kMaxTopKNet; ++k) sk += (k < need) ? acc.topk[k] : 0.0f;
+    if (need > 0) lb = sk / static_cast<float>(a.top_k);
+  }
+  return lb;
+}
+
+// The aggregated cost of all views: mean of the k smallest view costs (or of
+// all when top_k covers every view); kCostInf without a valid view or above
+// the photometric rejection threshold.
+__device__ __forceinline__ float FinishCost(const PlaneArgs& a, const CostAcc& acc, bool use_topk) {
+  if (acc.nvalid == 0) return kCostInf;
+  float cost;
+  if (use_topk) {
+    const int kk = min(a.top_k, acc.nvalid);
+    float sum = 0.0f;
+#pragma unroll
+    for (int k = 0; k < kMaxTopKNet; ++k) sum += (k < kk) ? acc.topk[k] : 0.0f;
+    cost = sum / static_cast<float>(kk);
+  } else {
+    cost = acc.csum / static_cast<float>(acc.nvalid);
+  }
+  return cost > a.photo_reject ? kCostInf : cost;
+}
+
 template <int HALF, int STRIDE, int FMT>
 __device__ float PlaneCost(const PlaneArgs& a, int x, int y, const RefWindow<HALF, STRIDE>& win,
                            const Hyp& h, int* valid_views_out, float* view_ncc,
                            float reject_at = kCostInf) {
-  constexpr int NS = RefWindow<HALF, STRIDE>::NS;
-  constexpr int S = RefWindow<HALF, STRIDE>::S;
   *valid_views_out = 0;
   if (!(h.d >= a.min_depth && h.d <= a.max_depth)) {
     if (view_ncc)
       for (int v = 0; v < a.nsrc; ++v) view_ncc[v] = __builtin_nanf("");
     return kCostInf;
   }
-  // FP64: plane -> m = K_r^-T n / rho
-  const double qx = x + 0.5, qy = y + 0.5;
-  const double rx = (qx - a.cx) / a.fx, ry = (qy - a.cy) / a.fy;
-  const double nx = h.nx, ny = h.ny, nz = h.nz, D = h.d;
-  const double nd = nx * rx + ny * ry + nz;
+  const HypGeom hg = MakeHypGeom(a, x, y, h);
   const bool use_topk = a.top_k < a.nsrc;
-  float topk[kMaxTopKNet];
-#pragma unroll
-  for (int k = 0; k < kMaxTopKNet; ++k) topk[k] = kCostInf;
-  float csum = 0.0f;
-  int nvalid = 0;
-  if (nd < 0.0) {
-    const double rho = D * nd;
-    const double m0 = nx / a.fx / rho;
-    const double m1 = ny / a.fy / rho;
-    const double m2 = (nz - nx * a.cx / a.fx - ny * a.cy / a.fy) / rho;
+  CostAcc acc;
+  ResetCostAcc(acc);
+  if (hg.front) {
     for (int v = 0; v < a.nsrc; ++v) {
-      float ncc_v = __builtin_nanf("");
-      // H = A + b m^T, centre projection
-      double H[9];
-#pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        H[3 * i + 0] = fma(a.b[v][i], m0, a.A[v][3 * i + 0]);
-        H[3 * i + 1] = fma(a.b[v][i], m1, a.A[v][3 * i + 1]);
-        H[3 * i + 2] = fma(a.b[v][i], m2, a.A[v][3 * i + 2]);
-      }
-      const double hc0 = fma(H[0], qx, fma(H[1], qy, H[2]));
-      const double hc1 = fma(H[3], qx, fma(H[4], qy, H[5]));
-      const double hc2 = fma(H[6], qx, fma(H[7], qy, H[8]));
-      if (hc2 * D > 1e-6) {
-        const double ih = 1.0 / hc2;
-        const double uc = hc0 * ih, vc = hc1 * ih;       // continuous source coords
-        const double su = uc - 0.5, sv = vc - 0.5;       // sampler coords (pixel centres at integers)
-        if (fabs(su) < 1e7 && fabs(sv) < 1e7) {
-          const double fsu = floor(su), fsv = floor(sv);
-          const int iu = static_cast<int>(fsu), iv = static_cast<int>(fsv);
-          const float fu = static_cast<float>(su - fsu), fv = static_cast<float>(sv - fsv);
-          // Offset of a sample from the centre projection, for a window offset
-          // (dx, dy): du = (g00 dx + g01 dy) / (1 + h20 dx + h21 dy), and dv
-          // likewise with g10, g11; g = (H_row - u_c H_2) / hc2 in FP64.
-          const float g00 = static_cast<float>((H[0] - uc * H[6]) * ih);
-          const float g01 = static_cast<float>((H[1] - uc * H[7]) * ih);
-          const float g10 = static_cast<float>((H[3] - vc * H[6]) * ih);
-          const float g11 = static_cast<float>((H[4] - vc * H[7]) * ih);
-          const float h20 = static_cast<float>(H[6] * ih), h21 = static_cast<float>(H[7] * ih);
-          const int w = a.src_w[v], hgt = a.src_h[v], pitch = a.src_pitch[v];
-          const float lox = static_cast<float>(-iu), hix = static_cast<float>(w - 1 - iu);
-          const float loy = static_cast<float>(-iv), hiy = static_cast<float>(hgt - 1 - iv);
-          constexpr int kEB = ElemBytes<FMT>();
-          const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
-              const_cast<void*>(a.src_img[v]), 0, pitch * hgt * kEB, 0x00020000);
-          // the window origin's row/column in the source, as a float index
-          // offset (exact: image sizes stay below 2^24 pixels)
-          const float fpitch = static_cast<float>(pitch);
-          const float obase = static_cast<float>(iv) * fpitch + static_cast<float>(iu);
-          const float shift = win.shift;
-          // A window entirely beyond one image edge cannot give a valid view:
-          // skip it.  Over the window |g0. dx + g0. dy| <= e0 (e1 for v) and
-          // the denominator is >= 1 - e2, so |du| <= e0 / (1 - e2).
-          const float e0 = HALF * (fabsf(g00) + fabsf(g01)), e1 = HALF * (fabsf(g10) + fabsf(g11));
-          const float e2 = HALF * (fabsf(h20) + fabsf(h21));
-          const float rd = __builtin_amdgcn_rcpf(1.0f - e2);
-          const float bu = e0 * rd + 1e-3f;
-          const float bv = e1 * rd + 1e-3f;
-          // e2 < 0.5 keeps every sample's denominator positive
-          const bool den_ok = e2 < 0.5f;
-          const bool outside = den_ok && (fu + bu < lox || fu - bu > hix || fv + bv < loy || fv - bv > hiy);
-          // Sums over the source samples; the reference sums start from the
-          // whole window and lose the reference values of excluded samples.
-          float ss = 0.0f, sss = 0.0f, srs = 0.0f, xr = 0.0f, xrr = 0.0f, xcnt = 0.0f;
-          // A window whose projection lies inside the source image with a
-          // 1e-4 px margin by the bound above (and whose reference window is
-          // whole) has every sample valid: such windows skip the per-sample
-          // masks.  The test is per window, so the branch below is taken once
-          // per (hypothesis, view), not per row.
-          const float mlox = lox + 1e-4f, mhix = hix - 1e-4f, mloy = loy + 1e-4f, mhiy = hiy - 1e-4f;
-          const bool win_in = den_ok & win.full & (fu - bu >= mlox) & (fu + bu <= mhix) &
-                              (fv - bv >= mloy) & (fv + bv <= mhiy);
-          if (!outside) {
-            if (__ballot(!win_in) == 0) {
-              // every lane's window is inside: no per-sample masks
-#pragma unroll 1
-              for (int j = 0; j < NS; ++j) {
-                const float dy = static_cast<float>(-HALF + j * STRIDE);
-                const float b0 = g01 * dy, b1 = g11 * dy, b2 = fmaf(h21, dy, 1.0f);
-                int off[NS];
-                float fxs[NS], fys[NS];
-#pragma unroll
-                for (int i = 0; i < NS; ++i) {
-                  const float dx = static_cast<float>(-HALF + i * STRIDE);
-                  const float r = __builtin_amdgcn_rcpf(fmaf(h20, dx, b2));
-                  const float tx = fmaf(fmaf(g00, dx, b0), r, fu);
-                  const float ty = fmaf(fmaf(g10, dx, b1), r, fv);
-                  const float flx = floorf(tx), fly = floorf(ty);
-                  fxs[i] = tx - flx;
-                  fys[i] = ty - fly;
-                  off[i] = static_cast<int>(fmaf(fly, fpitch, flx + obase)) * kEB;
-                }
-                float4 q[NS];
-#pragma unroll
-                for (int i = 0; i < NS; ++i) q[i] = BufQuad<FMT>(rsrc, off[i]);
-#pragma unroll
-                for (int i = 0; i < NS; ++i) {
-                  const float top = fmaf(fxs[i], q[i].z - q[i].x, q[i].x);
-                  const float bot = fmaf(fxs[i], q[i].w - q[i].y, q[i].y);
-                  const float s = fmaf(fys[i], bot - top, top) - shift;
-                  ss += s;
-                  sss = fmaf(s, s, sss);
-                  srs = fmaf(win.at(i, j), s, srs);
-                }
-              }
-            } else {
-              // some window crosses an edge: each sample is checked
-#pragma unroll 1
-              for (int j = 0; j < NS; ++j) {
-                const float dy = static_cast<float>(-HALF + j * STRIDE);
-                const float b0 = g01 * dy, b1 = g11 * dy, b2 = fmaf(h21, dy, 1.0f);
-                float txs[NS], tys[NS], dens[NS];
-#pragma unroll
-                for (int i = 0; i < NS; ++i) {
-                  const float dx = static_cast<float>(-HALF + i * STRIDE);
-                  dens[i] = fmaf(h20, dx, b2);
-                  const float r = __builtin_amdgcn_rcpf(dens[i]);
-                  txs[i] = fmaf(fmaf(g00, dx, b0), r, fu);
-                  tys[i] = fmaf(fmaf(g10, dx, b1), r, fv);
-                }
-                int off[NS];
-                float fxs[NS], fys[NS];
-#pragma unroll
-                for (int i = 0; i < NS; ++i) {
-                  const float flx = floorf(txs[i]), fly = floorf(tys[i]);
-                  fxs[i] = txs[i] - flx;
-                  fys[i] = tys[i] - fly;
-                  off[i] = static_cast<int>(fmaf(fly, fpitch, flx + obase)) * kEB;
-                }
-                float4 q[NS];
-#pragma unroll
-                for (int i = 0; i < NS; ++i) q[i] = BufQuad<FMT>(rsrc, off[i]);
-#pragma unroll
-                for (int i = 0; i < NS; ++i) {
-                  const float top = fmaf(fxs[i], q[i].z - q[i].x, q[i].x);
-                  const float bot = fmaf(fxs[i], q[i].w - q[i].y, q[i].y);
-                  const float sv = fmaf(fys[i], bot - top, top) - shift;
-                  const float rv = win.at(i, j);
-                  const bool rin = rv == rv;  // NaN: outside the reference image
-                  const bool ok = win_in | (rin & (dens[i] > 0.0f) & (txs[i] >= lox) &
-                                            (txs[i] <= hix) & (tys[i] >= loy) & (tys[i] <= hiy));
-                  const float s = ok ? sv : 0.0f;
-                  const float x = (ok || !rin) ? 0.0f : rv;
-                  ss += s;
-                  sss = fmaf(s, s, sss);
-                  srs = fmaf(ok ? rv : 0.0f, s, srs);
-                  xr += x;
-                  xrr = fmaf(x, x, xrr);
-                  xcnt += (ok || !rin) ? 0.0f : 1.0f;
-                }
-              }
-            }
-          } else {
-            xcnt = win.n;
-          }
-          const float cnt = win.n - xcnt;
-          const float sr = win.sum_r - xr, srr = win.sum_r2 - xrr;
-          if (cnt >= static_cast<float>(S / 2) && cnt > 0.0f) {
-            const float in = __builtin_amdgcn_rcpf(cnt);
-            const float mr = sr * in, ms = ss * in;
-            const float vr = fmaf(srr, in, -mr * mr);
-            const float vs = fmaf(sss, in, -ms * ms);
-            const float cv = fmaf(srs, in, -mr * ms);
-            const float d2 = vr * vs;
-            if (d2 >= a.eps2 && d2 > 0.0f) {
-              float ncc = cv * __builtin_amdgcn_rsqf(d2);
-              ncc = fminf(fmaxf(ncc, -1.0f), 1.0f);
-              ncc_v = ncc;
-              float c = 1.0f - ncc;
-              ++nvalid;
-              csum += c;
-              if (use_topk) {
-                // insert into the sorted k smallest
-#pragma unroll
-                for (int k = 0; k < kMaxTopKNet; ++k) {
-                  const float lo = fminf(topk[k], c);
-                  c = fmaxf(topk[k], c);
-                  topk[k] = lo;
-                }
-              }
-            }
-          }
-        }
-      }
+      const float ncc_v = ViewNcc<HALF, STRIDE, FMT>(a, v, hg, win, use_topk, acc);
       if (view_ncc) view_ncc[v] = ncc_v;
-      if (reject_at < kCostInf) {
-        const int u = a.nsrc - 1 - v;  // views still to come
-        float lb = 0.0f;
-        if (nvalid + u <= a.top_k) {
-          // every valid view is aggregated: the mean of the seen costs and u zeros
-          if (nvalid > 0) lb = csum / static_cast<float>(nvalid + u);
-        } else {
-          // the k smallest: u zeros and the (k - u) smallest seen costs
-          const int need = a.top_k - u;
-          float sk = 0.0f;
-#pragma unroll
-          for (int k = 0; k < kMaxTopKNet; ++k) sk += (k < need) ? topk[k] : 0.0f;
-          if (need > 0) lb = sk / static_cast<float>(a.top_k);
-        }
-        if (lb > reject_at) {
-          *valid_views_out = 0;
-          return kCostInf;
-        }
-      }
+      if (reject_at < kCostInf && CostLowerBound(a, v, acc) > reject_at) return kCostInf;
     }
   } else if (view_ncc) {
     for (int v = 0; v < a.nsrc; ++v) view_ncc[v] = __builtin_nanf("");
   }
-  *valid_views_out = nvalid;
-  if (nvalid == 0) return kCostInf;
-  float cost;
-  if (use_topk) {
-    const int kk = min(a.top_k, nvalid);
-    float sum = 0.0f;
-#pragma unroll
-    for (int k = 0; k < kMaxTopKNet; ++k) sum += (k < kk) ? topk[k] : 0.0f;
-    cost = sum / static_cast<float>(kk);
-  } else {
-    cost = csum / static_cast<float>(nvalid);
-  }
-  return cost > a.photo_reject ? kCostInf : cost;
+  *valid_views_out = acc.nvalid;
+  return FinishCost(a, acc, use_topk);
 }
 
 // ---------------------------------------------------------------------------
@@ -511,12 +561,14 @@ __device__ __forceinline__ bool TileOf(const PlaneArgs& a, int* tx, int* ty) {
 }
 
 // Reference tile of a workgroup: TW x TH pixels plus a HALF-pixel halo in
-// LDS, every value shifted by the tile's centre pixel (NaN outside the
-// image).  256 threads; a thread owns one pixel of each colour: thread t ->
-// row t / (TW/2), the (t % (TW/2))-th pixel of the colour in that row.
+// LDS, every value shifted by the value at the centre of the 128 x 16 block
+// that holds the tile (NaN outside the image).  256 threads; a thread owns
+// one pixel of each colour: thread t -> row t / (TW/2), the (t % (TW/2))-th
+// pixel of the colour in that row.
 template <int HALF, int FMT, int TW, int TH>
 struct RefTile {
   static_assert(TW * TH == 512, "256 threads, two pixels each");
+  static_assert(128 % TW == 0 && 16 % TH == 0, "a tile lies inside one 128 x 16 block");
   static constexpr int LW = TW + 2 * HALF + 1;  // +1: odd row pitch against bank conflicts
   static constexpr int LH = TH + 2 * HALF;
   float v[LH][LW];
@@ -524,7 +576,10 @@ struct RefTile {
 
   __device__ __forceinline__ void load(const PlaneArgs& a, int x0, int y0) {
     if (threadIdx.x == 0) {
-      const int cx = min(x0 + TW / 2, a.w - 1), cy = min(y0 + TH / 2, a.h - 1);
+      // the centre of the 128 x 16 block holding the tile: every tile
+      // schedule (64x8, 32x16, 128x4) lies inside one such block, so a pixel
+      // sees the same shift, and the same rounding, under every schedule
+      const int cx = min((x0 & ~127) + 64, a.w - 1), cy = min((y0 & ~15) + 8, a.h - 1);
       shift = Pix<FMT>(a.ref_img, cy * a.ref_pitch + cx);
     }
     __syncthreads();

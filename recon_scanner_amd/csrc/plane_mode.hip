@@ -82,6 +82,71 @@ __device__ __forceinline__ bool TransferPlane(const PlaneArgs& a, float4 p, int 
 
 constexpr int kMaxReach = 16;
 
+// Propagation sources of pixel (x, y): per direction the lowest-cost
+// other-colour pixel at odd distances 1 .. 2*reach-1 (all reads issued
+// before any comparison); -1 where there is none.
+__device__ __forceinline__ void PropagationSources(const PlaneArgs& a, int x, int y, int* src_x,
+                                                   int* src_y) {
+  const int ddx[4] = {-1, 1, 0, 0};
+  const int ddy[4] = {0, 0, -1, 1};
+#pragma unroll
+  for (int d = 0; d < 4; ++d) {
+    float cs[kMaxReach];
+#pragma unroll
+    for (int s = 0; s < kMaxReach; ++s) {
+      const int nx = x + ddx[d] * (2 * s + 1), ny = y + ddy[d] * (2 * s + 1);
+      const bool in = s < a.reach && nx >= 0 && ny >= 0 && nx < a.w && ny < a.h;
+      cs[s] = in ? a.cost[ny * a.w + nx] : kCostInf;
+    }
+    src_x[d] = -1;
+    src_y[d] = -1;
+    float best = kCostInf;
+#pragma unroll
+    for (int s = 0; s < kMaxReach; ++s) {
+      if (cs[s] < best) {
+        best = cs[s];
+        src_x[d] = x + ddx[d] * (2 * s + 1);
+        src_y[d] = y + ddy[d] * (2 * s + 1);
+      }
+    }
+  }
+}
+
+// Candidate c of a half-sweep at pixel (x, y): 0-3 the planes of the
+// propagation sources, 4-6 perturbations of `base` (the plane the pixel had
+// at the start of the sweep: 4 depth, 5 normal, 6 both), 7 a random plane.
+// false: no candidate.
+__device__ __forceinline__ bool SweepCandidate(const PlaneArgs& a, int c, const float* ray, uint32_t key,
+                                               float4 base, float dscale, float nscale, const int* src_x,
+                                               const int* src_y, Hyp* h) {
+  bool ok = true;
+  if (c < 4) {
+    if (src_x[c] < 0) return false;
+    ok = TransferPlane(a, a.plane[src_y[c] * a.w + src_x[c]], src_x[c], src_y[c], ray, h);
+  } else if (c < 7) {
+    const uint32_t k0 = 4u * static_cast<uint32_t>(c - 4);
+    float d = base.x;
+    float n[3] = {base.y, base.z, base.w};
+    if (c != 5) {
+      const float inv = 1.0f / d;
+      const float u = 2.0f * RngUniform(key, k0) - 1.0f;
+      const float inv2 = inv + inv * (dscale * u);
+      ok = inv2 >= a.inv_min && inv2 <= a.inv_max;
+      d = 1.0f / inv2;
+    }
+    if (ok && c != 4) {
+      n[0] = n[0] + nscale * (2.0f * RngUniform(key, k0 + 1) - 1.0f);
+      n[1] = n[1] + nscale * (2.0f * RngUniform(key, k0 + 2) - 1.0f);
+      n[2] = n[2] + nscale * (2.0f * RngUniform(key, k0 + 3) - 1.0f);
+      ok = Normalize3(n) && (n[0] * ray[0] + n[1] * ray[1] + n[2] * ray[2]) < 0.0f;
+    }
+    *h = Hyp{d, n[0], n[1], n[2]};
+  } else {
+    *h = RandomPlane(a, key, 12u, ray);
+  }
+  return ok;
+}
+
 // One red-black half-sweep of one tile over the pixels of colour `color`.
 template <int HALF, int STRIDE, int FMT, int TW, int TH>
 __device__ __forceinline__ void SweepTile(const PlaneArgs& a, RefTile<HALF, FMT, TW, TH>& tile,
@@ -105,64 +170,11 @@ __device__ __forceinline__ void SweepTile(const PlaneArgs& a, RefTile<HALF, FMT,
   const float dscale = ldexpf(a.depth_perturb, -iter);
   const float nscale = ldexpf(a.normal_perturb, -iter);
   const float4 base = cur;
-  // propagation sources: per direction the lowest-cost other-colour pixel at
-  // odd distances 1 .. 2*reach-1 (all reads issued before any comparison)
   int src_x[4], src_y[4];
-  {
-    const int ddx[4] = {-1, 1, 0, 0};
-    const int ddy[4] = {0, 0, -1, 1};
-#pragma unroll
-    for (int d = 0; d < 4; ++d) {
-      float cs[kMaxReach];
-#pragma unroll
-      for (int s = 0; s < kMaxReach; ++s) {
-        const int nx = x + ddx[d] * (2 * s + 1), ny = y + ddy[d] * (2 * s + 1);
-        const bool in = s < a.reach && nx >= 0 && ny >= 0 && nx < a.w && ny < a.h;
-        cs[s] = in ? a.cost[ny * a.w + nx] : kCostInf;
-      }
-      src_x[d] = -1;
-      src_y[d] = -1;
-      float best = kCostInf;
-#pragma unroll
-      for (int s = 0; s < kMaxReach; ++s) {
-        if (cs[s] < best) {
-          best = cs[s];
-          src_x[d] = x + ddx[d] * (2 * s + 1);
-          src_y[d] = y + ddy[d] * (2 * s + 1);
-        }
-      }
-    }
-  }
+  PropagationSources(a, x, y, src_x, src_y);
   for (int c = 0; c < 8; ++c) {
     Hyp h;
-    bool ok = true;
-    if (c < 4) {
-      if (src_x[c] < 0) continue;
-      ok = TransferPlane(a, a.plane[src_y[c] * a.w + src_x[c]], src_x[c], src_y[c], ray, &h);
-    } else if (c < 7) {
-      // 4: depth, 5: normal, 6: both, perturbed around the plane the pixel
-      // had at the start of this sweep
-      const uint32_t k0 = 4u * static_cast<uint32_t>(c - 4);
-      float d = base.x;
-      float n[3] = {base.y, base.z, base.w};
-      if (c != 5) {
-        const float inv = 1.0f / d;
-        const float u = 2.0f * RngUniform(key, k0) - 1.0f;
-        const float inv2 = inv + inv * (dscale * u);
-        ok = inv2 >= a.inv_min && inv2 <= a.inv_max;
-        d = 1.0f / inv2;
-      }
-      if (ok && c != 4) {
-        n[0] = n[0] + nscale * (2.0f * RngUniform(key, k0 + 1) - 1.0f);
-        n[1] = n[1] + nscale * (2.0f * RngUniform(key, k0 + 2) - 1.0f);
-        n[2] = n[2] + nscale * (2.0f * RngUniform(key, k0 + 3) - 1.0f);
-        ok = Normalize3(n) && (n[0] * ray[0] + n[1] * ray[1] + n[2] * ray[2]) < 0.0f;
-      }
-      h = Hyp{d, n[0], n[1], n[2]};
-    } else {
-      h = RandomPlane(a, key, 12u, ray);
-    }
-    if (!ok) continue;
+    if (!SweepCandidate(a, c, ray, key, base, dscale, nscale, src_x, src_y, &h)) continue;
     int nv = 0;
     // candidates that provably cannot beat the current cost stop early (the
     // margin covers the rounding of the bound against the full sum)
@@ -179,17 +191,165 @@ __device__ __forceinline__ void SweepTile(const PlaneArgs& a, RefTile<HALF, FMT,
   a.support[idx] = static_cast<uint8_t>(min(scur, 255));
 }
 
-template <int HALF, int STRIDE, int FMT, int TW, int TH>
 // Stride-2 windows: 3 waves per SIMD (at most 168 VGPRs; without the limit
 // the compiler takes ~172): converged half-sweeps are VALU bound and gain from
 // the third wave; the unconverged ones are held at lower occupancy by the
 // per-phase caps.  Stride-1 windows would spill under the limit.
+template <int HALF, int STRIDE, int FMT, int TW, int TH>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STRIDE == 2 ? 3 : 1)))
 void k_plane_sweep(PlaneArgs a) {
   __shared__ RefTile<HALF, FMT, TW, TH> tile;
   int tx, ty;
   if (!TileOf(a, &tx, &ty)) return;
   SweepTile<HALF, STRIDE, FMT, TW, TH>(a, tile, tx, ty, a.color, a.iter);
+}
+
+// ---------------------------------------------------------------------------
+// View-phased half-sweep: the same half-sweep as k_plane_sweep in 2 + nsrc
+// launches.  k_phase_prep makes each pixel's 8 candidates, one k_phase_view
+// launch per source view adds that view to every live candidate's
+// accumulators (in memory between launches), and k_phase_final scores the
+// candidates and keeps the best in candidate order.  Each view launch touches
+// one source image only, and visits the tiles in strips along that view's
+// epipolar lines (`order`), so the source footprint of the tiles in flight on
+// an XCD stays within its L2 -- the unconverged sweeps' gathers otherwise
+// spread over all sources at once.
+//
+// Results are bit-identical to k_plane_sweep: every candidate's view costs
+// are accumulated by the same operations in the same view order; the early
+// exit uses the cost the pixel had at the start of the sweep, a bound no
+// tighter than the running best of k_plane_sweep, so it drops only candidates
+// that k_plane_sweep drops too or that lose to the running best anyway.
+// ---------------------------------------------------------------------------
+
+// Candidate state in the colour's compact pixel index ci = y * cw + (x >> 1),
+// structure of arrays, slot c of candidate c: [c * n + ci].
+struct PhaseState {
+  float4* cand;  // candidate planes
+  float* topk;   // [(c * K + k) * n + ci]: the K smallest view costs so far
+  float* csum;   // sum of the view costs so far
+  int* nvalid;   // valid views so far; -1: candidate out (invalid or rejected)
+  int n, cw, K;  // K = top_k when the k smallest are aggregated, else 0
+};
+
+// The pixel of a thread in a tile, as RefTile::pixel.
+template <int TW>
+__device__ __forceinline__ void TilePixel(int x0, int y0, int color, int* x, int* y) {
+  const int r = threadIdx.x / (TW / 2);
+  const int j = threadIdx.x % (TW / 2);
+  *y = y0 + r;
+  *x = x0 + 2 * j + (((x0 + *y) & 1) ^ color);
+}
+
+template <int HALF, int STRIDE, int FMT, int TW, int TH>
+__global__ __launch_bounds__(256) void k_phase_prep(PlaneArgs a, PhaseState st, int color, int iter) {
+  int tx, ty;
+  if (!TileOf(a, &tx, &ty)) return;
+  int x, y;
+  TilePixel<TW>(tx * TW, ty * TH, color, &x, &y);
+  if (x >= a.w || y >= a.h) return;
+  const int idx = y * a.w + x;
+  const int ci = y * st.cw + (x >> 1);
+  float ray[3];
+  PixelRay(a, x, y, ray);
+  const float4 base = a.plane[idx];
+  const uint32_t key = RngKey(a.seed, static_cast<uint32_t>(1 + 2 * iter + color),
+                              static_cast<uint32_t>(idx));
+  const float dscale = ldexpf(a.depth_perturb, -iter);
+  const float nscale = ldexpf(a.normal_perturb, -iter);
+  int src_x[4], src_y[4];
+  PropagationSources(a, x, y, src_x, src_y);
+  for (int c = 0; c < 8; ++c) {
+    Hyp h{0.0f, 0.0f, 0.0f, -1.0f};
+    // a candidate PlaneCost would score kCostInf without any view is out now
+    const bool ok = SweepCandidate(a, c, ray, key, base, dscale, nscale, src_x, src_y, &h) &&
+                    h.d >= a.min_depth && h.d <= a.max_depth && MakeHypGeom(a, x, y, h).front;
+    const size_t i = static_cast<size_t>(c) * st.n + ci;
+    st.cand[i] = make_float4(h.d, h.nx, h.ny, h.nz);
+    st.nvalid[i] = ok ? 0 : -1;
+    st.csum[i] = 0.0f;
+    for (int k = 0; k < st.K; ++k) st.topk[(static_cast<size_t>(c) * st.K + k) * st.n + ci] = kCostInf;
+  }
+}
+
+template <int HALF, int STRIDE, int FMT, int TW, int TH>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STRIDE == 2 ? 3 : 1)))
+void k_phase_view(PlaneArgs a, PhaseState st, const int* __restrict__ order, int v, int color) {
+  __shared__ RefTile<HALF, FMT, TW, TH> tile;
+  const int b = blockIdx.x;
+  const int t = (b & 7) * a.tiles_per_xcd + (b >> 3);  // XCD-aware, as TileOf
+  if (t >= a.num_tiles) return;
+  const int packed = order[t];
+  const int x0 = (packed & 0xFFFF) * TW, y0 = (packed >> 16) * TH;
+  tile.load(a, x0, y0);
+  __syncthreads();
+  int x, y, r;
+  tile.pixel(x0, y0, color, &x, &y, &r);
+  if (x >= a.w || y >= a.h) return;
+  RefWindow<HALF, STRIDE> win;
+  tile.template window<STRIDE>(win, x, x0, r);
+  const int ci = y * st.cw + (x >> 1);
+  const float ccur = a.cost[y * a.w + x];
+  const float reject_at = ccur < kCostInf ? fmaf(ccur, 1.0f + 1e-5f, 1e-7f) : kCostInf;
+  const bool use_topk = a.top_k < a.nsrc;
+  for (int c = 0; c < 8; ++c) {
+    const size_t i = static_cast<size_t>(c) * st.n + ci;
+    const int nv = st.nvalid[i];
+    if (nv < 0) continue;
+    const float4 p = st.cand[i];
+    const HypGeom hg = MakeHypGeom(a, x, y, Hyp{p.x, p.y, p.z, p.w});
+    CostAcc acc;
+    acc.nvalid = nv;
+    acc.csum = st.csum[i];
+#pragma unroll
+    for (int k = 0; k < kMaxTopKNet; ++k)
+      acc.topk[k] = k < st.K ? st.topk[(static_cast<size_t>(c) * st.K + k) * st.n + ci] : kCostInf;
+    (void)ViewNcc<HALF, STRIDE, FMT>(a, v, hg, win, use_topk, acc);
+    if (reject_at < kCostInf && CostLowerBound(a, v, acc) > reject_at) {
+      st.nvalid[i] = -1;
+      continue;
+    }
+    if (acc.nvalid == nv) continue;  // view v invalid for it: nothing changed
+    st.nvalid[i] = acc.nvalid;
+    st.csum[i] = acc.csum;
+#pragma unroll
+    for (int k = 0; k < kMaxTopKNet; ++k)
+      if (k < st.K) st.topk[(static_cast<size_t>(c) * st.K + k) * st.n + ci] = acc.topk[k];
+  }
+}
+
+template <int HALF, int STRIDE, int FMT, int TW, int TH>
+__global__ __launch_bounds__(256) void k_phase_final(PlaneArgs a, PhaseState st, int color) {
+  int tx, ty;
+  if (!TileOf(a, &tx, &ty)) return;
+  int x, y;
+  TilePixel<TW>(tx * TW, ty * TH, color, &x, &y);
+  if (x >= a.w || y >= a.h) return;
+  const int idx = y * a.w + x;
+  const int ci = y * st.cw + (x >> 1);
+  float4 cur = a.plane[idx];
+  float ccur = a.cost[idx];
+  int scur = a.support[idx];
+  const bool use_topk = a.top_k < a.nsrc;
+  for (int c = 0; c < 8; ++c) {
+    const size_t i = static_cast<size_t>(c) * st.n + ci;
+    CostAcc acc;
+    acc.nvalid = st.nvalid[i];
+    if (acc.nvalid < 0) continue;
+    acc.csum = st.csum[i];
+#pragma unroll
+    for (int k = 0; k < kMaxTopKNet; ++k)
+      acc.topk[k] = k < st.K ? st.topk[(static_cast<size_t>(c) * st.K + k) * st.n + ci] : kCostInf;
+    const float cc = FinishCost(a, acc, use_topk);
+    if (cc < ccur) {
+      ccur = cc;
+      cur = st.cand[i];
+      scur = acc.nvalid;
+    }
+  }
+  a.plane[idx] = cur;
+  a.cost[idx] = ccur;
+  a.support[idx] = static_cast<uint8_t>(min(scur, 255));
 }
 
 // Batched, persistent form: one launch processes (job, tile) units of many
@@ -524,6 +684,8 @@ int32_t CheckParams(recon_mvs_hip_context* ctx, const recon_mvs_plane_params* p)
     return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "propagation_reach above 16");
   if (p->occupancy_cap < -1 || p->occupancy_cap > 64)
     return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "occupancy_cap out of [-1, 64]");
+  if (p->view_phased_iterations < -1)
+    return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "view_phased_iterations below -1");
   return RECON_MVS_OK;
 }
 
@@ -647,7 +809,11 @@ int32_t EnsureState(recon_mvs_hip_context* ctx, int view) {
   return RECON_MVS_OK;
 }
 
-enum KernelId { kSweep = 0, kInit = 1, kRefine = 2, kGeo = 3, kScore = 4, kEval = 5, kBatch = 6 };
+enum KernelId {
+  kSweep = 0, kInit = 1, kRefine = 2, kGeo = 3, kScore = 4, kEval = 5, kBatch = 6,
+  // view-phased half-sweep parts (timed together as kSweep)
+  kPhasePrep = 8, kPhaseView = 9, kPhaseFinal = 10
+};
 
 // Arguments of one persistent batched launch.
 struct BatchLaunch {
@@ -656,6 +822,13 @@ struct BatchLaunch {
   int num_units;
   unsigned int* counter;
   int phase;
+};
+
+// Arguments of one part of a view-phased half-sweep.
+struct PhaseLaunch {
+  PhaseState st;
+  const int* order;  // kPhaseView: this view's tile order
+  int view, color, iter;
 };
 
 // Tile geometry of schedule `s` (see kNumSchedules).
@@ -703,7 +876,8 @@ int BatchGrid(int cap) {
 
 template <int HALF, int STRIDE, int FMT, int TW, int TH>
 void LaunchTile(KernelId id, PlaneArgs a, hipStream_t s, float f0, int i0, int n, const float* hyps,
-                float* cost, int* nvalid, float* view_ncc, const BatchLaunch* bl, int cap) {
+                float* cost, int* nvalid, float* view_ncc, const BatchLaunch* bl, int cap,
+                const PhaseLaunch* pl) {
   constexpr size_t kTileLds = sizeof(RefTile<HALF, FMT, TW, TH>) + 64;
   const size_t pad = CapBytes(cap, kTileLds);
   if (id == kBatch) {
@@ -728,6 +902,20 @@ void LaunchTile(KernelId id, PlaneArgs a, hipStream_t s, float f0, int i0, int n
       hipLaunchKernelGGL((k_plane_eval<HALF, STRIDE, FMT>), dim3((n + 63) / 64), dim3(64), 0, s, a,
                          n, hyps, cost, nvalid, view_ncc);
       break;
+    case kPhasePrep:
+      if constexpr (HALF == 5 && STRIDE == 2)
+        hipLaunchKernelGGL((k_phase_prep<HALF, STRIDE, FMT, TW, TH>), grid, block, 0, s, a, pl->st, pl->color,
+                           pl->iter);
+      break;
+    case kPhaseView:
+      if constexpr (HALF == 5 && STRIDE == 2)
+        hipLaunchKernelGGL((k_phase_view<HALF, STRIDE, FMT, TW, TH>), grid, block, pad, s, a, pl->st, pl->order,
+                           pl->view, pl->color);
+      break;
+    case kPhaseFinal:
+      if constexpr (HALF == 5 && STRIDE == 2)
+        hipLaunchKernelGGL((k_phase_final<HALF, STRIDE, FMT, TW, TH>), grid, block, 0, s, a, pl->st, pl->color);
+      break;
     default: break;
   }
 }
@@ -735,17 +923,17 @@ void LaunchTile(KernelId id, PlaneArgs a, hipStream_t s, float f0, int i0, int n
 template <int HALF, int STRIDE, int FMT>
 bool LaunchT(int schedule, KernelId id, const PlaneArgs& a, hipStream_t s, float f0, int i0, int n,
              const float* hyps, float* cost, int* nvalid, float* view_ncc, const BatchLaunch* bl,
-             int cap) {
+             int cap, const PhaseLaunch* pl) {
   // the 11x11 stride-2 window is compiled for every tile schedule, the others
   // for the default one
   constexpr bool kAllSchedules = (HALF == 5 && STRIDE == 2);
   switch (schedule) {
-    case 0: LaunchTile<HALF, STRIDE, FMT, 64, 8>(id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cap); return true;
+    case 0: LaunchTile<HALF, STRIDE, FMT, 64, 8>(id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cap, pl); return true;
     case 1:
-      if constexpr (kAllSchedules) { LaunchTile<HALF, STRIDE, FMT, 32, 16>(id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cap); return true; }
+      if constexpr (kAllSchedules) { LaunchTile<HALF, STRIDE, FMT, 32, 16>(id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cap, pl); return true; }
       return false;
     case 2:
-      if constexpr (kAllSchedules) { LaunchTile<HALF, STRIDE, FMT, 128, 4>(id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cap); return true; }
+      if constexpr (kAllSchedules) { LaunchTile<HALF, STRIDE, FMT, 128, 4>(id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cap, pl); return true; }
       return false;
     default: return false;
   }
@@ -754,16 +942,16 @@ bool LaunchT(int schedule, KernelId id, const PlaneArgs& a, hipStream_t s, float
 template <int HALF, int STRIDE>
 bool LaunchFmt(int fmt, int schedule, KernelId id, const PlaneArgs& a, hipStream_t s, float f0, int i0,
                int n, const float* hyps, float* cost, int* nvalid, float* view_ncc, const BatchLaunch* bl,
-               int cap) {
+               int cap, const PhaseLaunch* pl) {
   // FP16 / BF16 storage is compiled for the 11x11 stride-2 and 7x7 windows
   constexpr bool kAllFormats = (HALF == 5 && STRIDE == 2) || (HALF == 3 && STRIDE == 1);
   switch (fmt) {
-    case RECON_MVS_PIXEL_F32: return LaunchT<HALF, STRIDE, RECON_MVS_PIXEL_F32>(schedule, id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cap);
+    case RECON_MVS_PIXEL_F32: return LaunchT<HALF, STRIDE, RECON_MVS_PIXEL_F32>(schedule, id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cap, pl);
     case RECON_MVS_PIXEL_F16:
-      if constexpr (kAllFormats) return LaunchT<HALF, STRIDE, RECON_MVS_PIXEL_F16>(schedule, id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cap);
+      if constexpr (kAllFormats) return LaunchT<HALF, STRIDE, RECON_MVS_PIXEL_F16>(schedule, id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cap, pl);
       return false;
     case RECON_MVS_PIXEL_BF16:
-      if constexpr (kAllFormats) return LaunchT<HALF, STRIDE, RECON_MVS_PIXEL_BF16>(schedule, id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cap);
+      if constexpr (kAllFormats) return LaunchT<HALF, STRIDE, RECON_MVS_PIXEL_BF16>(schedule, id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cap, pl);
       return false;
     default: return false;
   }
@@ -774,20 +962,20 @@ int32_t Launch(recon_mvs_hip_context* ctx, const recon_mvs_plane_params* p, Kern
                const PlaneArgs& a, float f0 = 0.0f, int i0 = 0, int n = 0,
                const float* hyps = nullptr, float* cost = nullptr, int* nvalid = nullptr,
                float* view_ncc = nullptr, const BatchLaunch* bl = nullptr, int schedule = -1,
-               int cap = 0) {
+               int cap = 0, const PhaseLaunch* pl = nullptr) {
   const int fmt = ctx->storage_format;
   const int sch = schedule >= 0 ? schedule : p->tile_schedule;
   hipEvent_t eb, ee;
   TimerBegin(ctx, static_cast<int>(id), &eb, &ee);
   bool ok = false;
   const int ps = p->patch_size, st = p->window_stride;
-  if (ps == 11 && st == 2) ok = LaunchFmt<5, 2>(fmt, sch, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cap);
+  if (ps == 11 && st == 2) ok = LaunchFmt<5, 2>(fmt, sch, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cap, pl);
 #ifndef RMVS_BENCH_WINDOW_ONLY  // (development builds compile the 11x11 stride-2 window only)
-  else if (ps == 11 && st == 1) ok = LaunchFmt<5, 1>(fmt, sch, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cap);
-  else if (ps == 7 && st == 1) ok = LaunchFmt<3, 1>(fmt, sch, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cap);
-  else if (ps == 5 && st == 1) ok = LaunchFmt<2, 1>(fmt, sch, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cap);
-  else if (ps == 7 && st == 2) ok = LaunchFmt<3, 2>(fmt, sch, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cap);
-  else if (ps == 9 && st == 2) ok = LaunchFmt<4, 2>(fmt, sch, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cap);
+  else if (ps == 11 && st == 1) ok = LaunchFmt<5, 1>(fmt, sch, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cap, pl);
+  else if (ps == 7 && st == 1) ok = LaunchFmt<3, 1>(fmt, sch, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cap, pl);
+  else if (ps == 5 && st == 1) ok = LaunchFmt<2, 1>(fmt, sch, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cap, pl);
+  else if (ps == 7 && st == 2) ok = LaunchFmt<3, 2>(fmt, sch, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cap, pl);
+  else if (ps == 9 && st == 2) ok = LaunchFmt<4, 2>(fmt, sch, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cap, pl);
 #endif
   if (!ok)
     return Fail(ctx, RECON_MVS_ERR_NOT_IMPLEMENTED,
@@ -795,6 +983,125 @@ int32_t Launch(recon_mvs_hip_context* ctx, const recon_mvs_plane_params* p, Kern
   TimerEnd(ctx, static_cast<int>(id), eb, ee);
   RMVS_LAUNCH_CHECK(ctx);
   return RECON_MVS_OK;
+}
+
+// Direction of the epipolar lines of source S in reference R at the image
+// centre (unit vector; false when the epipole is at the centre).  The lines
+// run through the epipole e = K_r (R_r C_s + t_r); the direction is defined
+// also when e is at infinity.
+bool EpipolarDirection(const ResidentView& R, const ResidentView& S, double* ux, double* uy) {
+  double C[3], P[3];
+  for (int i = 0; i < 3; ++i)  // source centre C = -R_s^T t_s
+    C[i] = -(S.cam.R[i] * S.cam.t[0] + S.cam.R[3 + i] * S.cam.t[1] + S.cam.R[6 + i] * S.cam.t[2]);
+  for (int i = 0; i < 3; ++i)
+    P[i] = R.cam.R[3 * i] * C[0] + R.cam.R[3 * i + 1] * C[1] + R.cam.R[3 * i + 2] * C[2] + R.cam.t[i];
+  const double ex = R.cam.fx * P[0] + R.cam.cx * P[2], ey = R.cam.fy * P[1] + R.cam.cy * P[2];
+  const double dx = ex - 0.5 * R.w * P[2], dy = ey - 0.5 * R.h * P[2];
+  const double len = std::hypot(dx, dy);
+  if (!(len > 1e-12)) return false;
+  *ux = dx / len;
+  *uy = dy / len;
+  return true;
+}
+
+// Tile schedule of a view-phased launch: tiles long along the epipolar lines
+// (128x4 for lines near the rows, 32x16 near the columns, 64x8 between), so a
+// tile's unconverged hypotheses sample a narrow band of the source.
+int PhaseSchedule(const ResidentView& R, const ResidentView& S) {
+  double ux = 1.0, uy = 0.0;
+  if (!EpipolarDirection(R, S, &ux, &uy)) return 0;
+  if (std::fabs(ux) > 2.0 * std::fabs(uy)) return 2;
+  if (std::fabs(uy) > 2.0 * std::fabs(ux)) return 1;
+  return 0;
+}
+
+// Tile order of a view-phased launch (ty << 16 | tx per entry): strips of
+// tiles along the epipolar lines, taken strip by strip.  Dealt to the XCDs in
+// runs (k_phase_view), each XCD's tiles in flight then form a stretch of a
+// strip whose source footprint is one band of the source image.
+std::vector<int> PhaseTileOrder(const ResidentView& R, const ResidentView& S, int tw, int th) {
+  const int tiles_x = (R.w + tw - 1) / tw, tiles_y = (R.h + th - 1) / th;
+  std::vector<int> order;
+  order.reserve(static_cast<size_t>(tiles_x) * tiles_y);
+  double ux, uy;
+  if (!EpipolarDirection(R, S, &ux, &uy)) {
+    // epipole at the image centre (motion along the axis): down the columns
+    for (int t = 0; t < tiles_x * tiles_y; ++t) order.push_back(((t % tiles_y) << 16) | (t / tiles_y));
+    return order;
+  }
+  const double nx = -uy, ny = ux, mx = 0.5 * R.w, my = 0.5 * R.h;
+  const double thick = std::fabs(nx) * tw + std::fabs(ny) * th;  // a tile's extent across the lines
+  std::vector<std::pair<std::pair<long long, double>, int>> keyed;
+  keyed.reserve(order.capacity());
+  for (int ty = 0; ty < tiles_y; ++ty)
+    for (int tx = 0; tx < tiles_x; ++tx) {
+      const double px = (tx + 0.5) * tw - mx, py = (ty + 0.5) * th - my;
+      const long long strip = static_cast<long long>(std::floor((nx * px + ny * py) / thick));
+      keyed.push_back({{strip, ux * px + uy * py}, (ty << 16) | tx});
+    }
+  std::sort(keyed.begin(), keyed.end());
+  for (const auto& k : keyed) order.push_back(k.second);
+  return order;
+}
+
+// One view-phased half-sweep (k_phase_prep, one k_phase_view per source,
+// k_phase_final) of the view `a` describes; timed as one kSweep.
+int32_t PhasedHalfSweep(recon_mvs_hip_context* ctx, const recon_mvs_plane_params* p, PlaneArgs a, int ref,
+                        const int32_t* srcs, int sched, int iter, int color) {
+  std::string key = std::to_string(ref) + "/" + std::to_string(a.w) + "x" + std::to_string(a.h) + ":";
+  for (int v = 0; v < a.nsrc; ++v) key += std::to_string(srcs[v]) + ",";
+  int*& orders = ctx->phase_orders[key];
+  std::vector<int> vsched(a.nsrc), voff(a.nsrc);
+  {
+    std::vector<int> host;
+    for (int v = 0; v < a.nsrc; ++v) {
+      vsched[v] = PhaseSchedule(*ctx->views[ref], *ctx->views[srcs[v]]);
+      int tw, th;
+      ScheduleTile(vsched[v], &tw, &th);
+      voff[v] = static_cast<int>(host.size());
+      if (!orders) {
+        const std::vector<int> o = PhaseTileOrder(*ctx->views[ref], *ctx->views[srcs[v]], tw, th);
+        host.insert(host.end(), o.begin(), o.end());
+      } else {
+        host.resize(host.size() + static_cast<size_t>((a.w + tw - 1) / tw) * ((a.h + th - 1) / th));
+      }
+    }
+    if (!orders) {
+      if (hipMalloc(&orders, host.size() * sizeof(int)) != hipSuccess) {
+        orders = nullptr;
+        return Fail(ctx, RECON_MVS_ERR_OUT_OF_MEMORY, "phase order allocation failed");
+      }
+      RMVS_HIP(ctx, hipMemcpy(orders, host.data(), host.size() * sizeof(int), hipMemcpyHostToDevice));
+    }
+  }
+  PhaseLaunch pl{};
+  PhaseState& st = pl.st;
+  st.cw = (a.w + 1) / 2;
+  st.n = a.h * st.cw;
+  st.K = a.top_k < a.nsrc ? a.top_k : 0;
+  const size_t n8 = static_cast<size_t>(st.n) * 8;
+  char* base = static_cast<char*>(Ensure(ctx, kSlotPhase, n8 * (sizeof(float4) + sizeof(float) * (st.K + 1) + sizeof(int))));
+  if (!base) return Fail(ctx, RECON_MVS_ERR_OUT_OF_MEMORY, "phase state allocation failed");
+  st.cand = reinterpret_cast<float4*>(base);
+  st.topk = reinterpret_cast<float*>(base + n8 * sizeof(float4));
+  st.csum = st.topk + n8 * st.K;
+  st.nvalid = reinterpret_cast<int*>(st.csum + n8);
+  pl.color = color;
+  pl.iter = iter;
+  a.color = color;
+  a.iter = iter;
+  hipEvent_t eb, ee;
+  TimerBegin(ctx, kSweep, &eb, &ee);
+  int32_t r = Launch(ctx, p, kPhasePrep, a, 0.0f, 0, 0, nullptr, nullptr, nullptr, nullptr, nullptr, sched, 0, &pl);
+  for (int v = 0; v < a.nsrc && r == RECON_MVS_OK; ++v) {
+    pl.view = v;
+    pl.order = orders + voff[v];
+    r = Launch(ctx, p, kPhaseView, a, 0.0f, 0, 0, nullptr, nullptr, nullptr, nullptr, nullptr, vsched[v], 0, &pl);
+  }
+  if (r == RECON_MVS_OK)
+    r = Launch(ctx, p, kPhaseFinal, a, 0.0f, 0, 0, nullptr, nullptr, nullptr, nullptr, nullptr, sched, 0, &pl);
+  TimerEnd(ctx, kSweep, eb, ee);
+  return r;
 }
 
 double MeanBaselineViews(recon_mvs_hip_context* ctx, int ref, int nsrc, const int32_t* srcs) {
@@ -997,6 +1304,12 @@ struct recon_mvs_hip_batch {
 };
 
 void DestroyPlaneState(recon_mvs_hip_context* ctx) {
+  if (!ctx->phase_orders.empty()) {
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto& kv : ctx->phase_orders)
+      if (kv.second) (void)hipFree(kv.second);
+    ctx->phase_orders.clear();
+  }
   for (auto* s : ctx->states) FreeState(s);
   ctx->states.clear();
   for (auto* v : ctx->views) FreeView(v);
@@ -1076,12 +1389,23 @@ int32_t RunPlanePatchmatch(recon_mvs_hip_context* ctx, int32_t ref_view, int32_t
   st = Launch(ctx, params, kInit, a, 0.0f, 0, 0, nullptr, nullptr, nullptr, nullptr, nullptr, sched,
               caps[0]);
   if (st != RECON_MVS_OK) return st;
+  // view-phased leading iterations (the 11x11 stride-2 window only);
+  // automatic: the first iteration for FP32 images (FP16 / BF16 images are
+  // half the bytes, and their fused sweeps already keep to the L2)
+  int phased = 0;
+  if (params->patch_size == 11 && params->window_stride == 2)
+    phased = params->view_phased_iterations >= 0
+                 ? params->view_phased_iterations
+                 : (ctx->storage_format == RECON_MVS_PIXEL_F32 ? 1 : 0);
   for (int it = 0; it < params->iterations; ++it) {
     for (int color = 0; color < 2; ++color) {
       a.iter = it;
       a.color = color;
-      st = Launch(ctx, params, kSweep, a, 0.0f, 0, 0, nullptr, nullptr, nullptr, nullptr, nullptr, sched,
-                  caps[1 + 2 * it + color]);
+      if (it < phased)
+        st = PhasedHalfSweep(ctx, params, a, ref_view, src_views, sched, it, color);
+      else
+        st = Launch(ctx, params, kSweep, a, 0.0f, 0, 0, nullptr, nullptr, nullptr, nullptr, nullptr, sched,
+                    caps[1 + 2 * it + color]);
       if (st != RECON_MVS_OK) return st;
     }
   }
@@ -1110,6 +1434,7 @@ RECON_MVS_API void recon_mvs_plane_params_default(recon_mvs_plane_params* p) {
   p->value_scale = 1.0;
   p->occupancy_cap = 0;
   p->capped_iterations = 1;
+  p->view_phased_iterations = -1;
 }
 
 RECON_MVS_API int32_t recon_mvs_hip_set_views(recon_mvs_hip_context* ctx, int32_t n,

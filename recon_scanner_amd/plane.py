@@ -30,7 +30,8 @@ class PlaneParams(C.Structure):
                 ("min_depth_m", C.c_double), ("max_depth_m", C.c_double),
                 ("depth_perturb", C.c_double), ("normal_perturb", C.c_double),
                 ("photo_threshold", C.c_double), ("value_scale", C.c_double),
-                ("occupancy_cap", C.c_int32), ("capped_iterations", C.c_int32)]
+                ("occupancy_cap", C.c_int32), ("capped_iterations", C.c_int32),
+                ("view_phased_iterations", C.c_int32)]
 
     def copy(self, **kw):
         p = PlaneParams()
