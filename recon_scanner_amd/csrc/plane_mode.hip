@@ -172,7 +172,28 @@ __device__ __forceinline__ void SweepTile(const PlaneArgs& a, RefTile<HALF, FMT,
   const float4 base = cur;
   int src_x[4], src_y[4];
   PropagationSources(a, x, y, src_x, src_y);
+  // propagated candidates equal to an earlier one (bit c): they would score
+  // the same cost again by the same operations, which lost or was taken
+  int dup_mask = 0;
+  {
+    float4 pc[4];
+    bool pok[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      Hyp h{0.0f, 0.0f, 0.0f, 0.0f};
+      pok[c] = SweepCandidate(a, c, ray, key, base, dscale, nscale, src_x, src_y, &h);
+      pc[c] = make_float4(h.d, h.nx, h.ny, h.nz);
+    }
+#pragma unroll
+    for (int c = 1; c < 4; ++c)
+#pragma unroll
+      for (int k = 0; k < c; ++k)
+        if (pok[c] && pok[k] && pc[c].x == pc[k].x && pc[c].y == pc[k].y && pc[c].z == pc[k].z &&
+            pc[c].w == pc[k].w)
+          dup_mask |= 1 << c;
+  }
   for (int c = 0; c < 8; ++c) {
+    if ((dup_mask >> c) & 1) continue;
     Hyp h;
     if (!SweepCandidate(a, c, ray, key, base, dscale, nscale, src_x, src_y, &h)) continue;
     int nv = 0;
@@ -874,12 +895,28 @@ int BatchGrid(int cap) {
   return per_cu * std::max(1, cus);
 }
 
+// Static LDS of a kernel as compiled (the compiler may place private arrays
+// in LDS too); on failure a size that disables the padding.
+inline size_t StaticLds(const void* kernel) {
+  hipFuncAttributes at;
+  if (hipFuncGetAttributes(&at, kernel) != hipSuccess) return 1u << 30;
+  return at.sharedSizeBytes;
+}
+
 template <int HALF, int STRIDE, int FMT, int TW, int TH>
 void LaunchTile(KernelId id, PlaneArgs a, hipStream_t s, float f0, int i0, int n, const float* hyps,
                 float* cost, int* nvalid, float* view_ncc, const BatchLaunch* bl, int cap,
                 const PhaseLaunch* pl) {
-  constexpr size_t kTileLds = sizeof(RefTile<HALF, FMT, TW, TH>) + 64;
-  const size_t pad = CapBytes(cap, kTileLds);
+  // padding for the occupancy cap, from each kernel's own static LDS
+  size_t pad = 0;
+  if (cap > 0) {
+    static const size_t lds_sweep = StaticLds(reinterpret_cast<const void*>(k_plane_sweep<HALF, STRIDE, FMT, TW, TH>));
+    static const size_t lds_init = StaticLds(reinterpret_cast<const void*>(k_plane_init<HALF, STRIDE, FMT, TW, TH>));
+    static const size_t lds_batch = StaticLds(reinterpret_cast<const void*>(k_plane_batch<HALF, STRIDE, FMT, TW, TH>));
+    if (id == kSweep) pad = CapBytes(cap, lds_sweep);
+    else if (id == kInit) pad = CapBytes(cap, lds_init);
+    else if (id == kBatch) pad = CapBytes(cap, lds_batch);
+  }
   if (id == kBatch) {
     hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, s, bl->counter);
     hipLaunchKernelGGL((k_plane_batch<HALF, STRIDE, FMT, TW, TH>), dim3(BatchGrid<HALF, STRIDE, FMT, TW, TH>(cap)),
