@@ -309,7 +309,8 @@ def main():
     value = units * args.steps / dt / 1e9
     S = 36
     hve = nv * P * V * (1 + HYPS_PER_ITER * args.iterations)
-    # roofline of the sweep kernel: algorithmic FLOPs per launch / mean duration
+    # roofline of the half-sweep (one k_plane_sweep launch, or the launches of a
+    # view-phased half-sweep, timed together): algorithmic FLOPs / mean duration
     sweep_ms, sweep_n = tim["sweep"]
     flop_launch = (P / 2) * HYPS_PER_ITER * V * S * FLOP_PER_SAMPLE
     achieved = flop_launch / (sweep_ms / sweep_n * 1e-3) / 1e12 if sweep_n else None
@@ -361,7 +362,9 @@ def main():
         "kept_pixels": kept,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": (achieved / PEAK_FP32_TFLOPS) if achieved else None,
-                     "traffic": traffic, "kernel": "k_plane_sweep",
+                     "traffic": traffic,
+                     "kernel": "half-sweep: k_plane_sweep (k_phase_prep + k_phase_view per source"
+                               " + k_phase_final in view-phased iterations)",
                      "mean_launch_ms": sweep_ms / sweep_n if sweep_n else None,
                      "flop_per_launch": flop_launch},
         "cpu_baseline": cpu,
