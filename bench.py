@@ -35,12 +35,14 @@ HYPS_PER_ITER = 8           # 4 propagated + 3 perturbed + 1 random
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", default="depthmaps",
-                    choices=["depthmaps", "batch", "adjoint", "copy"],
+                    choices=["depthmaps", "batch", "adjoint", "reference", "copy"],
                     help="depthmaps: the 9-view 1920x1440 job (default); batch: 48 references "
                          "of uneven size in one captured batch, FP16 images, against the same "
                          "jobs run one by one in FP32; adjoint: cost + gradients of a 960x720 "
-                         "depth map against 6 misaligned sources, and the 10-step polish; copy: "
-                         "a plain 1 GiB device copy (calibrates rocprofv3 byte counters)")
+                         "depth map against 6 misaligned sources, and the 10-step polish; "
+                         "reference: reference-mode PatchMatchStereo + RefineSubpixel of one "
+                         "320x240 view against the reference's CPU code on the same problem; "
+                         "copy: a plain 1 GiB device copy (calibrates rocprofv3 byte counters)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
@@ -229,6 +231,71 @@ def run_adjoint(args):
     print(json.dumps(out))
 
 
+def run_reference(args):
+    """The drop-in path: reference-mode PatchMatchStereo + RefineSubpixel (the
+    reference's algorithm, bit-exact) on one view of the sphere + planes scene
+    against its 8 sources, on the GPU; the reference's own CPU code on the same
+    problem (same config and num_threads, whose row-block RNG streams the GPU
+    reproduces) as the baseline, its outputs checked bit for bit."""
+    import oracle
+    from recon_scanner_amd import mvs, scenes
+    # the CPU reference takes ~30 s for one 320x240 view on 16 cores
+    w, h = (args.width, args.height) if args.width != 1920 else (320, 240)
+    f = 2500.0 * w / 1920.0
+    sc = scenes.sphere_planes(width=w, height=h, focal=f, n_views=9, supersample=2)
+    u8 = [sc.u8(i) for i in range(sc.num_views)]
+    cores = len(os.sched_getaffinity(0))
+    cfg = mvs.MvsConfig(patch_size=11, iterations=args.iterations, num_threads=cores,
+                        min_depth_m=sc.min_depth, max_depth_m=sc.max_depth, num_views=8)
+    srcs = list(range(1, 9))
+    poses = [mvs.CameraPose.from_camera(c) for c in sc.cams]
+    intrs = [mvs.CameraIntrinsics.from_camera(c) for c in sc.cams]
+
+    def gpu_once():
+        dm = mvs.DepthMap()
+        filled = mvs.PatchMatchStereo(u8[0], w, h, [u8[i] for i in srcs], [w] * 8, [h] * 8, poses[0],
+                                      poses, intrs[0], intrs, srcs, cfg, None, dm)
+        refined = mvs.RefineSubpixel(u8[0], w, h, [u8[i] for i in srcs], [w] * 8, [h] * 8, poses[0],
+                                     poses, intrs[0], intrs, srcs, cfg, dm)
+        return filled, refined, dm
+
+    for _ in range(args.warmup):
+        gpu_once()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        filled, refined, dm = gpu_once()
+    dt = time.perf_counter() - t
+    rec = base_record(args, dt, f"reference-mode PatchMatchStereo + RefineSubpixel, {w}x{h}, "
+                      f"8 sources, 11x11, {args.iterations} iterations x 4 directions, "
+                      f"num_threads={cores}", width=w, height=h, iterations=args.iterations)
+    units = w * h * 8 * args.iterations
+    rec.update(metric="reference-mode PatchMatchStereo px*view*iter/s (bit-exact with the reference)",
+               value=units * args.steps / dt / 1e9, unit="Gpix*view*iter/s", dtype="fp64")
+    truth = sc.depths[0]
+    d = dm.depth.reshape(h, w)
+    m = np.isfinite(d) & (truth > 0)
+    rec["median_depth_error_mm"] = float(np.median(np.abs(d[m] - truth[m]))) * 1000.0
+    if args.cpu_seconds > 0 and oracle.reference_available():
+        c0 = time.perf_counter()
+        rf, rd, rs, rc = oracle.ref_patchmatch(u8[0], [u8[i] for i in srcs], sc.cams[0], sc.cams,
+                                               sc.cams, srcs, cfg)
+        rn, rdr, rcr = oracle.ref_refine(u8[0], [u8[i] for i in srcs], sc.cams[0], sc.cams, sc.cams,
+                                         srcs, cfg, rd, rs, rc)
+        cdt = time.perf_counter() - c0
+        exact = (rf == filled and rn == refined and
+                 dm.depth.reshape(h, w).tobytes() == rdr.tobytes() and
+                 dm.support_count.reshape(h, w).tobytes() == rs.tobytes() and
+                 dm.confidence.reshape(h, w).tobytes() == rcr.tobytes())
+        rec["cpu_baseline"] = {"value": units / cdt / 1e9, "unit": "Gpix*view*iter/s", "cores": cores,
+                               "kind": "reference",
+                               "sample": f"the same problem, reference PatchMatchStereo + "
+                                         f"RefineSubpixel compiled from its sources; {cdt:.1f} s",
+                               "seconds": cdt}
+        rec["bit_exact_with_reference"] = bool(exact)
+        rec["speedup_vs_reference_cpu"] = cdt / (dt / args.steps)
+    print(json.dumps(rec))
+
+
 def run_copy(args):
     """A plain device-to-device copy of 1 GiB, `steps` times: known bytes for
     calibrating FETCH_SIZE / WRITE_SIZE counters."""
@@ -260,6 +327,8 @@ def main():
         return run_batch(args)
     if args.workload == "adjoint":
         return run_adjoint(args)
+    if args.workload == "reference":
+        return run_reference(args)
     if args.workload == "copy":
         return run_copy(args)
     args.storage = args.storage or "f32"
