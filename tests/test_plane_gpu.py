@@ -326,3 +326,15 @@ def test_view_phased_sweeps_bit_identical(storage, top_k):
             assert np.array_equal(o[k], outs[0][k], equal_nan=True), k
     fin = np.isfinite(outs[0]["depth"])
     assert fin.mean() > 0.9
+    # seeded init (phased init reproduces the seeding too)
+    rng = np.random.default_rng(2)
+    h, w = sc.depths[0].shape
+    us, vs = rng.uniform(0, w, 400), rng.uniform(0, h, 400)
+    t = sc.depths[0][vs.astype(int), us.astype(int)]
+    seeds = np.stack([us, vs, t], 1)[t > 0]
+    seeded = []
+    for phased in (0, 1):
+        pm.patchmatch(0, srcs, p.copy(view_phased_iterations=phased, iterations=1), seed_points=seeds)
+        seeded.append(pm.depth_map(0))
+    for k in ("depth", "normals", "cost", "support"):
+        assert np.array_equal(seeded[1][k], seeded[0][k], equal_nan=True), k
