@@ -32,11 +32,18 @@ struct RefProblem {
   int nsrc;
   int patch;
   const uint8_t* src[kMaxSrc];
+  const uint32_t* srcq[kMaxSrc];  // source k as 4-tap quads (k_make_quads)
   int sw[kMaxSrc];
   int sh[kMaxSrc];
   const recon_mvs_camera* cams;  // [0] reference, [1 + k] source k
   double min_depth, max_depth, photo_threshold;
 };
+
+// Pixel kernels run on TW x TH tiles: the reference patch samples of a tile
+// (a HALF-pixel halo around it) are computed once into LDS.
+constexpr int kTileW = 32;
+constexpr int kTileH = 4;
+constexpr int kTileThreads = kTileW * kTileH;
 
 // SampleBilinear (patchmatch_stereo.cpp:78-100): integer coordinates are pixel
 // centres, out-of-range samples are rejected, not clamped.
@@ -60,6 +67,29 @@ __device__ __forceinline__ bool SampleU8(const uint8_t* __restrict__ img, int w,
   return true;
 }
 
+// SampleU8 on a quad image: texel (y, x) packs the four pixels a sample with
+// floor (x, y) reads, p00 | p10 << 8 | p01 << 16 | p11 << 24, the clamps of x1
+// and y1 applied when the quads were built.  Same operations on the same
+// values as SampleU8, one 4-byte load instead of four 1-byte loads.
+__device__ __forceinline__ bool SampleQuad(const uint32_t* __restrict__ q, int w, int h, double u,
+                                           double v, double* out) {
+  if (u < 0.0 || v < 0.0 || u > static_cast<double>(w - 1) || v > static_cast<double>(h - 1))
+    return false;
+  const int x0 = static_cast<int>(floor(u));
+  const int y0 = static_cast<int>(floor(v));
+  const double fx = u - x0;
+  const double fy = v - y0;
+  const uint32_t t = q[y0 * w + x0];
+  const double p00 = static_cast<double>(t & 0xffu);
+  const double p10 = static_cast<double>((t >> 8) & 0xffu);
+  const double p01 = static_cast<double>((t >> 16) & 0xffu);
+  const double p11 = static_cast<double>(t >> 24);
+  const double top = p00 * (1.0 - fx) + p10 * fx;
+  const double bot = p01 * (1.0 - fx) + p11 * fx;
+  *out = top * (1.0 - fy) + bot * fy;
+  return true;
+}
+
 struct Ncc {
   bool valid;
   bool weak;
@@ -67,25 +97,66 @@ struct Ncc {
   double ncc;
 };
 
-__device__ Ncc NccU8(const uint8_t* __restrict__ a, int aw, int ah, const uint8_t* __restrict__ b,
-                     int bw, int bh, double ru, double rv, double su, double sv, int patch) {
-  Ncc r{false, false, 0, 0.0};
-  const int half = patch / 2;
+// Statistics of the sample pairs, in the reference's scan order (dy outer,
+// dx inner) and accumulation order (patchmatch_stereo.cpp:231-250).
+struct NccSums {
   double s_r = 0.0, s_s = 0.0, s_r2 = 0.0, s_s2 = 0.0, s_x = 0.0;
   int valid = 0;
+  __device__ __forceinline__ void add(double vr, double vs) {
+    s_r += vr;
+    s_s += vs;
+    s_r2 += vr * vr;
+    s_s2 += vs * vs;
+    s_x += vr * vs;
+    ++valid;
+  }
+};
+
+__device__ Ncc FinishNcc(const NccSums& a, int patch);
+
+__device__ Ncc NccU8(const uint8_t* __restrict__ a, int aw, int ah, const uint8_t* __restrict__ b,
+                     int bw, int bh, double ru, double rv, double su, double sv, int patch) {
+  const int half = patch / 2;
+  NccSums acc;
   for (int dy = -half; dy <= half; ++dy) {
     for (int dx = -half; dx <= half; ++dx) {
       double vr = 0.0, vs = 0.0;
       if (!SampleU8(a, aw, ah, ru + dx, rv + dy, &vr)) continue;
       if (!SampleU8(b, bw, bh, su + dx, sv + dy, &vs)) continue;
-      s_r += vr;
-      s_s += vs;
-      s_r2 += vr * vr;
-      s_s2 += vs * vs;
-      s_x += vr * vs;
-      ++valid;
+      acc.add(vr, vs);
     }
   }
+  return FinishNcc(acc, patch);
+}
+
+// NccU8 for a reference patch centred on a pixel centre, its samples read
+// from the LDS tile of RefTile (row `trow` of the tile, column `tcol`
+// holding dx = -half; NaN = outside the reference image) and the source's
+// from its quads.  A sample pair counts when both samples are inside, as in
+// NccU8; the sampling has no side effects, so the order of the two tests does
+// not matter.
+__device__ Ncc NccTile(const double* __restrict__ tile, int tw, int trow, int tcol,
+                       const uint32_t* __restrict__ q, int bw, int bh, double su, double sv,
+                       int patch) {
+  const int half = patch / 2;
+  NccSums acc;
+  for (int dy = -half; dy <= half; ++dy) {
+    const double* row = tile + (trow + dy + half) * tw + tcol;
+    for (int dx = -half; dx <= half; ++dx) {
+      const double vr = row[dx + half];
+      if (vr != vr) continue;
+      double vs = 0.0;
+      if (!SampleQuad(q, bw, bh, su + dx, sv + dy, &vs)) continue;
+      acc.add(vr, vs);
+    }
+  }
+  return FinishNcc(acc, patch);
+}
+
+__device__ Ncc FinishNcc(const NccSums& a, int patch) {
+  Ncc r{false, false, 0, 0.0};
+  const int valid = a.valid;
+  const double s_r = a.s_r, s_s = a.s_s, s_r2 = a.s_r2, s_s2 = a.s_s2, s_x = a.s_x;
   if (valid < patch * patch / 2) return r;
   const double n = static_cast<double>(valid);
   const double mr = s_r / n;
@@ -108,8 +179,11 @@ struct Cost {
   int weak, valid_views;
 };
 
-// EvaluateCandidate with the reference ray of (u, v) precomputed by the caller.
-__device__ Cost Evaluate(const RefProblem& p, double u, double v, const double* ray, double inv) {
+// EvaluateCandidate with the reference ray precomputed by the caller; `ncc(k,
+// su, sv)` is the NCC of the reference patch against source k at (su, sv).
+template <class NccFn>
+__device__ __forceinline__ Cost EvaluateWith(const RefProblem& p, const double* ray, double inv,
+                                             NccFn ncc) {
   Cost r{1e30, 0.0, 0.0, 0, 0};
   if (inv <= 1e-9) return r;
   const double depth = 1.0 / inv;
@@ -124,7 +198,7 @@ __device__ Cost Evaluate(const RefProblem& p, double u, double v, const double* 
     if (ps[2] <= 1e-6) continue;
     double su = 0.0, sv = 0.0;
     if (!Project(sc, ps, &su, &sv)) continue;
-    const Ncc n = NccU8(p.ref, p.rw, p.rh, p.src[k], p.sw[k], p.sh[k], u, v, su, sv, p.patch);
+    const Ncc n = ncc(k, su, sv);
     if (!n.valid) continue;
     ++r.valid_views;
     if (n.weak) ++r.weak;
@@ -136,6 +210,76 @@ __device__ Cost Evaluate(const RefProblem& p, double u, double v, const double* 
   r.cost = 1.0 - mean;
   if (r.cost > 3.0 * p.photo_threshold) r.cost = 1e30;
   return r;
+}
+
+// Reference patch at any (u, v), sampled from the image.
+__device__ Cost Evaluate(const RefProblem& p, double u, double v, const double* ray, double inv) {
+  return EvaluateWith(p, ray, inv, [&](int k, double su, double sv) {
+    return NccU8(p.ref, p.rw, p.rh, p.src[k], p.sw[k], p.sh[k], u, v, su, sv, p.patch);
+  });
+}
+
+// The reference tile of a TW x TH pixel tile: the reference samples at
+// (X + 0.5, Y + 0.5) for X in [x0 - half, x0 + TW + half), Y likewise, i.e.
+// the samples (x + 0.5) + dx, (y + 0.5) + dy of every patch of the tile (equal
+// in FP64: all terms are small integers and halves), NaN where SampleU8
+// rejects them.  Dynamic LDS of RefTileBytes(patch).
+struct RefTile {
+  const double* t;
+  int tw;        // row length: TW + 2 * half
+  int lx, ly;    // this thread's pixel in the tile
+  int x, y;      // and in the image
+  bool active;   // inside the image
+};
+
+inline size_t RefTileBytes(int patch) {
+  const int half = patch / 2;
+  return sizeof(double) * (kTileW + 2 * half) * (kTileH + 2 * half);
+}
+
+// XCD-aware tile of this workgroup: consecutive tiles go to one XCD (the
+// hardware deals workgroups round-robin over the 8 XCDs), so an XCD's L2
+// holds the source rows of a compact region.  A bijection on [0, gridDim.x).
+__device__ __forceinline__ int XcdTile() {
+  const int b = blockIdx.x, n = gridDim.x;
+  const int q = n >> 3, r = n & 7, j = b & 7;
+  return j * q + min(j, r) + (b >> 3);
+}
+
+__device__ RefTile LoadRefTile(const RefProblem& p, double* lds) {
+  const int half = p.patch / 2;
+  const int tiles_x = (p.rw + kTileW - 1) / kTileW;
+  const int t = XcdTile();
+  const int x0 = (t % tiles_x) * kTileW, y0 = (t / tiles_x) * kTileH;
+  const int tw = kTileW + 2 * half, th = kTileH + 2 * half;
+  for (int i = threadIdx.x; i < tw * th; i += kTileThreads) {
+    const int X = x0 - half + i % tw;
+    const int Y = y0 - half + i / tw;
+    double v = 0.0;
+    lds[i] = SampleU8(p.ref, p.rw, p.rh, X + 0.5, Y + 0.5, &v) ? v : __builtin_nan("");
+  }
+  __syncthreads();
+  RefTile r;
+  r.t = lds;
+  r.tw = tw;
+  r.lx = threadIdx.x % kTileW;
+  r.ly = threadIdx.x / kTileW;
+  r.x = x0 + r.lx;
+  r.y = y0 + r.ly;
+  r.active = r.x < p.rw && r.y < p.rh;
+  return r;
+}
+
+// Reference patch of the tile pixel, sources sampled from their quads.
+__device__ __forceinline__ Cost EvaluateTile(const RefProblem& p, const RefTile& rt,
+                                             const double* ray, double inv) {
+  return EvaluateWith(p, ray, inv, [&](int k, double su, double sv) {
+    return NccTile(rt.t, rt.tw, rt.ly, rt.lx, p.srcq[k], p.sw[k], p.sh[k], su, sv, p.patch);
+  });
+}
+
+inline unsigned RefTiles(int w, int h) {
+  return static_cast<unsigned>(((w + kTileW - 1) / kTileW) * ((h + kTileH - 1) / kTileH));
 }
 
 // ---------------------------------------------------------------------------
@@ -177,15 +321,19 @@ __global__ void k_evaluate(RefProblem p, int n, const double* __restrict__ uvi,
 }
 
 // Initial cost of every pixel (patchmatch_stereo.cpp:543-581).
-__global__ void k_pm_init_cost(RefProblem p, const double* __restrict__ inv,
-                               double* __restrict__ cost, int* __restrict__ support) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= p.rw * p.rh) return;
-  const int x = idx % p.rw, y = idx / p.rw;
-  const double u = x + 0.5, v = y + 0.5;
+// One RefTile per workgroup (RefTiles(w, h) workgroups of kTileThreads).
+__global__ __launch_bounds__(kTileThreads) void k_pm_init_cost(RefProblem p,
+                                                               const double* __restrict__ inv,
+                                                               double* __restrict__ cost,
+                                                               int* __restrict__ support) {
+  extern __shared__ double lds[];
+  const RefTile rt = LoadRefTile(p, lds);
+  if (!rt.active) return;
+  const int idx = rt.y * p.rw + rt.x;
+  const double u = rt.x + 0.5, v = rt.y + 0.5;
   double ray[3];
   PixelToRay(p.cams[0], u, v, ray);
-  const Cost c = Evaluate(p, u, v, ray, inv[idx]);
+  const Cost c = EvaluateTile(p, rt, ray, inv[idx]);
   cost[idx] = c.cost;
   support[idx] = c.valid_views;
 }
@@ -193,16 +341,19 @@ __global__ void k_pm_init_cost(RefProblem p, const double* __restrict__ inv,
 // One directional pass (patchmatch_stereo.cpp:655-721) for every pixel.
 // tab holds this pass's replayed draws, SoA [K + 2][P]: K random-search draws,
 // the restart test draw, the restart value draw (0 when not drawn).
-__global__ void k_pm_pass(RefProblem p, const double* __restrict__ snap_inv,
+// One RefTile per workgroup (RefTiles(w, h) workgroups of kTileThreads).
+__global__ __launch_bounds__(kTileThreads) void k_pm_pass(RefProblem p, const double* __restrict__ snap_inv,
                           const double* __restrict__ snap_cost, const int* __restrict__ snap_sup,
                           double* __restrict__ out_inv, double* __restrict__ out_cost,
                           int* __restrict__ out_sup, const uint32_t* __restrict__ tab, int K,
                           double ratio, double restart_ratio, double inv_min, double inv_max,
                           int propagate) {
+  extern __shared__ double lds[];
   const int P = p.rw * p.rh;
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= P) return;
-  const int x = idx % p.rw, y = idx / p.rw;
+  const RefTile rt = LoadRefTile(p, lds);
+  if (!rt.active) return;
+  const int x = rt.x, y = rt.y;
+  const int idx = y * p.rw + x;
   const double u = x + 0.5, v = y + 0.5;
   double ray[3];
   PixelToRay(p.cams[0], u, v, ray);
@@ -218,7 +369,7 @@ __global__ void k_pm_pass(RefProblem p, const double* __restrict__ snap_inv,
       const int nidx = ny * p.rw + nx;
       if (snap_cost[nidx] >= 1e29) continue;
       const double cand = snap_inv[nidx];
-      const Cost c = Evaluate(p, u, v, ray, cand);
+      const Cost c = EvaluateTile(p, rt, ray, cand);
       if (c.cost < best_c) {
         best_c = c.cost;
         best_i = cand;
@@ -234,7 +385,7 @@ __global__ void k_pm_pass(RefProblem p, const double* __restrict__ snap_inv,
       radius *= 0.5;
       continue;
     }
-    const Cost c = Evaluate(p, u, v, ray, cand);
+    const Cost c = EvaluateTile(p, rt, ray, cand);
     if (c.cost < best_c) {
       best_c = c.cost;
       best_i = cand;
@@ -246,7 +397,7 @@ __global__ void k_pm_pass(RefProblem p, const double* __restrict__ snap_inv,
   if (ur < restart_ratio) {
     const double uv = static_cast<double>(tab[static_cast<size_t>(K + 1) * P + idx]) / 4294967296.0;
     const double cand = inv_min + uv * (inv_max - inv_min);
-    const Cost c = Evaluate(p, u, v, ray, cand);
+    const Cost c = EvaluateTile(p, rt, ray, cand);
     if (c.cost < best_c) {
       best_c = c.cost;
       best_i = cand;
@@ -288,6 +439,22 @@ __global__ void k_pm_output(int P, const double* __restrict__ inv, const double*
   if ((threadIdx.x & 63) == 0 && m) atomicAdd(filled, static_cast<unsigned int>(__popcll(m)));
 }
 
+// Quads of an image for SampleQuad: texel (y, x) = I(y, x) | I(y, x1) << 8 |
+// I(y1, x) << 16 | I(y1, x1) << 24, x1 = min(x + 1, w - 1), y1 = min(y + 1, h - 1).
+__global__ void k_make_quads(const uint8_t* __restrict__ img, int w, int h,
+                             uint32_t* __restrict__ quad) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= static_cast<size_t>(w) * h) return;
+  const int x = static_cast<int>(i % w), y = static_cast<int>(i / w);
+  const int x1 = min(x + 1, w - 1), y1 = min(y + 1, h - 1);
+  const uint8_t* r0 = img + static_cast<size_t>(y) * w;
+  const uint8_t* r1 = img + static_cast<size_t>(y1) * w;
+  quad[static_cast<size_t>(y) * w + x] = static_cast<uint32_t>(r0[x]) |
+                                         static_cast<uint32_t>(r0[x1]) << 8 |
+                                         static_cast<uint32_t>(r1[x]) << 16 |
+                                         static_cast<uint32_t>(r1[x1]) << 24;
+}
+
 __device__ int Solve3(double A[3][3], double b[3], double x[3]) {
   double m[3][4];
   for (int i = 0; i < 3; ++i) {
@@ -320,24 +487,30 @@ __device__ int Solve3(double A[3][3], double b[3], double x[3]) {
 }
 
 // RefineSubpixel body for one pixel (patchmatch_stereo.cpp:807-976).
-__global__ void k_refine(RefProblem p, float* __restrict__ depth, float* __restrict__ conf,
-                         double inv_min, double inv_max, double inv_per_px, int fit_points,
-                         int iterations, unsigned int* __restrict__ refined) {
-  const int P = p.rw * p.rh;
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+// One RefTile per workgroup (RefTiles(w, h) workgroups of kTileThreads).
+__global__ __launch_bounds__(kTileThreads) void k_refine(RefProblem p, float* __restrict__ depth,
+                                                         float* __restrict__ conf, double inv_min,
+                                                         double inv_max, double inv_per_px,
+                                                         int fit_points, int iterations,
+                                                         unsigned int* __restrict__ refined) {
+  extern __shared__ double lds[];
+  const RefTile rt = LoadRefTile(p, lds);
   bool ok = false;
-  if (idx < P) {
+  if (rt.active) {
+    const int idx = rt.y * p.rw + rt.x;
     const float d0 = depth[idx];
-    const int x = idx % p.rw, y = idx / p.rw;
-    const double u = x + 0.5, v = y + 0.5;
+    const double u = rt.x + 0.5, v = rt.y + 0.5;
     const double inv0 = DepthToInv(d0);
     if (isfinite(d0) && !(inv0 < inv_min || inv0 > inv_max)) {
       double ray[3];
       PixelToRay(p.cams[0], u, v, ray);
       const int half_fit = fit_points / 2;
       const double fit_step = inv_per_px * 0.5;
-      double fit_cost[kMaxFitPoints];
-      for (int k = 0; k < fit_points; ++k) fit_cost[k] = 1e30;
+      // The fit costs enter only the normal equations, which are used only
+      // when every one is finite: they are summed as they come, in the
+      // reference's order, and the scan stops at the first failure (the
+      // reference evaluates the remaining ones and discards them).
+      double s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, t0 = 0, t1 = 0, t2 = 0;
       bool fit_ok = true;
       for (int k = 0; k < fit_points; ++k) {
         const double d = inv0 + (k - half_fit) * fit_step;
@@ -345,25 +518,24 @@ __global__ void k_refine(RefProblem p, float* __restrict__ depth, float* __restr
           fit_ok = false;
           break;
         }
-        fit_cost[k] = Evaluate(p, u, v, ray, d).cost;
-        if (fit_cost[k] >= 1e29) fit_ok = false;
+        const double c = EvaluateTile(p, rt, ray, d).cost;
+        if (c >= 1e29) {
+          fit_ok = false;
+          break;
+        }
+        const double xx = static_cast<double>(k - half_fit);
+        s0 += 1.0;
+        s1 += xx;
+        s2 += xx * xx;
+        s3 += xx * xx * xx;
+        s4 += xx * xx * xx * xx;
+        t0 += c;
+        t1 += xx * c;
+        t2 += xx * xx * c;
       }
       double best_d = inv0;
-      double best_c = Evaluate(p, u, v, ray, inv0).cost;
+      double best_c = EvaluateTile(p, rt, ray, inv0).cost;
       if (fit_ok) {
-        double s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, t0 = 0, t1 = 0, t2 = 0;
-        for (int k = 0; k < fit_points; ++k) {
-          const double xx = static_cast<double>(k - half_fit);
-          const double c = fit_cost[k];
-          s0 += 1.0;
-          s1 += xx;
-          s2 += xx * xx;
-          s3 += xx * xx * xx;
-          s4 += xx * xx * xx * xx;
-          t0 += c;
-          t1 += xx * c;
-          t2 += xx * xx * c;
-        }
         double A[3][3] = {{s4, s3, s2}, {s3, s2, s1}, {s2, s1, s0}};
         double b[3] = {t2, t1, t0};
         double sol[3] = {0.0, 0.0, 0.0};
@@ -374,7 +546,7 @@ __global__ void k_refine(RefProblem p, float* __restrict__ depth, float* __restr
             if (vertex >= -static_cast<double>(half_fit) && vertex <= static_cast<double>(half_fit)) {
               // The reference adds the vertex in sample-index units (line 898).
               const double cand = inv0 + vertex;
-              const double cc = Evaluate(p, u, v, ray, cand).cost;
+              const double cc = EvaluateTile(p, rt, ray, cand).cost;
               if (cc < best_c) {
                 best_c = cc;
                 best_d = cand;
@@ -391,25 +563,25 @@ __global__ void k_refine(RefProblem p, float* __restrict__ depth, float* __restr
         const double inv_phi = 0.6180339887498949;
         double c1 = hi - inv_phi * (hi - lo);
         double c2 = lo + inv_phi * (hi - lo);
-        double f1 = Evaluate(p, u, v, ray, c1).cost;
-        double f2 = Evaluate(p, u, v, ray, c2).cost;
+        double f1 = EvaluateTile(p, rt, ray, c1).cost;
+        double f2 = EvaluateTile(p, rt, ray, c2).cost;
         for (int k = 0; k < iterations; ++k) {
           if (f1 < f2) {
             hi = c2;
             c2 = c1;
             f2 = f1;
             c1 = hi - inv_phi * (hi - lo);
-            f1 = Evaluate(p, u, v, ray, c1).cost;
+            f1 = EvaluateTile(p, rt, ray, c1).cost;
           } else {
             lo = c1;
             c1 = c2;
             f1 = f2;
             c2 = lo + inv_phi * (hi - lo);
-            f2 = Evaluate(p, u, v, ray, c2).cost;
+            f2 = EvaluateTile(p, rt, ray, c2).cost;
           }
         }
         const double cm = 0.5 * (lo + hi);
-        const double fm = Evaluate(p, u, v, ray, cm).cost;
+        const double fm = EvaluateTile(p, rt, ray, cm).cost;
         if (fm < best_c) {
           best_c = fm;
           best_d = cm;
@@ -520,9 +692,12 @@ int32_t ValidateProblem(recon_mvs_hip_context* ctx, const recon_mvs_problem* pr,
   return RECON_MVS_OK;
 }
 
-// Uploads images and cameras of a problem; fills the device RefProblem.
+inline unsigned Blocks(size_t n, unsigned t) { return static_cast<unsigned>((n + t - 1) / t); }
+
+// Uploads images and cameras of a problem; fills the device RefProblem.  With
+// `quads`, also builds each source's quad image (for the tile kernels).
 int32_t Upload(recon_mvs_hip_context* ctx, const recon_mvs_problem* pr,
-               const recon_mvs_config* cfg, RefProblem* dp) {
+               const recon_mvs_config* cfg, RefProblem* dp, bool quads = false) {
   size_t total = static_cast<size_t>(pr->ref_w) * pr->ref_h;
   std::vector<size_t> off(pr->num_src);
   for (int k = 0; k < pr->num_src; ++k) {
@@ -558,14 +733,30 @@ int32_t Upload(recon_mvs_hip_context* ctx, const recon_mvs_problem* pr,
     dp->sw[k] = pr->src_w[k];
     dp->sh[k] = pr->src_h[k];
   }
+  if (quads) {
+    std::vector<size_t> qoff(pr->num_src);
+    size_t qtotal = 0;
+    for (int k = 0; k < pr->num_src; ++k) {
+      qoff[k] = qtotal;
+      qtotal += ((static_cast<size_t>(pr->src_w[k]) * pr->src_h[k] * 4 + 255) & ~static_cast<size_t>(255));
+    }
+    auto* q = static_cast<uint8_t*>(Ensure(ctx, kSlotSrcImg, qtotal));
+    if (!q) return Fail(ctx, RECON_MVS_ERR_OUT_OF_MEMORY, "device allocation failed");
+    for (int k = 0; k < pr->num_src; ++k) {
+      auto* qk = reinterpret_cast<uint32_t*>(q + qoff[k]);
+      hipLaunchKernelGGL(k_make_quads, dim3(Blocks(static_cast<size_t>(pr->src_w[k]) * pr->src_h[k], 256)),
+                         dim3(256), 0,
+                         ctx->stream, img + off[k], pr->src_w[k], pr->src_h[k], qk);
+      RMVS_LAUNCH_CHECK(ctx);
+      dp->srcq[k] = qk;
+    }
+  }
   dp->cams = cams;
   dp->min_depth = cfg->min_depth_m;
   dp->max_depth = cfg->max_depth_m;
   dp->photo_threshold = cfg->photo_threshold;
   return RECON_MVS_OK;
 }
-
-inline unsigned Blocks(size_t n, unsigned t) { return static_cast<unsigned>((n + t - 1) / t); }
 
 }  // namespace
 }  // namespace rmvs
@@ -660,7 +851,7 @@ RECON_MVS_API int32_t recon_mvs_hip_patchmatch_stereo(recon_mvs_hip_context* ctx
   if (!(inv_max > inv_min) || inv_min <= 0.0) return RECON_MVS_OK;
 
   RefProblem dp;
-  st = Upload(ctx, problem, config, &dp);
+  st = Upload(ctx, problem, config, &dp, true);
   if (st != RECON_MVS_OK) return st;
 
   // Seeding + random init on the host: one sequential stream (lines 496-539).
@@ -720,8 +911,10 @@ RECON_MVS_API int32_t recon_mvs_hip_patchmatch_stereo(recon_mvs_hip_context* ctx
     return Fail(ctx, RECON_MVS_ERR_OUT_OF_MEMORY, "device allocation failed");
 
   RMVS_HIP(ctx, hipMemcpyAsync(invA, inv0.data(), P * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-  const unsigned TB = 128;
-  hipLaunchKernelGGL(k_pm_init_cost, dim3(Blocks(P, TB)), dim3(TB), 0, ctx->stream, dp, invA, costA, supA);
+  const unsigned tiles = RefTiles(w, h);
+  const size_t tile_lds = RefTileBytes(config->patch_size);
+  hipLaunchKernelGGL(k_pm_init_cost, dim3(tiles), dim3(kTileThreads), tile_lds, ctx->stream, dp, invA,
+                     costA, supA);
   RMVS_LAUNCH_CHECK(ctx);
 
   const int threads = ResolveThreads(config->num_threads);
@@ -737,7 +930,7 @@ RECON_MVS_API int32_t recon_mvs_hip_patchmatch_stereo(recon_mvs_hip_context* ctx
       FillPassTable(host, w, h, iter, dir, threads, K, config->restart_ratio);
       RMVS_HIP(ctx, hipMemcpyAsync(tab[b], host, tab_bytes, hipMemcpyHostToDevice, ctx->stream));
       RMVS_HIP(ctx, hipEventRecord(ctx->pinned_ready[b], ctx->stream));
-      hipLaunchKernelGGL(k_pm_pass, dim3(Blocks(P, TB)), dim3(TB), 0, ctx->stream, dp, invA, costA,
+      hipLaunchKernelGGL(k_pm_pass, dim3(tiles), dim3(kTileThreads), tile_lds, ctx->stream, dp, invA, costA,
                          supA, invB, costB, supB, tab[b], K, config->random_search_radius_ratio,
                          config->restart_ratio, inv_min, inv_max, config->propagation_enabled ? 1 : 0);
       RMVS_LAUNCH_CHECK(ctx);
@@ -780,7 +973,7 @@ RECON_MVS_API int32_t recon_mvs_hip_refine_subpixel(recon_mvs_hip_context* ctx,
   const double focal = 0.5 * (problem->ref_camera.fx + problem->ref_camera.fy);
   const double inv_per_px = 1.0 / (focal * baseline);
   RefProblem dp;
-  st = Upload(ctx, problem, config, &dp);
+  st = Upload(ctx, problem, config, &dp, true);
   if (st != RECON_MVS_OK) return st;
   const size_t P = static_cast<size_t>(problem->ref_w) * problem->ref_h;
   float* d_depth = static_cast<float*>(Ensure(ctx, kSlotOutDepth, P * sizeof(float)));
@@ -790,7 +983,8 @@ RECON_MVS_API int32_t recon_mvs_hip_refine_subpixel(recon_mvs_hip_context* ctx,
   RMVS_HIP(ctx, hipMemcpyAsync(d_depth, depth, P * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
   RMVS_HIP(ctx, hipMemcpyAsync(d_conf, confidence, P * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
   RMVS_HIP(ctx, hipMemsetAsync(d_cnt, 0, sizeof(unsigned int), ctx->stream));
-  hipLaunchKernelGGL(k_refine, dim3(Blocks(P, 128)), dim3(128), 0, ctx->stream, dp, d_depth, d_conf,
+  hipLaunchKernelGGL(k_refine, dim3(RefTiles(problem->ref_w, problem->ref_h)), dim3(kTileThreads),
+                     RefTileBytes(config->patch_size), ctx->stream, dp, d_depth, d_conf,
                      inv_min, inv_max, inv_per_px, fit_points, config->subpixel_iterations, d_cnt);
   RMVS_LAUNCH_CHECK(ctx);
   unsigned int cnt = 0;

@@ -217,3 +217,37 @@ def test_reference_unit_test_scene():
     assert median <= 19.1 * 0.15 * 1.5 and p95 <= 19.1 * 0.15 * 3.0
     cloud = mvs.FuseDepthMaps([dm], [pose(cams[0])], [intr(cams[0])], cfg)
     assert cloud.size() > 1000
+
+
+@pytest.mark.parametrize("patch", [9, 4])
+def test_ragged_tiles_and_source_sizes(patch):
+    """Reference and sources off the pixel kernels' 32x4 tile grid (101x67,
+    sources cropped to other sizes: the quads of each source carry its own
+    edge clamps), an odd and an even patch, PatchMatchStereo then
+    RefineSubpixel, against the oracle bit for bit."""
+    say("ragged tiles, patch", patch)
+    sc = scenes.sphere_arc(width=101, height=67, focal=160.0)
+    u8 = [sc.u8(i) for i in range(5)]
+    srcs = [u8[1][:61, :99], u8[2][:67, :90], np.ascontiguousarray(u8[3][:50, :101]), u8[4]]
+    srcs = [np.ascontiguousarray(s) for s in srcs]
+    sw = [s.shape[1] for s in srcs]
+    sh = [s.shape[0] for s in srcs]
+    cfg = mvs.MvsConfig(patch_size=patch, iterations=2, num_threads=3, min_depth_m=0.25,
+                        max_depth_m=0.45)
+    cams = sc.cams
+    dm = mvs.DepthMap()
+    f = mvs.PatchMatchStereo(u8[0], 101, 67, srcs, sw, sh, pose(cams[0]), [pose(c) for c in cams],
+                             intr(cams[0]), [intr(c) for c in cams], [1, 2, 3, 4], cfg, None, dm)
+    of, od, os_, oc = oracle.orc_patchmatch(u8[0], srcs, cams[0], cams[1:], cfg)
+    say("filled", f, of)
+    assert f == of and f > 0
+    assert bits_equal(dm.depth.reshape(67, 101), od)
+    assert bits_equal(dm.support_count.reshape(67, 101), os_)
+    assert bits_equal(dm.confidence.reshape(67, 101), oc)
+    n = mvs.RefineSubpixel(u8[0], 101, 67, srcs, sw, sh, pose(cams[0]), [pose(c) for c in cams],
+                           intr(cams[0]), [intr(c) for c in cams], [1, 2, 3, 4], cfg, dm)
+    on, odr, ocr = oracle.orc_refine(u8[0], srcs, cams[0], cams[1:], cfg, od, oc)
+    say("refined", n, on)
+    assert n == on and n > 0
+    assert bits_equal(dm.depth.reshape(67, 101), odr)
+    assert bits_equal(dm.confidence.reshape(67, 101), ocr)
