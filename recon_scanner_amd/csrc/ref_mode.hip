@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <cstring>
 #include <thread>
+#include <type_traits>
 
 #include "mvs_common.hpp"
 
@@ -230,6 +231,8 @@ struct RefTile {
   int lx, ly;    // this thread's pixel in the tile
   int x, y;      // and in the image
   bool active;   // inside the image
+  bool full;     // every sample of the pixel's patch inside the reference
+  double rs, rs2;  // with full: the patch's sum and sum of squares, in scan order
 };
 
 inline size_t RefTileBytes(int patch) {
@@ -267,14 +270,139 @@ __device__ RefTile LoadRefTile(const RefProblem& p, double* lds) {
   r.x = x0 + r.lx;
   r.y = y0 + r.ly;
   r.active = r.x < p.rw && r.y < p.rh;
+  // The reference sums s_r and s_r2 of NccU8 run over the reference samples
+  // alone, each its own chain of additions: when every sample is valid they
+  // are the same for all sources and candidates, and summed here once, in
+  // the same order, they are bit-identical.
+  r.full = true;
+  r.rs = 0.0;
+  r.rs2 = 0.0;
+  for (int dy = 0; dy <= 2 * half; ++dy) {
+    const double* row = lds + (r.ly + dy) * tw + r.lx;
+    for (int dx = 0; dx <= 2 * half; ++dx) {
+      const double vr = row[dx];
+      if (vr != vr) {
+        r.full = false;
+        continue;
+      }
+      r.rs += vr;
+      r.rs2 += vr * vr;
+    }
+  }
   return r;
 }
 
-// Reference patch of the tile pixel, sources sampled from their quads.
+// NccTile when every sample pair is valid: the reference patch inside the
+// reference image (rt.full) and the source window inside the source.  Then
+// no sample is skipped, s_r / s_r2 are the precomputed rt.rs / rt.rs2, and
+// each source sample is SampleQuad's arithmetic without its bounds tests:
+// a non-negative coordinate truncates to its floor.  HALF = patch / 2 when
+// known at compile time (unrolled rows), -1 otherwise.
+// Byte i of a quad as a double.
+__device__ __forceinline__ double QuadTap(uint32_t t, int i) {
+  return static_cast<double>((t >> (8 * i)) & 0xffu);
+}
+
+// SampleQuad's arithmetic for a sample inside the source (no bounds tests).
+__device__ __forceinline__ double Bilinear(uint32_t t, double fx, double fy) {
+  const double top = QuadTap(t, 0) * (1.0 - fx) + QuadTap(t, 1) * fx;
+  const double bot = QuadTap(t, 2) * (1.0 - fx) + QuadTap(t, 3) * fx;
+  return top * (1.0 - fy) + bot * fy;
+}
+
+// NccTile when every sample pair is valid: the reference patch inside the
+// reference image (rt.full) and the source window inside the source.  Then
+// no sample is skipped, s_r / s_r2 are the precomputed rt.rs / rt.rs2, and
+// each source sample is SampleQuad's arithmetic without its bounds tests:
+// a non-negative coordinate truncates to its floor.  HALF = patch / 2 when
+// known at compile time, -1 otherwise.  With HALF known, a row's columns
+// (x0, fx: the same on every row) are computed once and the next row's quads
+// are loaded while the current row is summed.
+template <int HALF>
+__device__ __forceinline__ Ncc NccTileFull(const RefTile& rt, const uint32_t* __restrict__ q, int bw,
+                                           int bh, double su, double sv, int patch) {
+  const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<uint32_t*>(q), 0, bw * bh * 4, 0x00020000);
+  NccSums a;
+  if constexpr (HALF >= 0) {
+    constexpr int N = 2 * HALF + 1;
+    int xo[N];
+    double fx[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      const double u = su + (i - HALF);
+      const int x0 = static_cast<int>(u);
+      fx[i] = u - x0;
+      xo[i] = x0 * 4;
+    }
+    // q[i] holds column i of the current row; once summed it is reloaded
+    // with column i of the next row, consumed a whole row later.
+    uint32_t qv[N];
+    {
+      const int rowoff = static_cast<int>(sv + (-HALF)) * bw * 4;
+#pragma unroll
+      for (int i = 0; i < N; ++i) qv[i] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, rowoff + xo[i], 0, 0);
+    }
+#pragma unroll 1
+    for (int dy = -HALF; dy <= HALF; ++dy) {
+      const double v = sv + dy;
+      const double fy = v - static_cast<int>(v);
+      // the last row reloads itself (in bounds, unused)
+      const int next = static_cast<int>(sv + (dy < HALF ? dy + 1 : dy)) * bw * 4;
+      const double* trow = rt.t + (rt.ly + dy + HALF) * rt.tw + rt.lx;
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        const double vs = Bilinear(qv[i], fx[i], fy);
+        qv[i] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, next + xo[i], 0, 0);
+        const double vr = trow[i];
+        a.s_s += vs;
+        a.s_s2 += vs * vs;
+        a.s_x += vr * vs;
+      }
+    }
+  } else {
+    const int half = patch / 2;
+    for (int dy = -half; dy <= half; ++dy) {
+      const double v = sv + dy;
+      const int y0 = static_cast<int>(v);
+      const double fy = v - y0;
+      const int rowoff = y0 * bw * 4;
+      const double* trow = rt.t + (rt.ly + dy + half) * rt.tw + rt.lx + half;
+      for (int dx = -half; dx <= half; ++dx) {
+        const double u = su + dx;
+        const int x0 = static_cast<int>(u);
+        const uint32_t t = __builtin_amdgcn_raw_buffer_load_b32(rsrc, rowoff + x0 * 4, 0, 0);
+        const double vs = Bilinear(t, u - x0, fy);
+        const double vr = trow[dx];
+        a.s_s += vs;
+        a.s_s2 += vs * vs;
+        a.s_x += vr * vs;
+      }
+    }
+  }
+  const int n = 2 * (HALF >= 0 ? HALF : patch / 2) + 1;
+  a.s_r = rt.rs;
+  a.s_r2 = rt.rs2;
+  a.valid = n * n;
+  return FinishNcc(a, patch);
+}
+
+// Reference patch of the tile pixel, sources sampled from their quads.  The
+// source window [su - half, su + half] x [sv - half, sv + half] is inside
+// the source when its extreme samples are (FP64 addition is monotonic, so
+// su + dx lies between su - half and su + half); a wave takes the full-window
+// path when all its active lanes can, else the per-sample tests.
+template <int HALF>
 __device__ __forceinline__ Cost EvaluateTile(const RefProblem& p, const RefTile& rt,
                                              const double* ray, double inv) {
   return EvaluateWith(p, ray, inv, [&](int k, double su, double sv) {
-    return NccTile(rt.t, rt.tw, rt.ly, rt.lx, p.srcq[k], p.sw[k], p.sh[k], su, sv, p.patch);
+    const int half = HALF >= 0 ? HALF : p.patch / 2;
+    const int bw = p.sw[k], bh = p.sh[k];
+    const bool full = rt.full && su + (-half) >= 0.0 && sv + (-half) >= 0.0 &&
+                      su + half <= static_cast<double>(bw - 1) &&
+                      sv + half <= static_cast<double>(bh - 1);
+    if (__ballot(!full) == 0) return NccTileFull<HALF>(rt, p.srcq[k], bw, bh, su, sv, p.patch);
+    return NccTile(rt.t, rt.tw, rt.ly, rt.lx, p.srcq[k], bw, bh, su, sv, p.patch);
   });
 }
 
@@ -321,8 +449,10 @@ __global__ void k_evaluate(RefProblem p, int n, const double* __restrict__ uvi,
 }
 
 // Initial cost of every pixel (patchmatch_stereo.cpp:543-581).
-// One RefTile per workgroup (RefTiles(w, h) workgroups of kTileThreads).
-__global__ __launch_bounds__(kTileThreads) void k_pm_init_cost(RefProblem p,
+// One RefTile per workgroup (RefTiles(w, h) workgroups of kTileThreads);
+// HALF = patch / 2, or -1 for any patch.
+template <int HALF>
+__global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(3))) void k_pm_init_cost(RefProblem p,
                                                                const double* __restrict__ inv,
                                                                double* __restrict__ cost,
                                                                int* __restrict__ support) {
@@ -333,7 +463,7 @@ __global__ __launch_bounds__(kTileThreads) void k_pm_init_cost(RefProblem p,
   const double u = rt.x + 0.5, v = rt.y + 0.5;
   double ray[3];
   PixelToRay(p.cams[0], u, v, ray);
-  const Cost c = EvaluateTile(p, rt, ray, inv[idx]);
+  const Cost c = EvaluateTile<HALF>(p, rt, ray, inv[idx]);
   cost[idx] = c.cost;
   support[idx] = c.valid_views;
 }
@@ -341,8 +471,10 @@ __global__ __launch_bounds__(kTileThreads) void k_pm_init_cost(RefProblem p,
 // One directional pass (patchmatch_stereo.cpp:655-721) for every pixel.
 // tab holds this pass's replayed draws, SoA [K + 2][P]: K random-search draws,
 // the restart test draw, the restart value draw (0 when not drawn).
-// One RefTile per workgroup (RefTiles(w, h) workgroups of kTileThreads).
-__global__ __launch_bounds__(kTileThreads) void k_pm_pass(RefProblem p, const double* __restrict__ snap_inv,
+// One RefTile per workgroup (RefTiles(w, h) workgroups of kTileThreads);
+// HALF = patch / 2, or -1 for any patch.
+template <int HALF>
+__global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(3))) void k_pm_pass(RefProblem p, const double* __restrict__ snap_inv,
                           const double* __restrict__ snap_cost, const int* __restrict__ snap_sup,
                           double* __restrict__ out_inv, double* __restrict__ out_cost,
                           int* __restrict__ out_sup, const uint32_t* __restrict__ tab, int K,
@@ -369,7 +501,7 @@ __global__ __launch_bounds__(kTileThreads) void k_pm_pass(RefProblem p, const do
       const int nidx = ny * p.rw + nx;
       if (snap_cost[nidx] >= 1e29) continue;
       const double cand = snap_inv[nidx];
-      const Cost c = EvaluateTile(p, rt, ray, cand);
+      const Cost c = EvaluateTile<HALF>(p, rt, ray, cand);
       if (c.cost < best_c) {
         best_c = c.cost;
         best_i = cand;
@@ -385,7 +517,7 @@ __global__ __launch_bounds__(kTileThreads) void k_pm_pass(RefProblem p, const do
       radius *= 0.5;
       continue;
     }
-    const Cost c = EvaluateTile(p, rt, ray, cand);
+    const Cost c = EvaluateTile<HALF>(p, rt, ray, cand);
     if (c.cost < best_c) {
       best_c = c.cost;
       best_i = cand;
@@ -397,7 +529,7 @@ __global__ __launch_bounds__(kTileThreads) void k_pm_pass(RefProblem p, const do
   if (ur < restart_ratio) {
     const double uv = static_cast<double>(tab[static_cast<size_t>(K + 1) * P + idx]) / 4294967296.0;
     const double cand = inv_min + uv * (inv_max - inv_min);
-    const Cost c = EvaluateTile(p, rt, ray, cand);
+    const Cost c = EvaluateTile<HALF>(p, rt, ray, cand);
     if (c.cost < best_c) {
       best_c = c.cost;
       best_i = cand;
@@ -487,8 +619,10 @@ __device__ int Solve3(double A[3][3], double b[3], double x[3]) {
 }
 
 // RefineSubpixel body for one pixel (patchmatch_stereo.cpp:807-976).
-// One RefTile per workgroup (RefTiles(w, h) workgroups of kTileThreads).
-__global__ __launch_bounds__(kTileThreads) void k_refine(RefProblem p, float* __restrict__ depth,
+// One RefTile per workgroup (RefTiles(w, h) workgroups of kTileThreads);
+// HALF = patch / 2, or -1 for any patch.
+template <int HALF>
+__global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(3))) void k_refine(RefProblem p, float* __restrict__ depth,
                                                          float* __restrict__ conf, double inv_min,
                                                          double inv_max, double inv_per_px,
                                                          int fit_points, int iterations,
@@ -518,7 +652,7 @@ __global__ __launch_bounds__(kTileThreads) void k_refine(RefProblem p, float* __
           fit_ok = false;
           break;
         }
-        const double c = EvaluateTile(p, rt, ray, d).cost;
+        const double c = EvaluateTile<HALF>(p, rt, ray, d).cost;
         if (c >= 1e29) {
           fit_ok = false;
           break;
@@ -534,7 +668,7 @@ __global__ __launch_bounds__(kTileThreads) void k_refine(RefProblem p, float* __
         t2 += xx * xx * c;
       }
       double best_d = inv0;
-      double best_c = EvaluateTile(p, rt, ray, inv0).cost;
+      double best_c = EvaluateTile<HALF>(p, rt, ray, inv0).cost;
       if (fit_ok) {
         double A[3][3] = {{s4, s3, s2}, {s3, s2, s1}, {s2, s1, s0}};
         double b[3] = {t2, t1, t0};
@@ -546,7 +680,7 @@ __global__ __launch_bounds__(kTileThreads) void k_refine(RefProblem p, float* __
             if (vertex >= -static_cast<double>(half_fit) && vertex <= static_cast<double>(half_fit)) {
               // The reference adds the vertex in sample-index units (line 898).
               const double cand = inv0 + vertex;
-              const double cc = EvaluateTile(p, rt, ray, cand).cost;
+              const double cc = EvaluateTile<HALF>(p, rt, ray, cand).cost;
               if (cc < best_c) {
                 best_c = cc;
                 best_d = cand;
@@ -563,25 +697,25 @@ __global__ __launch_bounds__(kTileThreads) void k_refine(RefProblem p, float* __
         const double inv_phi = 0.6180339887498949;
         double c1 = hi - inv_phi * (hi - lo);
         double c2 = lo + inv_phi * (hi - lo);
-        double f1 = EvaluateTile(p, rt, ray, c1).cost;
-        double f2 = EvaluateTile(p, rt, ray, c2).cost;
+        double f1 = EvaluateTile<HALF>(p, rt, ray, c1).cost;
+        double f2 = EvaluateTile<HALF>(p, rt, ray, c2).cost;
         for (int k = 0; k < iterations; ++k) {
           if (f1 < f2) {
             hi = c2;
             c2 = c1;
             f2 = f1;
             c1 = hi - inv_phi * (hi - lo);
-            f1 = EvaluateTile(p, rt, ray, c1).cost;
+            f1 = EvaluateTile<HALF>(p, rt, ray, c1).cost;
           } else {
             lo = c1;
             c1 = c2;
             f1 = f2;
             c2 = lo + inv_phi * (hi - lo);
-            f2 = EvaluateTile(p, rt, ray, c2).cost;
+            f2 = EvaluateTile<HALF>(p, rt, ray, c2).cost;
           }
         }
         const double cm = 0.5 * (lo + hi);
-        const double fm = EvaluateTile(p, rt, ray, cm).cost;
+        const double fm = EvaluateTile<HALF>(p, rt, ray, cm).cost;
         if (fm < best_c) {
           best_c = fm;
           best_d = cm;
@@ -687,12 +821,30 @@ int32_t ValidateProblem(recon_mvs_hip_context* ctx, const recon_mvs_problem* pr,
   for (int k = 0; k < pr->num_src; ++k)
     if (!pr->src_pixels[k] || pr->src_w[k] <= 0 || pr->src_h[k] <= 0)
       return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "bad source image");
+  // pixel indices and the quads' byte offsets are 32-bit
+  constexpr int64_t kMaxPixels = int64_t{1} << 29;
+  if (static_cast<int64_t>(pr->ref_w) * pr->ref_h > kMaxPixels)
+    return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "reference image above 2^29 pixels");
+  for (int k = 0; k < pr->num_src; ++k)
+    if (static_cast<int64_t>(pr->src_w[k]) * pr->src_h[k] > kMaxPixels)
+      return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "source image above 2^29 pixels");
   if (cfg->patch_size < 1 || cfg->patch_size > 63)
     return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "patch_size out of [1, 63]");
   return RECON_MVS_OK;
 }
 
 inline unsigned Blocks(size_t n, unsigned t) { return static_cast<unsigned>((n + t - 1) / t); }
+
+// Calls f(std::integral_constant<int, HALF>) with the pixel kernels'
+// instantiation for the patch: unrolled for 7x7 and 11x11, generic otherwise.
+template <class F>
+void WithHalf(int patch, F f) {
+  switch (patch / 2) {
+    case 3: f(std::integral_constant<int, 3>()); break;
+    case 5: f(std::integral_constant<int, 5>()); break;
+    default: f(std::integral_constant<int, -1>()); break;
+  }
+}
 
 // Uploads images and cameras of a problem; fills the device RefProblem.  With
 // `quads`, also builds each source's quad image (for the tile kernels).
@@ -913,8 +1065,10 @@ RECON_MVS_API int32_t recon_mvs_hip_patchmatch_stereo(recon_mvs_hip_context* ctx
   RMVS_HIP(ctx, hipMemcpyAsync(invA, inv0.data(), P * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
   const unsigned tiles = RefTiles(w, h);
   const size_t tile_lds = RefTileBytes(config->patch_size);
-  hipLaunchKernelGGL(k_pm_init_cost, dim3(tiles), dim3(kTileThreads), tile_lds, ctx->stream, dp, invA,
-                     costA, supA);
+  WithHalf(config->patch_size, [&](auto hc) {
+    hipLaunchKernelGGL(k_pm_init_cost<decltype(hc)::value>, dim3(tiles), dim3(kTileThreads), tile_lds,
+                       ctx->stream, dp, invA, costA, supA);
+  });
   RMVS_LAUNCH_CHECK(ctx);
 
   const int threads = ResolveThreads(config->num_threads);
@@ -930,9 +1084,12 @@ RECON_MVS_API int32_t recon_mvs_hip_patchmatch_stereo(recon_mvs_hip_context* ctx
       FillPassTable(host, w, h, iter, dir, threads, K, config->restart_ratio);
       RMVS_HIP(ctx, hipMemcpyAsync(tab[b], host, tab_bytes, hipMemcpyHostToDevice, ctx->stream));
       RMVS_HIP(ctx, hipEventRecord(ctx->pinned_ready[b], ctx->stream));
-      hipLaunchKernelGGL(k_pm_pass, dim3(tiles), dim3(kTileThreads), tile_lds, ctx->stream, dp, invA, costA,
-                         supA, invB, costB, supB, tab[b], K, config->random_search_radius_ratio,
-                         config->restart_ratio, inv_min, inv_max, config->propagation_enabled ? 1 : 0);
+      WithHalf(config->patch_size, [&](auto hc) {
+        hipLaunchKernelGGL(k_pm_pass<decltype(hc)::value>, dim3(tiles), dim3(kTileThreads), tile_lds,
+                           ctx->stream, dp, invA, costA, supA, invB, costB, supB, tab[b], K,
+                           config->random_search_radius_ratio, config->restart_ratio, inv_min,
+                           inv_max, config->propagation_enabled ? 1 : 0);
+      });
       RMVS_LAUNCH_CHECK(ctx);
       std::swap(invA, invB);
       std::swap(costA, costB);
@@ -983,9 +1140,12 @@ RECON_MVS_API int32_t recon_mvs_hip_refine_subpixel(recon_mvs_hip_context* ctx,
   RMVS_HIP(ctx, hipMemcpyAsync(d_depth, depth, P * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
   RMVS_HIP(ctx, hipMemcpyAsync(d_conf, confidence, P * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
   RMVS_HIP(ctx, hipMemsetAsync(d_cnt, 0, sizeof(unsigned int), ctx->stream));
-  hipLaunchKernelGGL(k_refine, dim3(RefTiles(problem->ref_w, problem->ref_h)), dim3(kTileThreads),
-                     RefTileBytes(config->patch_size), ctx->stream, dp, d_depth, d_conf,
-                     inv_min, inv_max, inv_per_px, fit_points, config->subpixel_iterations, d_cnt);
+  WithHalf(config->patch_size, [&](auto hc) {
+    hipLaunchKernelGGL(k_refine<decltype(hc)::value>, dim3(RefTiles(problem->ref_w, problem->ref_h)),
+                       dim3(kTileThreads), RefTileBytes(config->patch_size), ctx->stream, dp, d_depth,
+                       d_conf, inv_min, inv_max, inv_per_px, fit_points, config->subpixel_iterations,
+                       d_cnt);
+  });
   RMVS_LAUNCH_CHECK(ctx);
   unsigned int cnt = 0;
   RMVS_HIP(ctx, hipMemcpyAsync(depth, d_depth, P * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
