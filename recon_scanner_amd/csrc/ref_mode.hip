@@ -387,11 +387,81 @@ __device__ __forceinline__ Ncc NccTileFull(const RefTile& rt, const uint32_t* __
   return FinishNcc(a, patch);
 }
 
+// NccTile for any window, HALF known: branch-free and pipelined like
+// NccTileFull.  Whether a sample pair counts is separable: the reference
+// sample (x + 0.5) + dx is inside iff 0 <= x + dx <= rw - 2 (likewise rows),
+// the source sample su + dx by SampleQuad's test on its column (likewise
+// rows), so a pair counts iff its column and its row both do.  A pair that
+// does not count adds +0.0 to every (non-negative) sum, which leaves it
+// unchanged, so the sums equal NccTile's bit for bit.  Coordinates are
+// clamped into the source before their conversion to int; the buffer
+// descriptor bounds every load (a load outside it returns 0).
+template <int HALF>
+__device__ __forceinline__ Ncc NccTileMasked(const RefTile& rt, int rw, int rh,
+                                             const uint32_t* __restrict__ q, int bw, int bh,
+                                             double su, double sv, int patch) {
+  constexpr int N = 2 * HALF + 1;
+  const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<uint32_t*>(q), 0, bw * bh * 4, 0x00020000);
+  const double umax = static_cast<double>(bw - 1), vmax = static_cast<double>(bh - 1);
+  uint32_t xo[N];
+  double fx[N];
+  uint32_t cols = 0;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const double u = su + (i - HALF);
+    const int X = rt.x + i - HALF;
+    const bool ok = !(u < 0.0 || u > umax) && X >= 0 && X <= rw - 2;
+    const int x0 = static_cast<int>(fmin(fmax(u, 0.0), umax));
+    fx[i] = u - x0;
+    xo[i] = static_cast<uint32_t>(x0) * 4u;
+    cols |= static_cast<uint32_t>(ok) << i;
+  }
+  const int ncols = __builtin_popcount(cols);
+  const uint32_t pitch = static_cast<uint32_t>(bw) * 4u;
+  auto row_off = [&](int dy) {
+    return static_cast<uint32_t>(static_cast<int>(fmin(fmax(sv + dy, 0.0), vmax))) * pitch;
+  };
+  NccSums a;
+  uint32_t qv[N];
+  {
+    const uint32_t r0 = row_off(-HALF);
+#pragma unroll
+    for (int i = 0; i < N; ++i) qv[i] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, r0 + xo[i], 0, 0);
+  }
+#pragma unroll 1
+  for (int dy = -HALF; dy <= HALF; ++dy) {
+    const double v = sv + dy;
+    const int Y = rt.y + dy;
+    const bool rok = !(v < 0.0 || v > vmax) && Y >= 0 && Y <= rh - 2;
+    const double fy = v - static_cast<int>(fmin(fmax(v, 0.0), vmax));
+    const uint32_t next = row_off(dy < HALF ? dy + 1 : dy);
+    const double* trow = rt.t + (rt.ly + dy + HALF) * rt.tw + rt.lx;
+    const uint32_t m = rok ? cols : 0u;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      const double vs0 = Bilinear(qv[i], fx[i], fy);
+      qv[i] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, next + xo[i], 0, 0);
+      const bool ok = (m >> i) & 1u;
+      const double vs = ok ? vs0 : 0.0;
+      const double vr = ok ? trow[i] : 0.0;
+      a.s_r += vr;
+      a.s_s += vs;
+      a.s_r2 += vr * vr;
+      a.s_s2 += vs * vs;
+      a.s_x += vr * vs;
+    }
+    a.valid += rok ? ncols : 0;
+  }
+  return FinishNcc(a, patch);
+}
+
 // Reference patch of the tile pixel, sources sampled from their quads.  The
 // source window [su - half, su + half] x [sv - half, sv + half] is inside
 // the source when its extreme samples are (FP64 addition is monotonic, so
 // su + dx lies between su - half and su + half); a wave takes the full-window
-// path when all its active lanes can, else the per-sample tests.
+// path when all its active lanes can, else the masked one (HALF known) or
+// the per-sample tests.
 template <int HALF>
 __device__ __forceinline__ Cost EvaluateTile(const RefProblem& p, const RefTile& rt,
                                              const double* ray, double inv) {
@@ -402,7 +472,10 @@ __device__ __forceinline__ Cost EvaluateTile(const RefProblem& p, const RefTile&
                       su + half <= static_cast<double>(bw - 1) &&
                       sv + half <= static_cast<double>(bh - 1);
     if (__ballot(!full) == 0) return NccTileFull<HALF>(rt, p.srcq[k], bw, bh, su, sv, p.patch);
-    return NccTile(rt.t, rt.tw, rt.ly, rt.lx, p.srcq[k], bw, bh, su, sv, p.patch);
+    if constexpr (HALF >= 0)
+      return NccTileMasked<HALF>(rt, p.rw, p.rh, p.srcq[k], bw, bh, su, sv, p.patch);
+    else
+      return NccTile(rt.t, rt.tw, rt.ly, rt.lx, p.srcq[k], bw, bh, su, sv, p.patch);
   });
 }
 
@@ -452,7 +525,7 @@ __global__ void k_evaluate(RefProblem p, int n, const double* __restrict__ uvi,
 // One RefTile per workgroup (RefTiles(w, h) workgroups of kTileThreads);
 // HALF = patch / 2, or -1 for any patch.
 template <int HALF>
-__global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(3))) void k_pm_init_cost(RefProblem p,
+__global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(2))) void k_pm_init_cost(RefProblem p,
                                                                const double* __restrict__ inv,
                                                                double* __restrict__ cost,
                                                                int* __restrict__ support) {
@@ -474,7 +547,7 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(3)
 // One RefTile per workgroup (RefTiles(w, h) workgroups of kTileThreads);
 // HALF = patch / 2, or -1 for any patch.
 template <int HALF>
-__global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(3))) void k_pm_pass(RefProblem p, const double* __restrict__ snap_inv,
+__global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(2))) void k_pm_pass(RefProblem p, const double* __restrict__ snap_inv,
                           const double* __restrict__ snap_cost, const int* __restrict__ snap_sup,
                           double* __restrict__ out_inv, double* __restrict__ out_cost,
                           int* __restrict__ out_sup, const uint32_t* __restrict__ tab, int K,
@@ -622,7 +695,7 @@ __device__ int Solve3(double A[3][3], double b[3], double x[3]) {
 // One RefTile per workgroup (RefTiles(w, h) workgroups of kTileThreads);
 // HALF = patch / 2, or -1 for any patch.
 template <int HALF>
-__global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(3))) void k_refine(RefProblem p, float* __restrict__ depth,
+__global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(2))) void k_refine(RefProblem p, float* __restrict__ depth,
                                                          float* __restrict__ conf, double inv_min,
                                                          double inv_max, double inv_per_px,
                                                          int fit_points, int iterations,
