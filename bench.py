@@ -51,7 +51,11 @@ def parse():
     ap.add_argument("--height", type=int, default=1440)
     ap.add_argument("--iterations", type=int, default=5)
     ap.add_argument("--storage", default=None, choices=["f32", "f16", "bf16"],
-                    help="image storage (default: f32; f16 for --workload batch)")
+                    help="image storage (default: f16, which holds 8-bit images exactly; f32 "
+                         "for --images float)")
+    ap.add_argument("--images", default="u8", choices=["float", "u8"],
+                    help="plane-mode input images: the renderer's images quantized to 8 bits "
+                         "(what a camera and the reference deliver), or its float images")
     ap.add_argument("--view-phased", type=int, default=-1,
                     help="leading iterations run view-phased (one launch per source view); "
                          "-1: the library's automatic choice")
@@ -94,9 +98,10 @@ def cpu_baseline(sc, iterations, target_s):
         err = float(np.median(np.abs(d[m] - truth[m]))) if m.any() else float("nan")
         return dt, err
 
-    # calibrate on a small crop, then size the sample to ~target_s
-    dt, _ = run(64, 48, 1)
-    per_unit = dt / (64 * 48 * 8 * 1)
+    # calibrate on a small crop (large enough that thread start-up is a
+    # small part of it), then size the sample to ~target_s
+    dt, _ = run(160, 120, 1)
+    per_unit = dt / (160 * 120 * 8 * 1)
     area = max(64 * 48, min(W * H, int(target_s / (per_unit * 8 * iterations))))
     ch = max(48, min(H, int((area * 3 / 4) ** 0.5)))
     cw = max(64, min(W, int(ch * 4 / 3)))
@@ -340,7 +345,9 @@ def main():
         return run_reference(args)
     if args.workload == "copy":
         return run_copy(args)
-    args.storage = args.storage or "f32"
+    # 8-bit values are exact in FP16: the depth maps are bit-identical to
+    # FP32 storage (a GPU test holds this), at half the image bytes
+    args.storage = args.storage or ("f16" if args.images == "u8" else "f32")
     from recon_scanner_amd import plane, scenes
 
     t0 = time.time()
@@ -354,7 +361,8 @@ def main():
                              min_depth_m=sc.min_depth, max_depth_m=sc.max_depth, value_scale=1.0,
                              tile_schedule=args.tile_schedule,
                              view_phased_iterations=args.view_phased)
-    pm = plane.PlaneMVS(sc.cams, sc.images, storage=args.storage)
+    imgs = sc.images if args.images == "float" else [sc.u8(i) for i in range(nv)]
+    pm = plane.PlaneMVS(sc.cams, imgs, storage=args.storage)
 
     def step():
         for r in views:
@@ -426,15 +434,16 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32" if args.storage == "f32" else f"fp32 ({args.storage} images)",
-        "data": "synthetic (rendered sphere + 2 tilted planes, analytic depth)",
+        "dtype": "fp32" if (args.storage == "f32" or args.images == "u8") else f"fp32 ({args.storage} images)",
+        "data": "synthetic (rendered sphere + 2 tilted planes, analytic depth"
+                + (", 8-bit images)" if args.images == "u8" else ", float images)"),
         "config": {"workload": f"plane-mode PatchMatch, 9 views {args.width}x{args.height}, "
                                f"each view vs 8 sources, 11x11 stride 2, top-4, "
                                f"{args.iterations} red-black iterations + geometric-consistency "
                                f"filter over 9 depth maps",
                    "views": nv, "width": args.width, "height": args.height,
-                   "iterations": args.iterations, "storage": args.storage,
-                   "view_phased_iterations": args.view_phased},
+                   "iterations": args.iterations, "images": args.images,
+                   "storage": args.storage, "view_phased_iterations": args.view_phased},
         "hypothesis_view_evals_per_s": hve * args.steps / dt,
         "median_depth_error_mm": med,
         "kept_pixels": kept,

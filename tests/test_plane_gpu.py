@@ -338,3 +338,28 @@ def test_view_phased_sweeps_bit_identical(storage, top_k):
         seeded.append(pm.depth_map(0))
     for k in ("depth", "normals", "cost", "support"):
         assert np.array_equal(seeded[1][k], seeded[0][k], equal_nan=True), k
+
+
+def test_u8_images_half_storage_bit_identical():
+    """8-bit images stored as FP16 or BF16 (both hold the integers 0..255
+    exactly) give the FP32-storage results bit for bit: planes, costs,
+    supports, confidences, and the consistency filter over them.  The bench
+    stores its 8-bit images as FP16 on this ground."""
+    say("u8 images: f32 vs f16 vs bf16 storage, sphere_planes 480x360 11x11/2")
+    sc = scenes.sphere_planes(width=480, height=360, focal=625.0, supersample=1, n_views=5)
+    u8 = [sc.u8(i) for i in range(sc.num_views)]
+    views = list(range(sc.num_views))
+    p = plane.default_params(patch_size=11, window_stride=2, top_k=4, iterations=3,
+                             min_depth_m=0.25, max_depth_m=0.60, value_scale=1.0)
+    res = {}
+    for storage in ("f32", "f16", "bf16"):
+        pm = plane.PlaneMVS(sc.cams, u8, storage=storage)
+        for r in views:
+            pm.patchmatch(r, [v for v in views if v != r], p)
+        pm.geo_consistency(views, 0.01)
+        res[storage] = [pm.depth_map(r, geo=True) for r in views]
+    for storage in ("f16", "bf16"):
+        for a, b in zip(res["f32"], res[storage]):
+            for k in a:
+                assert a[k].tobytes() == b[k].tobytes(), (storage, k)
+    assert np.isfinite(res["f32"][0]["depth"]).mean() > 0.9
