@@ -343,7 +343,8 @@ def test_view_phased_sweeps_bit_identical(storage, top_k):
 def test_u8_images_half_storage_bit_identical():
     """8-bit images stored as FP16 or BF16 (both hold the integers 0..255
     exactly) give the FP32-storage results bit for bit: planes, costs,
-    supports, confidences, and the consistency filter over them.  The bench
+    supports, confidences, after refinement and polish, and the consistency
+    filter over them.  The bench
     stores its 8-bit images as FP16 on this ground."""
     say("u8 images: f32 vs f16 vs bf16 storage, sphere_planes 480x360 11x11/2")
     sc = scenes.sphere_planes(width=480, height=360, focal=625.0, supersample=1, n_views=5)
@@ -355,7 +356,11 @@ def test_u8_images_half_storage_bit_identical():
     for storage in ("f32", "f16", "bf16"):
         pm = plane.PlaneMVS(sc.cams, u8, storage=storage)
         for r in views:
-            pm.patchmatch(r, [v for v in views if v != r], p)
+            srcs = [v for v in views if v != r]
+            pm.patchmatch(r, srcs, p)
+            if r == 0:  # the sub-pixel refinement and the gradient polish too
+                pm.refine(r, srcs, p)
+                pm.polish(r, srcs, p, steps=5)
         pm.geo_consistency(views, 0.01)
         res[storage] = [pm.depth_map(r, geo=True) for r in views]
     for storage in ("f16", "bf16"):

@@ -84,12 +84,16 @@ def surface_distance(p):
     return np.minimum(ds, np.minimum(da, db))
 
 
-def test_plane_mode_cloud_on_the_surfaces(keyframes):
+@pytest.mark.parametrize("patch", [7, 11])
+def test_plane_mode_cloud_on_the_surfaces(keyframes, patch):
     """Plane mode's cloud is at least as close to the surfaces as reference
     mode's on the same keyframes, and within one pixel footprint (Z / f ~
-    1.9 mm here) at the median."""
+    1.9 mm here) at the median.  Patch 7 runs the (7, 1) window on FP32
+    images; patch 11 the (11, 2) window on FP16 images."""
     sc, gray, cfg = keyframes
-    say("stage, plane mode (batch + consistency), 5 keyframes 160x120")
+    cfg = type(cfg).from_buffer_copy(cfg)
+    cfg.patch_size = patch
+    say("stage, plane mode (batch + consistency), 5 keyframes 160x120, patch", patch)
     produced, maps, cloud = mvs.RunMvsStage(gray, sc.cams, cfg, mvs.STAGE_PLANE)
     _, _, ref_cloud = mvs.RunMvsStage(gray, sc.cams, cfg, mvs.STAGE_REFERENCE)
     dist = surface_distance(cloud.positions)
