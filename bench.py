@@ -30,6 +30,11 @@ PEAK_FP32_TFLOPS = 157.3    # MI355X FP32 vector peak
 PEAK_HBM_GBS = 8000.0
 FLOP_PER_SAMPLE = 20        # warp 6 + bilinear 9 + accumulate 5 per window sample
 HYPS_PER_ITER = 8           # 4 propagated + 3 perturbed + 1 random
+PEAK_FP64_TFLOPS = 78.6     # MI355X FP64 vector peak (reference mode)
+# reference mode, per sample pair: source bilinear 12 (u, fx, 1 - fx, three
+# lerps of 3) + the sums of the source values, their squares and the cross
+# products 5; the reference patch's own sums are per pixel
+REF_FLOP_PER_PAIR = 17
 
 
 def parse():
@@ -270,12 +275,22 @@ def run_reference(args):
 
     w, h = (args.width, args.height) if args.width != 1920 else (1280, 960)
     sc, u8, cfg = problem(w, h)
+    L, ctx = mvs.lib(), mvs.context()
+
+    def timers(enable, reset):
+        ms, cnt = np.zeros(8), np.zeros(8, np.int64)
+        mvs.check(L.recon_mvs_hip_timing(ctx, int(enable), int(reset), ms.ctypes.data,
+                                         cnt.ctypes.data), "timing")
+        return ms, cnt
+
     for _ in range(args.warmup):
         gpu_once(sc, u8, cfg)
+    timers(True, True)
     t = time.perf_counter()
     for _ in range(args.steps):
         filled, refined, dm = gpu_once(sc, u8, cfg)
     dt = time.perf_counter() - t
+    ms, cnt = timers(False, False)
     rec = base_record(args, dt, f"reference-mode PatchMatchStereo + RefineSubpixel, {w}x{h}, "
                       f"8 sources, 11x11, {args.iterations} iterations x 4 directions, "
                       f"num_threads={cores}", width=w, height=h, iterations=args.iterations)
@@ -286,6 +301,25 @@ def run_reference(args):
     d = dm.depth.reshape(h, w)
     m = np.isfinite(d) & (truth > 0)
     rec["median_depth_error_mm"] = float(np.median(np.abs(d[m] - truth[m]))) * 1000.0
+    # roofline of one directional pass (k_pm_pass, HIP events on the
+    # library's stream): nominal hypotheses per pixel and pass H = 4
+    # neighbours + random_search_count + restart_ratio, each over 8 views x
+    # patch^2 sample pairs of REF_FLOP_PER_PAIR FP64 operations
+    pass_ms = ms[0] / cnt[0] if cnt[0] else None
+    hyps = 4 + cfg.random_search_count + cfg.restart_ratio
+    flop = w * h * hyps * 8 * cfg.patch_size ** 2 * REF_FLOP_PER_PAIR
+    achieved = flop / (pass_ms * 1e-3) / 1e12 if pass_ms else None
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "sweep_traffic.txt")
+    if os.path.exists(tf):
+        for line in open(tf):
+            if line.startswith(f"traffic_bytes_per_launch reference {w}x{h} "):
+                traffic = float(line.split()[-1])
+    rec["roofline"] = {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
+                       "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS if achieved else None,
+                       "traffic": traffic, "kernel": "k_pm_pass (one directional pass)",
+                       "mean_launch_ms": pass_ms, "flop_per_launch": flop,
+                       "hypotheses_per_pixel": hyps}
     if args.cpu_seconds > 0 and oracle.reference_available():
         sc, u8, cfg = problem(320, 240)
         filled, refined, dm = gpu_once(sc, u8, cfg)

@@ -547,10 +547,11 @@ RECON_MVS_API int32_t recon_mvs_hip_run_mvs_stage(recon_mvs_hip_context* ctx, in
 
 /*
  * Kernel timers.  enable != 0 records HIP events around every launch of the
- * plane-mode kernels on the context's stream; ms / counts (8 entries each,
- * may be NULL) receive the accumulated milliseconds and launch counts per
- * kernel class: 0 sweep, 1 init, 2 refine, 3 consistency, 4 score, 5 eval,
- * 6 batch (one graph run), 7 adjoint / polish.
+ * plane-mode kernels, and of reference mode's pass / initial-cost / refine
+ * kernels, on the context's stream; ms / counts (8 entries each, may be NULL)
+ * receive the accumulated milliseconds and launch counts per kernel class:
+ * 0 sweep (reference mode: one directional pass), 1 init, 2 refine,
+ * 3 consistency, 4 score, 5 eval, 6 batch (one graph run), 7 adjoint / polish.
  * reset != 0 clears the accumulators.
  */
 RECON_MVS_API int32_t recon_mvs_hip_timing(recon_mvs_hip_context* ctx, int32_t enable,

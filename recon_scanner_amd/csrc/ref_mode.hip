@@ -26,6 +26,11 @@ namespace rmvs {
 
 constexpr int kMaxSrc = 32;
 constexpr int kMaxFitPoints = 31;
+// Timer classes (recon_mvs_hip_timing) of the reference-mode kernels: a
+// directional pass counts as a sweep, the initial costs as init.
+constexpr int kTimerPass = 0;
+constexpr int kTimerInit = 1;
+constexpr int kTimerRefine = 2;
 
 struct RefProblem {
   const uint8_t* ref;
@@ -1138,11 +1143,14 @@ RECON_MVS_API int32_t recon_mvs_hip_patchmatch_stereo(recon_mvs_hip_context* ctx
   RMVS_HIP(ctx, hipMemcpyAsync(invA, inv0.data(), P * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
   const unsigned tiles = RefTiles(w, h);
   const size_t tile_lds = RefTileBytes(config->patch_size);
+  hipEvent_t eb, ee;
+  TimerBegin(ctx, kTimerInit, &eb, &ee);
   WithHalf(config->patch_size, [&](auto hc) {
     hipLaunchKernelGGL(k_pm_init_cost<decltype(hc)::value>, dim3(tiles), dim3(kTileThreads), tile_lds,
                        ctx->stream, dp, invA, costA, supA);
   });
   RMVS_LAUNCH_CHECK(ctx);
+  TimerEnd(ctx, kTimerInit, eb, ee);
 
   const int threads = ResolveThreads(config->num_threads);
   const int dirs = std::max(1, config->propagation_directions);
@@ -1157,6 +1165,7 @@ RECON_MVS_API int32_t recon_mvs_hip_patchmatch_stereo(recon_mvs_hip_context* ctx
       FillPassTable(host, w, h, iter, dir, threads, K, config->restart_ratio);
       RMVS_HIP(ctx, hipMemcpyAsync(tab[b], host, tab_bytes, hipMemcpyHostToDevice, ctx->stream));
       RMVS_HIP(ctx, hipEventRecord(ctx->pinned_ready[b], ctx->stream));
+      TimerBegin(ctx, kTimerPass, &eb, &ee);
       WithHalf(config->patch_size, [&](auto hc) {
         hipLaunchKernelGGL(k_pm_pass<decltype(hc)::value>, dim3(tiles), dim3(kTileThreads), tile_lds,
                            ctx->stream, dp, invA, costA, supA, invB, costB, supB, tab[b], K,
@@ -1164,6 +1173,7 @@ RECON_MVS_API int32_t recon_mvs_hip_patchmatch_stereo(recon_mvs_hip_context* ctx
                            inv_max, config->propagation_enabled ? 1 : 0);
       });
       RMVS_LAUNCH_CHECK(ctx);
+      TimerEnd(ctx, kTimerPass, eb, ee);
       std::swap(invA, invB);
       std::swap(costA, costB);
       std::swap(supA, supB);
@@ -1213,6 +1223,8 @@ RECON_MVS_API int32_t recon_mvs_hip_refine_subpixel(recon_mvs_hip_context* ctx,
   RMVS_HIP(ctx, hipMemcpyAsync(d_depth, depth, P * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
   RMVS_HIP(ctx, hipMemcpyAsync(d_conf, confidence, P * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
   RMVS_HIP(ctx, hipMemsetAsync(d_cnt, 0, sizeof(unsigned int), ctx->stream));
+  hipEvent_t eb, ee;
+  TimerBegin(ctx, kTimerRefine, &eb, &ee);
   WithHalf(config->patch_size, [&](auto hc) {
     hipLaunchKernelGGL(k_refine<decltype(hc)::value>, dim3(RefTiles(problem->ref_w, problem->ref_h)),
                        dim3(kTileThreads), RefTileBytes(config->patch_size), ctx->stream, dp, d_depth,
@@ -1220,6 +1232,7 @@ RECON_MVS_API int32_t recon_mvs_hip_refine_subpixel(recon_mvs_hip_context* ctx,
                        d_cnt);
   });
   RMVS_LAUNCH_CHECK(ctx);
+  TimerEnd(ctx, kTimerRefine, eb, ee);
   unsigned int cnt = 0;
   RMVS_HIP(ctx, hipMemcpyAsync(depth, d_depth, P * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
   RMVS_HIP(ctx, hipMemcpyAsync(confidence, d_conf, P * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));

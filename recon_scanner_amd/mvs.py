@@ -129,6 +129,7 @@ def lib():
         L.recon_mvs_hip_last_error.argtypes = [ctxp]
         L.recon_mvs_hip_last_error.restype = C.c_char_p
         L.recon_mvs_hip_synchronize.argtypes = [ctxp]
+        L.recon_mvs_hip_timing.argtypes = [ctxp, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p]
         L.recon_mvs_config_default.argtypes = [P(N.Config)]
         L.recon_mvs_config_default.restype = None
         L.recon_mvs_hip_compute_ncc.argtypes = [ctxp, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p,
