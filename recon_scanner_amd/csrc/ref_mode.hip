@@ -32,13 +32,17 @@ constexpr int kTimerPass = 0;
 constexpr int kTimerInit = 1;
 constexpr int kTimerRefine = 2;
 
+// A source texel of the quad images (see MakeQuad).
+using Quad = uint32_t;
+constexpr int kQuadBytes = sizeof(Quad);
+
 struct RefProblem {
   const uint8_t* ref;
   int rw, rh;
   int nsrc;
   int patch;
   const uint8_t* src[kMaxSrc];
-  const uint32_t* srcq[kMaxSrc];  // source k as 4-tap quads (k_make_quads)
+  const Quad* srcq[kMaxSrc];  // source k as quads (k_make_quads)
   int sw[kMaxSrc];
   int sh[kMaxSrc];
   const recon_mvs_camera* cams;  // [0] reference, [1 + k] source k
@@ -73,26 +77,41 @@ __device__ __forceinline__ bool SampleU8(const uint8_t* __restrict__ img, int w,
   return true;
 }
 
-// SampleU8 on a quad image: texel (y, x) packs the four pixels a sample with
-// floor (x, y) reads, p00 | p10 << 8 | p01 << 16 | p11 << 24, the clamps of x1
-// and y1 applied when the quads were built.  Same operations on the same
-// values as SampleU8, one 4-byte load instead of four 1-byte loads.
-__device__ __forceinline__ bool SampleQuad(const uint32_t* __restrict__ q, int w, int h, double u,
+// Quad image of a source: texel (y, x) holds the four pixels a sample with
+// floor (x, y) reads — p00, p10, p01, p11, the clamps of x1 and y1 applied
+// when the quads were built (k_make_quads) — as bytes p00 | p10 << 8 |
+// p01 << 16 | p11 << 24: one 4-byte load per sample instead of four 1-byte
+// loads.
+__device__ __forceinline__ Quad MakeQuad(uint8_t p00, uint8_t p10, uint8_t p01, uint8_t p11) {
+  return static_cast<uint32_t>(p00) | static_cast<uint32_t>(p10) << 8 |
+         static_cast<uint32_t>(p01) << 16 | static_cast<uint32_t>(p11) << 24;
+}
+
+// Tap i (0 p00, 1 p10, 2 p01, 3 p11) as a double.
+__device__ __forceinline__ double QuadTap(Quad t, int i) {
+  return static_cast<double>((t >> (8 * i)) & 0xffu);
+}
+
+// The texel at byte offset `off` through a raw buffer descriptor.
+__device__ __forceinline__ Quad LoadQuad(__amdgpu_buffer_rsrc_t rsrc, uint32_t off) {
+  return __builtin_amdgcn_raw_buffer_load_b32(rsrc, off, 0, 0);
+}
+
+// SampleQuad's arithmetic for a sample inside the source (no bounds tests).
+__device__ __forceinline__ double Bilinear(Quad t, double fx, double fy) {
+  const double top = QuadTap(t, 0) * (1.0 - fx) + QuadTap(t, 1) * fx;
+  const double bot = QuadTap(t, 2) * (1.0 - fx) + QuadTap(t, 3) * fx;
+  return top * (1.0 - fy) + bot * fy;
+}
+
+// SampleU8 on a quad image: same operations on the same values.
+__device__ __forceinline__ bool SampleQuad(const Quad* __restrict__ q, int w, int h, double u,
                                            double v, double* out) {
   if (u < 0.0 || v < 0.0 || u > static_cast<double>(w - 1) || v > static_cast<double>(h - 1))
     return false;
   const int x0 = static_cast<int>(floor(u));
   const int y0 = static_cast<int>(floor(v));
-  const double fx = u - x0;
-  const double fy = v - y0;
-  const uint32_t t = q[y0 * w + x0];
-  const double p00 = static_cast<double>(t & 0xffu);
-  const double p10 = static_cast<double>((t >> 8) & 0xffu);
-  const double p01 = static_cast<double>((t >> 16) & 0xffu);
-  const double p11 = static_cast<double>(t >> 24);
-  const double top = p00 * (1.0 - fx) + p10 * fx;
-  const double bot = p01 * (1.0 - fx) + p11 * fx;
-  *out = top * (1.0 - fy) + bot * fy;
+  *out = Bilinear(q[y0 * w + x0], u - x0, v - y0);
   return true;
 }
 
@@ -142,7 +161,7 @@ __device__ Ncc NccU8(const uint8_t* __restrict__ a, int aw, int ah, const uint8_
 // NccU8; the sampling has no side effects, so the order of the two tests does
 // not matter.
 __device__ Ncc NccTile(const double* __restrict__ tile, int tw, int trow, int tcol,
-                       const uint32_t* __restrict__ q, int bw, int bh, double su, double sv,
+                       const Quad* __restrict__ q, int bw, int bh, double su, double sv,
                        int patch) {
   const int half = patch / 2;
   NccSums acc;
@@ -303,18 +322,6 @@ __device__ RefTile LoadRefTile(const RefProblem& p, double* lds) {
 // each source sample is SampleQuad's arithmetic without its bounds tests:
 // a non-negative coordinate truncates to its floor.  HALF = patch / 2 when
 // known at compile time (unrolled rows), -1 otherwise.
-// Byte i of a quad as a double.
-__device__ __forceinline__ double QuadTap(uint32_t t, int i) {
-  return static_cast<double>((t >> (8 * i)) & 0xffu);
-}
-
-// SampleQuad's arithmetic for a sample inside the source (no bounds tests).
-__device__ __forceinline__ double Bilinear(uint32_t t, double fx, double fy) {
-  const double top = QuadTap(t, 0) * (1.0 - fx) + QuadTap(t, 1) * fx;
-  const double bot = QuadTap(t, 2) * (1.0 - fx) + QuadTap(t, 3) * fx;
-  return top * (1.0 - fy) + bot * fy;
-}
-
 // NccTile when every sample pair is valid: the reference patch inside the
 // reference image (rt.full) and the source window inside the source.  Then
 // no sample is skipped, s_r / s_r2 are the precomputed rt.rs / rt.rs2, and
@@ -324,10 +331,10 @@ __device__ __forceinline__ double Bilinear(uint32_t t, double fx, double fy) {
 // (x0, fx: the same on every row) are computed once and the next row's quads
 // are loaded while the current row is summed.
 template <int HALF>
-__device__ __forceinline__ Ncc NccTileFull(const RefTile& rt, const uint32_t* __restrict__ q, int bw,
+__device__ __forceinline__ Ncc NccTileFull(const RefTile& rt, const Quad* __restrict__ q, int bw,
                                            int bh, double su, double sv, int patch) {
   const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
-      const_cast<uint32_t*>(q), 0, bw * bh * 4, 0x00020000);
+      const_cast<Quad*>(q), 0, bw * bh * kQuadBytes, 0x00020000);
   NccSums a;
   if constexpr (HALF >= 0) {
     constexpr int N = 2 * HALF + 1;
@@ -338,27 +345,27 @@ __device__ __forceinline__ Ncc NccTileFull(const RefTile& rt, const uint32_t* __
       const double u = su + (i - HALF);
       const int x0 = static_cast<int>(u);
       fx[i] = u - x0;
-      xo[i] = x0 * 4;
+      xo[i] = x0 * kQuadBytes;
     }
     // q[i] holds column i of the current row; once summed it is reloaded
     // with column i of the next row, consumed a whole row later.
-    uint32_t qv[N];
+    Quad qv[N];
     {
-      const int rowoff = static_cast<int>(sv + (-HALF)) * bw * 4;
+      const int rowoff = static_cast<int>(sv + (-HALF)) * bw * kQuadBytes;
 #pragma unroll
-      for (int i = 0; i < N; ++i) qv[i] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, rowoff + xo[i], 0, 0);
+      for (int i = 0; i < N; ++i) qv[i] = LoadQuad(rsrc, rowoff + xo[i]);
     }
 #pragma unroll 1
     for (int dy = -HALF; dy <= HALF; ++dy) {
       const double v = sv + dy;
       const double fy = v - static_cast<int>(v);
       // the last row reloads itself (in bounds, unused)
-      const int next = static_cast<int>(sv + (dy < HALF ? dy + 1 : dy)) * bw * 4;
+      const int next = static_cast<int>(sv + (dy < HALF ? dy + 1 : dy)) * bw * kQuadBytes;
       const double* trow = rt.t + (rt.ly + dy + HALF) * rt.tw + rt.lx;
 #pragma unroll
       for (int i = 0; i < N; ++i) {
         const double vs = Bilinear(qv[i], fx[i], fy);
-        qv[i] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, next + xo[i], 0, 0);
+        qv[i] = LoadQuad(rsrc, next + xo[i]);
         const double vr = trow[i];
         a.s_s += vs;
         a.s_s2 += vs * vs;
@@ -371,12 +378,12 @@ __device__ __forceinline__ Ncc NccTileFull(const RefTile& rt, const uint32_t* __
       const double v = sv + dy;
       const int y0 = static_cast<int>(v);
       const double fy = v - y0;
-      const int rowoff = y0 * bw * 4;
+      const int rowoff = y0 * bw * kQuadBytes;
       const double* trow = rt.t + (rt.ly + dy + half) * rt.tw + rt.lx + half;
       for (int dx = -half; dx <= half; ++dx) {
         const double u = su + dx;
         const int x0 = static_cast<int>(u);
-        const uint32_t t = __builtin_amdgcn_raw_buffer_load_b32(rsrc, rowoff + x0 * 4, 0, 0);
+        const Quad t = LoadQuad(rsrc, rowoff + x0 * kQuadBytes);
         const double vs = Bilinear(t, u - x0, fy);
         const double vr = trow[dx];
         a.s_s += vs;
@@ -403,11 +410,11 @@ __device__ __forceinline__ Ncc NccTileFull(const RefTile& rt, const uint32_t* __
 // descriptor bounds every load (a load outside it returns 0).
 template <int HALF>
 __device__ __forceinline__ Ncc NccTileMasked(const RefTile& rt, int rw, int rh,
-                                             const uint32_t* __restrict__ q, int bw, int bh,
+                                             const Quad* __restrict__ q, int bw, int bh,
                                              double su, double sv, int patch) {
   constexpr int N = 2 * HALF + 1;
   const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
-      const_cast<uint32_t*>(q), 0, bw * bh * 4, 0x00020000);
+      const_cast<Quad*>(q), 0, bw * bh * kQuadBytes, 0x00020000);
   const double umax = static_cast<double>(bw - 1), vmax = static_cast<double>(bh - 1);
   uint32_t xo[N];
   double fx[N];
@@ -419,20 +426,20 @@ __device__ __forceinline__ Ncc NccTileMasked(const RefTile& rt, int rw, int rh,
     const bool ok = !(u < 0.0 || u > umax) && X >= 0 && X <= rw - 2;
     const int x0 = static_cast<int>(fmin(fmax(u, 0.0), umax));
     fx[i] = u - x0;
-    xo[i] = static_cast<uint32_t>(x0) * 4u;
+    xo[i] = static_cast<uint32_t>(x0) * kQuadBytes;
     cols |= static_cast<uint32_t>(ok) << i;
   }
   const int ncols = __builtin_popcount(cols);
-  const uint32_t pitch = static_cast<uint32_t>(bw) * 4u;
+  const uint32_t pitch = static_cast<uint32_t>(bw) * kQuadBytes;
   auto row_off = [&](int dy) {
     return static_cast<uint32_t>(static_cast<int>(fmin(fmax(sv + dy, 0.0), vmax))) * pitch;
   };
   NccSums a;
-  uint32_t qv[N];
+  Quad qv[N];
   {
     const uint32_t r0 = row_off(-HALF);
 #pragma unroll
-    for (int i = 0; i < N; ++i) qv[i] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, r0 + xo[i], 0, 0);
+    for (int i = 0; i < N; ++i) qv[i] = LoadQuad(rsrc, r0 + xo[i]);
   }
 #pragma unroll 1
   for (int dy = -HALF; dy <= HALF; ++dy) {
@@ -446,7 +453,7 @@ __device__ __forceinline__ Ncc NccTileMasked(const RefTile& rt, int rw, int rh,
 #pragma unroll
     for (int i = 0; i < N; ++i) {
       const double vs0 = Bilinear(qv[i], fx[i], fy);
-      qv[i] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, next + xo[i], 0, 0);
+      qv[i] = LoadQuad(rsrc, next + xo[i]);
       const bool ok = (m >> i) & 1u;
       const double vs = ok ? vs0 : 0.0;
       const double vr = ok ? trow[i] : 0.0;
@@ -649,20 +656,16 @@ __global__ void k_pm_output(int P, const double* __restrict__ inv, const double*
   if ((threadIdx.x & 63) == 0 && m) atomicAdd(filled, static_cast<unsigned int>(__popcll(m)));
 }
 
-// Quads of an image for SampleQuad: texel (y, x) = I(y, x) | I(y, x1) << 8 |
-// I(y1, x) << 16 | I(y1, x1) << 24, x1 = min(x + 1, w - 1), y1 = min(y + 1, h - 1).
-__global__ void k_make_quads(const uint8_t* __restrict__ img, int w, int h,
-                             uint32_t* __restrict__ quad) {
+// Quads of an image for SampleQuad: texel (y, x) = MakeQuad(I(y, x), I(y, x1),
+// I(y1, x), I(y1, x1)), x1 = min(x + 1, w - 1), y1 = min(y + 1, h - 1).
+__global__ void k_make_quads(const uint8_t* __restrict__ img, int w, int h, Quad* __restrict__ quad) {
   const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= static_cast<size_t>(w) * h) return;
   const int x = static_cast<int>(i % w), y = static_cast<int>(i / w);
   const int x1 = min(x + 1, w - 1), y1 = min(y + 1, h - 1);
   const uint8_t* r0 = img + static_cast<size_t>(y) * w;
   const uint8_t* r1 = img + static_cast<size_t>(y1) * w;
-  quad[static_cast<size_t>(y) * w + x] = static_cast<uint32_t>(r0[x]) |
-                                         static_cast<uint32_t>(r0[x1]) << 8 |
-                                         static_cast<uint32_t>(r1[x]) << 16 |
-                                         static_cast<uint32_t>(r1[x1]) << 24;
+  quad[static_cast<size_t>(y) * w + x] = MakeQuad(r0[x], r0[x1], r1[x], r1[x1]);
 }
 
 __device__ int Solve3(double A[3][3], double b[3], double x[3]) {
@@ -968,12 +971,12 @@ int32_t Upload(recon_mvs_hip_context* ctx, const recon_mvs_problem* pr,
     size_t qtotal = 0;
     for (int k = 0; k < pr->num_src; ++k) {
       qoff[k] = qtotal;
-      qtotal += ((static_cast<size_t>(pr->src_w[k]) * pr->src_h[k] * 4 + 255) & ~static_cast<size_t>(255));
+      qtotal += ((static_cast<size_t>(pr->src_w[k]) * pr->src_h[k] * kQuadBytes + 255) & ~static_cast<size_t>(255));
     }
     auto* q = static_cast<uint8_t*>(Ensure(ctx, kSlotSrcImg, qtotal));
     if (!q) return Fail(ctx, RECON_MVS_ERR_OUT_OF_MEMORY, "device allocation failed");
     for (int k = 0; k < pr->num_src; ++k) {
-      auto* qk = reinterpret_cast<uint32_t*>(q + qoff[k]);
+      auto* qk = reinterpret_cast<Quad*>(q + qoff[k]);
       hipLaunchKernelGGL(k_make_quads, dim3(Blocks(static_cast<size_t>(pr->src_w[k]) * pr->src_h[k], 256)),
                          dim3(256), 0,
                          ctx->stream, img + off[k], pr->src_w[k], pr->src_h[k], qk);
