@@ -129,17 +129,24 @@ RECON_MVS_API int32_t recon_mvs_hip_create(int32_t device, recon_mvs_hip_context
   auto* ctx = new (std::nothrow) recon_mvs_hip_context();
   if (!ctx) return RECON_MVS_ERR_OUT_OF_MEMORY;
   ctx->device = device;
+  // both streams at the default priority
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
     delete ctx;
     return RECON_MVS_ERR_GPU_FAILURE;
   }
+  if (hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking) != hipSuccess) {
+    ctx->copy_stream = nullptr;
+    (void)recon_mvs_hip_destroy(ctx);
+    return RECON_MVS_ERR_GPU_FAILURE;
+  }
   for (int i = 0; i < 2; ++i) {
-    if (hipEventCreateWithFlags(&ctx->pinned_ready[i], hipEventDisableTiming) != hipSuccess) {
-      (void)hipStreamDestroy(ctx->stream);
-      delete ctx;
+    if (hipEventCreateWithFlags(&ctx->pinned_ready[i], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->tab_free[i], hipEventDisableTiming) != hipSuccess) {
+      (void)recon_mvs_hip_destroy(ctx);
       return RECON_MVS_ERR_GPU_FAILURE;
     }
     (void)hipEventRecord(ctx->pinned_ready[i], ctx->stream);
+    (void)hipEventRecord(ctx->tab_free[i], ctx->stream);
   }
   *out = ctx;
   return RECON_MVS_OK;
@@ -149,15 +156,18 @@ RECON_MVS_API int32_t recon_mvs_hip_destroy(recon_mvs_hip_context* ctx) {
   if (!ctx) return RECON_MVS_OK;
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->copy_stream) (void)hipStreamSynchronize(ctx->copy_stream);
   DestroyPlaneState(ctx);
   for (auto& b : ctx->slots)
     if (b.ptr) (void)hipFree(b.ptr);
   for (int i = 0; i < 2; ++i) {
     if (ctx->pinned[i]) (void)hipHostFree(ctx->pinned[i]);
     if (ctx->pinned_ready[i]) (void)hipEventDestroy(ctx->pinned_ready[i]);
+    if (ctx->tab_free[i]) (void)hipEventDestroy(ctx->tab_free[i]);
   }
   TimerCollect(ctx);
   for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
+  if (ctx->copy_stream) (void)hipStreamDestroy(ctx->copy_stream);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return RECON_MVS_OK;

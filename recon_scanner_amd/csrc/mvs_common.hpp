@@ -83,6 +83,10 @@ struct recon_mvs_hip_context {
   void* pinned[2] = {nullptr, nullptr};
   size_t pinned_bytes[2] = {0, 0};
   hipEvent_t pinned_ready[2] = {nullptr, nullptr};
+  // uploads that overlap the kernels on `stream` (reference mode's per-pass
+  // random tables); tab_free[i]: the kernel reading device table i is done
+  hipStream_t copy_stream = nullptr;
+  hipEvent_t tab_free[2] = {nullptr, nullptr};
   // fusion result kept for recon_mvs_hip_fetch_point_cloud
   std::vector<double> cloud_pos, cloud_nrm;
   std::vector<float> cloud_conf;

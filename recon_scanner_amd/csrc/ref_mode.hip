@@ -1166,8 +1166,13 @@ RECON_MVS_API int32_t recon_mvs_hip_patchmatch_stereo(recon_mvs_hip_context* ctx
       // the previous upload from this staging buffer must have finished
       RMVS_HIP(ctx, hipEventSynchronize(ctx->pinned_ready[b]));
       FillPassTable(host, w, h, iter, dir, threads, K, config->restart_ratio);
-      RMVS_HIP(ctx, hipMemcpyAsync(tab[b], host, tab_bytes, hipMemcpyHostToDevice, ctx->stream));
-      RMVS_HIP(ctx, hipEventRecord(ctx->pinned_ready[b], ctx->stream));
+      // the upload runs on the copy stream, overlapping the previous pass;
+      // it waits for the pass that last read device table b, and this pass
+      // waits for it
+      RMVS_HIP(ctx, hipStreamWaitEvent(ctx->copy_stream, ctx->tab_free[b], 0));
+      RMVS_HIP(ctx, hipMemcpyAsync(tab[b], host, tab_bytes, hipMemcpyHostToDevice, ctx->copy_stream));
+      RMVS_HIP(ctx, hipEventRecord(ctx->pinned_ready[b], ctx->copy_stream));
+      RMVS_HIP(ctx, hipStreamWaitEvent(ctx->stream, ctx->pinned_ready[b], 0));
       TimerBegin(ctx, kTimerPass, &eb, &ee);
       WithHalf(config->patch_size, [&](auto hc) {
         hipLaunchKernelGGL(k_pm_pass<decltype(hc)::value>, dim3(tiles), dim3(kTileThreads), tile_lds,
@@ -1177,6 +1182,7 @@ RECON_MVS_API int32_t recon_mvs_hip_patchmatch_stereo(recon_mvs_hip_context* ctx
       });
       RMVS_LAUNCH_CHECK(ctx);
       TimerEnd(ctx, kTimerPass, eb, ee);
+      RMVS_HIP(ctx, hipEventRecord(ctx->tab_free[b], ctx->stream));
       std::swap(invA, invB);
       std::swap(costA, costB);
       std::swap(supA, supB);
