@@ -219,11 +219,13 @@ def test_reference_unit_test_scene():
     assert cloud.size() > 1000
 
 
-@pytest.mark.parametrize("patch", [9, 4])
+@pytest.mark.parametrize("patch", [9, 4, 10, 6])
 def test_ragged_tiles_and_source_sizes(patch):
     """Reference and sources off the pixel kernels' 32x4 tile grid (101x67,
     sources cropped to other sizes: the quads of each source carry its own
-    edge clamps), an odd and an even patch, PatchMatchStereo then
+    edge clamps), odd and even patches on the generic kernels (9, 4) and on
+    the unrolled ones (10 and 6 share 11x11 and 7x7's half-width, with their
+    own validity threshold patch^2 / 2), PatchMatchStereo then
     RefineSubpixel, against the oracle bit for bit."""
     say("ragged tiles, patch", patch)
     sc = scenes.sphere_arc(width=101, height=67, focal=160.0)
