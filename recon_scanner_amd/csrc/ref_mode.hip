@@ -54,6 +54,11 @@ struct RefProblem {
 constexpr int kTileW = 32;
 constexpr int kTileH = 4;
 constexpr int kTileThreads = kTileW * kTileH;
+// tiles per block of the blocked tile order (LoadRefTile): 64 x 128 pixels
+// (measured at 1280x960 against 8 sources: 262 ms per view; whole tile rows
+// 283, 256 x 64 268, 128 x 128 265, 64 x 256 262)
+constexpr int kBlockW = 2;
+constexpr int kBlockH = 32;
 
 // SampleBilinear (patchmatch_stereo.cpp:78-100): integer coordinates are pixel
 // centres, out-of-range samples are rejected, not clamped.
@@ -275,9 +280,20 @@ __device__ __forceinline__ int XcdTile() {
 
 __device__ RefTile LoadRefTile(const RefProblem& p, double* lds) {
   const int half = p.patch / 2;
-  const int tiles_x = (p.rw + kTileW - 1) / kTileW;
+  const int tiles_x = (p.rw + kTileW - 1) / kTileW, tiles_y = (p.rh + kTileH - 1) / kTileH;
   const int t = XcdTile();
-  const int x0 = (t % tiles_x) * kTileW, y0 = (t / tiles_x) * kTileH;
+  // Blocked order: bands of kBlockH tile rows, each walked in blocks of
+  // kBlockW tile columns, raster inside a block (the last band and the last
+  // block of a band may be smaller).  An XCD's consecutive tiles then cover
+  // a compact image region, whose windows in the sources along any epipolar
+  // direction stay closer than those of whole tile rows.
+  const int band = t / (tiles_x * kBlockH);
+  const int u = t - band * tiles_x * kBlockH;
+  const int bh = min(kBlockH, tiles_y - band * kBlockH);
+  const int blk = u / (kBlockW * bh);
+  const int v = u - blk * kBlockW * bh;
+  const int bw = min(kBlockW, tiles_x - blk * kBlockW);
+  const int x0 = (blk * kBlockW + v % bw) * kTileW, y0 = (band * kBlockH + v / bw) * kTileH;
   const int tw = kTileW + 2 * half, th = kTileH + 2 * half;
   for (int i = threadIdx.x; i < tw * th; i += kTileThreads) {
     const int X = x0 - half + i % tw;
