@@ -103,10 +103,13 @@ def cpu_baseline(sc, iterations, target_s):
         err = float(np.median(np.abs(d[m] - truth[m]))) if m.any() else float("nan")
         return dt, err
 
-    # calibrate on a small crop (large enough that thread start-up is a
-    # small part of it), then size the sample to ~target_s
-    dt, _ = run(160, 120, 1)
-    per_unit = dt / (160 * 120 * 8 * 1)
+    # calibrate on a crop tall enough for every host thread (the reference
+    # splits rows into min(threads, rows / 16) blocks), then size the sample
+    # to ~target_s
+    ch0 = min(H, max(120, 16 * cores))
+    cw0 = min(W, max(160, ch0 * 4 // 3))
+    dt, _ = run(cw0, ch0, 1)
+    per_unit = dt / (cw0 * ch0 * 8 * 1)
     area = max(64 * 48, min(W * H, int(target_s / (per_unit * 8 * iterations))))
     ch = max(48, min(H, int((area * 3 / 4) ** 0.5)))
     cw = max(64, min(W, int(ch * 4 / 3)))
