@@ -125,7 +125,9 @@ typedef struct recon_mvs_cost_result {
  * One reference view and its sources, already resolved through the index
  * contract of mvs.h:151-161 (src_cameras[k] pairs src_pixels[k]).  Pixels are
  * host pointers in `pixel_format`; the reference-mode entry points accept
- * RECON_MVS_PIXEL_U8 only, as the reference does.
+ * RECON_MVS_PIXEL_U8 only, as the reference does, at most 32 sources, patch
+ * sizes 1..63 and images of at most 2^29 pixels (32-bit indexing);
+ * anything else returns RECON_MVS_ERR_INVALID_ARGUMENT.
  */
 typedef struct recon_mvs_problem {
   const void* ref_pixels;
