@@ -67,6 +67,9 @@ def parse():
     ap.add_argument("--tile-schedule", type=int, default=-1,
                     help="workgroup tile schedule; -1 = autotuned with the per-phase occupancy "
                          "(during the warm-up step)")
+    ap.add_argument("--check-size", default="320x240",
+                    help="reference workload: size (WxH) of the view run by the reference's CPU "
+                         "code as the baseline and compared with the GPU bit for bit")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target duration of the CPU-baseline sample (0 = skip)")
     return ap.parse_args()
@@ -250,9 +253,10 @@ def run_reference(args):
     reference's algorithm, bit-exact) of one view of the sphere + planes scene
     against its 8 sources.  Throughput on the GPU at 1280x960 (or --width x
     --height); the reference's own CPU code (same config and num_threads, whose
-    row-block RNG streams the GPU reproduces) on a 320x240 view as the
-    baseline, where the GPU's outputs are also checked against it bit for bit
-    (the CPU takes ~30 s there on 16 cores)."""
+    row-block RNG streams the GPU reproduces) on a 320x240 view (or
+    --check-size) as the baseline, where the GPU's outputs are also checked
+    against it bit for bit (the CPU takes ~20 s there on 16 cores, ~80 s at
+    640x480)."""
     import oracle
     from recon_scanner_amd import mvs, scenes
     cores = len(os.sched_getaffinity(0))
@@ -324,7 +328,8 @@ def run_reference(args):
                        "mean_launch_ms": pass_ms, "flop_per_launch": flop,
                        "hypotheses_per_pixel": hyps}
     if args.cpu_seconds > 0 and oracle.reference_available():
-        sc, u8, cfg = problem(320, 240)
+        cw, ch = (int(x) for x in args.check_size.lower().split("x"))
+        sc, u8, cfg = problem(cw, ch)
         filled, refined, dm = gpu_once(sc, u8, cfg)
         c0 = time.perf_counter()
         rf, rd, rs, rc = oracle.ref_patchmatch(u8[0], [u8[i] for i in srcs], sc.cams[0], sc.cams,
@@ -333,16 +338,16 @@ def run_reference(args):
                                          srcs, cfg, rd, rs, rc)
         cdt = time.perf_counter() - c0
         exact = (rf == filled and rn == refined and
-                 dm.depth.reshape(240, 320).tobytes() == rdr.tobytes() and
-                 dm.support_count.reshape(240, 320).tobytes() == rs.tobytes() and
-                 dm.confidence.reshape(240, 320).tobytes() == rcr.tobytes())
-        cpu_rate = 320 * 240 * 8 * args.iterations / cdt / 1e9
+                 dm.depth.reshape(ch, cw).tobytes() == rdr.tobytes() and
+                 dm.support_count.reshape(ch, cw).tobytes() == rs.tobytes() and
+                 dm.confidence.reshape(ch, cw).tobytes() == rcr.tobytes())
+        cpu_rate = cw * ch * 8 * args.iterations / cdt / 1e9
         rec["cpu_baseline"] = {"value": cpu_rate, "unit": "Gpix*view*iter/s", "cores": cores,
                                "kind": "reference",
                                "sample": f"reference PatchMatchStereo + RefineSubpixel compiled from "
-                                         f"its sources, the same scene and config at 320x240; {cdt:.1f} s",
+                                         f"its sources, the same scene and config at {cw}x{ch}; {cdt:.1f} s",
                                "seconds": cdt}
-        rec["bit_exact_with_reference_320x240"] = bool(exact)
+        rec[f"bit_exact_with_reference_{cw}x{ch}"] = bool(exact)
         rec["speedup_vs_reference_cpu"] = rec["value"] / cpu_rate
     print(json.dumps(rec))
 
