@@ -15,6 +15,14 @@
 // (lines 647-649), consumed in the pass's scan order; the host replays those
 // streams into a per-pass table (draws are data-independent), so the GPU
 // result equals the reference's for the same config.num_threads.
+//
+// The pixel kernels (initial costs, passes, refinement) change only where the
+// values come from, never an operation: a workgroup owns a 32x4 tile whose
+// reference patch samples are computed once into LDS; sources are read as
+// 4-tap quads (one load per sample); a wave whose windows all lie inside the
+// images takes a path without per-sample tests and with the reference sums
+// precomputed per pixel, the others a masked path in which an invalid pair
+// adds +0.0 to every sum.
 #include <algorithm>
 #include <cstring>
 #include <thread>
