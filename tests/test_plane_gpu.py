@@ -368,3 +368,24 @@ def test_u8_images_half_storage_bit_identical():
             for k in a:
                 assert a[k].tobytes() == b[k].tobytes(), (storage, k)
     assert np.isfinite(res["f32"][0]["depth"]).mean() > 0.9
+
+
+def test_tuned_plan_bit_identical():
+    """tile_schedule = -1 with occupancy_cap = 0 (the tuned schedule and
+    per-phase occupancy caps) on 8-bit images in FP16: the results equal
+    every fixed schedule's bit for bit."""
+    say("tuned plan vs fixed schedules, sphere_planes 480x360 u8/f16, 3 iterations")
+    sc = scenes.sphere_planes(width=480, height=360, focal=625.0, supersample=1)
+    u8 = [sc.u8(i) for i in range(sc.num_views)]
+    srcs = list(range(1, 9))
+    pm = plane.PlaneMVS(sc.cams, u8, storage="f16")
+    base = plane.default_params(patch_size=11, window_stride=2, top_k=4, iterations=3,
+                                min_depth_m=0.25, max_depth_m=0.60, value_scale=1.0)
+    outs = []
+    for sched in (-1, 0, 1, 2, -1):
+        pm.patchmatch(0, srcs, base.copy(tile_schedule=sched, occupancy_cap=0))
+        outs.append(pm.depth_map(0))
+    for o in outs[1:]:
+        for k in ("depth", "normals", "cost", "support"):
+            assert np.array_equal(o[k], outs[0][k], equal_nan=True), k
+    assert np.isfinite(outs[0]["depth"]).mean() > 0.9
