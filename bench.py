@@ -324,7 +324,8 @@ def run_reference(args):
                 traffic = float(line.split()[-1])
     rec["roofline"] = {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
                        "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS if achieved else None,
-                       "traffic": traffic, "kernel": "k_pm_pass (one directional pass)",
+                       "traffic": traffic,
+                       "kernel": "one directional pass: k_pm_pass + k_pm_restart",
                        "mean_launch_ms": pass_ms, "flop_per_launch": flop,
                        "hypotheses_per_pixel": hyps}
     if args.cpu_seconds > 0 and oracle.reference_available():

@@ -67,6 +67,7 @@ enum Slot : int {
   kSlotAdjOut,
   kSlotAdjPart,
   kSlotPhase,
+  kSlotRestart,
   kSlotCount
 };
 

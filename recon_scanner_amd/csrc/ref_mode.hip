@@ -588,7 +588,8 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(2)
                           double* __restrict__ out_inv, double* __restrict__ out_cost,
                           int* __restrict__ out_sup, const uint32_t* __restrict__ tab, int K,
                           double ratio, double restart_ratio, double inv_min, double inv_max,
-                          int propagate) {
+                          int propagate, uint32_t* __restrict__ restart_list,
+                          unsigned int* __restrict__ restart_count) {
   extern __shared__ double lds[];
   const int P = p.rw * p.rh;
   const RefTile rt = LoadRefTile(p, lds);
@@ -634,20 +635,70 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(2)
     }
     radius *= 0.5;
   }
-  const double ur = static_cast<double>(tab[static_cast<size_t>(K) * P + idx]) / 4294967296.0;
-  if (ur < restart_ratio) {
-    const double uv = static_cast<double>(tab[static_cast<size_t>(K + 1) * P + idx]) / 4294967296.0;
-    const double cand = inv_min + uv * (inv_max - inv_min);
-    const Cost c = EvaluateTile<HALF>(p, rt, ray, cand);
-    if (c.cost < best_c) {
-      best_c = c.cost;
-      best_i = cand;
-      best_s = c.valid_views;
-    }
-  }
   out_inv[idx] = best_i;
   out_cost[idx] = best_c;
   out_sup[idx] = best_s;
+  // The restart candidate (drawn by ~restart_ratio of the pixels) is the
+  // pass's last step: the pixel goes to a list that k_pm_restart evaluates
+  // densely, instead of whole waves running it for a few lanes.  One atomic
+  // per wave reserves the wave's entries.
+  const double ur = static_cast<double>(tab[static_cast<size_t>(K) * P + idx]) / 4294967296.0;
+  const bool restart = ur < restart_ratio;
+  const unsigned long long m = __ballot(restart);
+  if (m) {
+    const int lane = static_cast<int>(__lane_id());
+    const int leader = __ffsll(static_cast<long long>(m)) - 1;
+    unsigned int base = 0;
+    if (lane == leader) base = atomicAdd(restart_count, static_cast<unsigned int>(__popcll(m)));
+    base = __shfl(base, leader);
+    if (restart)
+      restart_list[base + static_cast<unsigned int>(__popcll(m & ((1ull << lane) - 1ull)))] =
+          static_cast<uint32_t>(idx);
+  }
+}
+
+// NccU8 with the source sampled from its quads (same values).
+__device__ Ncc NccU8Quad(const uint8_t* __restrict__ a, int aw, int ah, const Quad* __restrict__ q,
+                         int bw, int bh, double ru, double rv, double su, double sv, int patch) {
+  const int half = patch / 2;
+  NccSums acc;
+  for (int dy = -half; dy <= half; ++dy) {
+    for (int dx = -half; dx <= half; ++dx) {
+      double vr = 0.0, vs = 0.0;
+      if (!SampleU8(a, aw, ah, ru + dx, rv + dy, &vr)) continue;
+      if (!SampleQuad(q, bw, bh, su + dx, sv + dy, &vs)) continue;
+      acc.add(vr, vs);
+    }
+  }
+  return FinishNcc(acc, patch);
+}
+
+// The restart step of a pass (patchmatch_stereo.cpp:707-719) for the pixels
+// k_pm_pass listed, against the state it wrote.  Any grid: every thread
+// strides over the list up to the count the pass left.
+__global__ void k_pm_restart(RefProblem p, const uint32_t* __restrict__ list,
+                             const unsigned int* __restrict__ count, const uint32_t* __restrict__ tab,
+                             int K, double inv_min, double inv_max, double* __restrict__ inv,
+                             double* __restrict__ cost, int* __restrict__ sup) {
+  const size_t P = static_cast<size_t>(p.rw) * p.rh;
+  const unsigned int n = *count;
+  for (unsigned int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t idx = list[i];
+    const int x = static_cast<int>(idx % p.rw), y = static_cast<int>(idx / p.rw);
+    const double u = x + 0.5, v = y + 0.5;
+    double ray[3];
+    PixelToRay(p.cams[0], u, v, ray);
+    const double uv = static_cast<double>(tab[(K + 1) * P + idx]) / 4294967296.0;
+    const double cand = inv_min + uv * (inv_max - inv_min);
+    const Cost c = EvaluateWith(p, ray, cand, [&](int k, double su, double sv) {
+      return NccU8Quad(p.ref, p.rw, p.rh, p.srcq[k], p.sw[k], p.sh[k], u, v, su, sv, p.patch);
+    });
+    if (c.cost < cost[idx]) {
+      cost[idx] = c.cost;
+      inv[idx] = cand;
+      sup[idx] = c.valid_views;
+    }
+  }
 }
 
 // Output stage (patchmatch_stereo.cpp:746-763).
@@ -1163,9 +1214,16 @@ RECON_MVS_API int32_t recon_mvs_hip_patchmatch_stereo(recon_mvs_hip_context* ctx
   uint8_t* d_sup = static_cast<uint8_t*>(Ensure(ctx, kSlotOutSupport, P));
   float* d_conf = static_cast<float*>(Ensure(ctx, kSlotOutConf, P * sizeof(float)));
   unsigned int* d_cnt = static_cast<unsigned int*>(Ensure(ctx, kSlotCounter, 64));
+  // restart list of a pass: its count, then up to P pixel indices
+  auto* restart = static_cast<uint32_t*>(Ensure(ctx, kSlotRestart, (P + 64) * sizeof(uint32_t)));
   if (!invA || !invB || !costA || !costB || !supA || !supB || !tab[0] || !tab[1] || !d_depth ||
-      !d_sup || !d_conf || !d_cnt)
+      !d_sup || !d_conf || !d_cnt || !restart)
     return Fail(ctx, RECON_MVS_ERR_OUT_OF_MEMORY, "device allocation failed");
+  unsigned int* restart_count = restart;
+  uint32_t* restart_list = restart + 64;
+  int restart_blocks = 0;
+  RMVS_HIP(ctx, hipDeviceGetAttribute(&restart_blocks, hipDeviceAttributeMultiprocessorCount, ctx->device));
+  restart_blocks = std::max(1, restart_blocks) * 4;
 
   RMVS_HIP(ctx, hipMemcpyAsync(invA, inv0.data(), P * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
   const unsigned tiles = RefTiles(w, h);
@@ -1198,12 +1256,17 @@ RECON_MVS_API int32_t recon_mvs_hip_patchmatch_stereo(recon_mvs_hip_context* ctx
       RMVS_HIP(ctx, hipEventRecord(ctx->pinned_ready[b], ctx->copy_stream));
       RMVS_HIP(ctx, hipStreamWaitEvent(ctx->stream, ctx->pinned_ready[b], 0));
       TimerBegin(ctx, kTimerPass, &eb, &ee);
+      RMVS_HIP(ctx, hipMemsetAsync(restart_count, 0, sizeof(unsigned int), ctx->stream));
       WithHalf(config->patch_size, [&](auto hc) {
         hipLaunchKernelGGL(k_pm_pass<decltype(hc)::value>, dim3(tiles), dim3(kTileThreads), tile_lds,
                            ctx->stream, dp, invA, costA, supA, invB, costB, supB, tab[b], K,
                            config->random_search_radius_ratio, config->restart_ratio, inv_min,
-                           inv_max, config->propagation_enabled ? 1 : 0);
+                           inv_max, config->propagation_enabled ? 1 : 0, restart_list,
+                           restart_count);
       });
+      RMVS_LAUNCH_CHECK(ctx);
+      hipLaunchKernelGGL(k_pm_restart, dim3(restart_blocks), dim3(256), 0, ctx->stream, dp, restart_list,
+                         restart_count, tab[b], K, inv_min, inv_max, invB, costB, supB);
       RMVS_LAUNCH_CHECK(ctx);
       TimerEnd(ctx, kTimerPass, eb, ee);
       RMVS_HIP(ctx, hipEventRecord(ctx->tab_free[b], ctx->stream));
