@@ -308,8 +308,9 @@ def run_reference(args):
     d = dm.depth.reshape(h, w)
     m = np.isfinite(d) & (truth > 0)
     rec["median_depth_error_mm"] = float(np.median(np.abs(d[m] - truth[m]))) * 1000.0
-    # roofline of one directional pass (k_pm_pass, HIP events on the
-    # library's stream): nominal hypotheses per pixel and pass H = 4
+    # roofline of one directional pass (k_pm_pass and its k_pm_restart, timed
+    # together by HIP events on the library's stream): nominal hypotheses per
+    # pixel and pass H = 4
     # neighbours + random_search_count + restart_ratio, each over 8 views x
     # patch^2 sample pairs of REF_FLOP_PER_PAIR FP64 operations
     pass_ms = ms[0] / cnt[0] if cnt[0] else None
