@@ -605,12 +605,28 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(2)
   if (propagate) {
     const int nbx[4] = {-1, 1, 0, 0};
     const int nby[4] = {0, 0, -1, 1};
+    // A candidate equal to the current best, or to a neighbour already
+    // evaluated here, cannot win: its cost is the same function of (pixel,
+    // inverse depth) as the one it would be compared with (the state always
+    // holds the cost of its own inverse depth), and the comparison is
+    // strict.  Propagation spreads identical values, so converged regions
+    // skip many of these evaluations; the result is unchanged.
+    double seen[4];
+    int nseen = 0;
     for (int k = 0; k < 4; ++k) {
       const int nx = x + nbx[k], ny = y + nby[k];
       if (nx < 0 || ny < 0 || nx >= p.rw || ny >= p.rh) continue;
       const int nidx = ny * p.rw + nx;
       if (snap_cost[nidx] >= 1e29) continue;
       const double cand = snap_inv[nidx];
+      bool dup = cand == best_i;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) dup |= q < nseen && seen[q] == cand;
+      if (dup) continue;
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if (q == nseen) seen[q] = cand;
+      ++nseen;
       const Cost c = EvaluateTile<HALF>(p, rt, ray, cand);
       if (c.cost < best_c) {
         best_c = c.cost;
