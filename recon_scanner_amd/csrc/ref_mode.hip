@@ -881,6 +881,10 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(2)
       // reference evaluates the remaining ones and discards them).
       double s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, t0 = 0, t1 = 0, t2 = 0;
       bool fit_ok = true;
+      // the middle fit point is inv0 + 0.0 = inv0: its cost is the cost at
+      // inv0 the reference evaluates next, reused when the scan got there
+      bool have_c0 = false;
+      double c0 = 0.0;
       for (int k = 0; k < fit_points; ++k) {
         const double d = inv0 + (k - half_fit) * fit_step;
         if (d < inv_min || d > inv_max) {
@@ -888,6 +892,10 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(2)
           break;
         }
         const double c = EvaluateTile<HALF>(p, rt, ray, d).cost;
+        if (k == half_fit) {
+          have_c0 = true;
+          c0 = c;
+        }
         if (c >= 1e29) {
           fit_ok = false;
           break;
@@ -903,7 +911,7 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(2)
         t2 += xx * xx * c;
       }
       double best_d = inv0;
-      double best_c = EvaluateTile<HALF>(p, rt, ray, inv0).cost;
+      double best_c = have_c0 ? c0 : EvaluateTile<HALF>(p, rt, ray, inv0).cost;
       if (fit_ok) {
         double A[3][3] = {{s4, s3, s2}, {s3, s2, s1}, {s2, s1, s0}};
         double b[3] = {t2, t1, t0};
