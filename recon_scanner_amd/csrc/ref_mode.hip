@@ -306,8 +306,8 @@ __device__ RefTile LoadRefTile(const RefProblem& p, double* lds) {
   for (int i = threadIdx.x; i < tw * th; i += kTileThreads) {
     const int X = x0 - half + i % tw;
     const int Y = y0 - half + i / tw;
-    double v = 0.0;
-    lds[i] = SampleU8(p.ref, p.rw, p.rh, X + 0.5, Y + 0.5, &v) ? v : __builtin_nan("");
+    double val = 0.0;
+    lds[i] = SampleU8(p.ref, p.rw, p.rh, X + 0.5, Y + 0.5, &val) ? val : __builtin_nan("");
   }
   __syncthreads();
   RefTile r;
@@ -340,12 +340,6 @@ __device__ RefTile LoadRefTile(const RefProblem& p, double* lds) {
   return r;
 }
 
-// NccTile when every sample pair is valid: the reference patch inside the
-// reference image (rt.full) and the source window inside the source.  Then
-// no sample is skipped, s_r / s_r2 are the precomputed rt.rs / rt.rs2, and
-// each source sample is SampleQuad's arithmetic without its bounds tests:
-// a non-negative coordinate truncates to its floor.  HALF = patch / 2 when
-// known at compile time (unrolled rows), -1 otherwise.
 // NccTile when every sample pair is valid: the reference patch inside the
 // reference image (rt.full) and the source window inside the source.  Then
 // no sample is skipped, s_r / s_r2 are the precomputed rt.rs / rt.rs2, and
