@@ -219,9 +219,19 @@ struct Cost {
 
 // EvaluateCandidate with the reference ray precomputed by the caller; `ncc(k,
 // su, sv)` is the NCC of the reference patch against source k at (su, sv).
+//
+// reject_above: a caller that only asks whether the cost is below some c
+// passes c + kRejectMargin, and gets cost 1e30 as soon as the views left
+// cannot bring the cost down to it.  Each view's NCC is at most 1, so with
+// S the NCC sum over the n valid views so far and R views left the final
+// cost is at least 1 - (S + R) / (n + R) (that bound grows with the number
+// of valid views among the R); the margin, many orders above the rounding
+// of these sums, keeps the test on the safe side.
+constexpr double kRejectMargin = 1e-9;
+
 template <class NccFn>
 __device__ __forceinline__ Cost EvaluateWith(const RefProblem& p, const double* ray, double inv,
-                                             NccFn ncc) {
+                                             NccFn ncc, double reject_above = 1e300) {
   Cost r{1e30, 0.0, 0.0, 0, 0};
   if (inv <= 1e-9) return r;
   const double depth = 1.0 / inv;
@@ -230,6 +240,9 @@ __device__ __forceinline__ Cost EvaluateWith(const RefProblem& p, const double* 
   double pw[3];
   CamToWorld(p.cams[0], pc, pw);
   for (int k = 0; k < p.nsrc; ++k) {
+    const double rem = static_cast<double>(p.nsrc - k);
+    if (k > 0 && 1.0 - (r.ncc_sum + rem) / (r.weight_sum + rem) > reject_above)
+      return Cost{1e30, 0.0, 0.0, 0, 0};
     const recon_mvs_camera& sc = p.cams[1 + k];
     double ps[3];
     WorldToCam(sc, pw, ps);
@@ -494,7 +507,8 @@ __device__ __forceinline__ Ncc NccTileMasked(const RefTile& rt, int rw, int rh,
 // the per-sample tests.
 template <int HALF>
 __device__ __forceinline__ Cost EvaluateTile(const RefProblem& p, const RefTile& rt,
-                                             const double* ray, double inv) {
+                                             const double* ray, double inv,
+                                             double reject_above = 1e300) {
   return EvaluateWith(p, ray, inv, [&](int k, double su, double sv) {
     const int half = HALF >= 0 ? HALF : p.patch / 2;
     const int bw = p.sw[k], bh = p.sh[k];
@@ -506,7 +520,7 @@ __device__ __forceinline__ Cost EvaluateTile(const RefProblem& p, const RefTile&
       return NccTileMasked<HALF>(rt, p.rw, p.rh, p.srcq[k], bw, bh, su, sv, p.patch);
     else
       return NccTile(rt.t, rt.tw, rt.ly, rt.lx, p.srcq[k], bw, bh, su, sv, p.patch);
-  });
+  }, reject_above);
 }
 
 inline unsigned RefTiles(int w, int h) {
@@ -621,7 +635,7 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(2)
       for (int q = 0; q < 4; ++q)
         if (q == nseen) seen[q] = cand;
       ++nseen;
-      const Cost c = EvaluateTile<HALF>(p, rt, ray, cand);
+      const Cost c = EvaluateTile<HALF>(p, rt, ray, cand, best_c + kRejectMargin);
       if (c.cost < best_c) {
         best_c = c.cost;
         best_i = cand;
@@ -637,7 +651,7 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(2)
       radius *= 0.5;
       continue;
     }
-    const Cost c = EvaluateTile<HALF>(p, rt, ray, cand);
+    const Cost c = EvaluateTile<HALF>(p, rt, ray, cand, best_c + kRejectMargin);
     if (c.cost < best_c) {
       best_c = c.cost;
       best_i = cand;
