@@ -931,7 +931,8 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(2)
             if (vertex >= -static_cast<double>(half_fit) && vertex <= static_cast<double>(half_fit)) {
               // The reference adds the vertex in sample-index units (line 898).
               const double cand = inv0 + vertex;
-              const double cc = EvaluateTile<HALF>(p, rt, ray, cand).cost;
+              // compared with best_c only: rejected early when it cannot win
+              const double cc = EvaluateTile<HALF>(p, rt, ray, cand, best_c + kRejectMargin).cost;
               if (cc < best_c) {
                 best_c = cc;
                 best_d = cand;
@@ -966,7 +967,7 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(2)
           }
         }
         const double cm = 0.5 * (lo + hi);
-        const double fm = EvaluateTile<HALF>(p, rt, ray, cm).cost;
+        const double fm = EvaluateTile<HALF>(p, rt, ray, cm, best_c + kRejectMargin).cost;
         if (fm < best_c) {
           best_c = fm;
           best_d = cm;
