@@ -5,7 +5,8 @@
 //   ComputeNcc        patchmatch_stereo.cpp:214-274  -> NccU8 / k_compute_ncc
 //   EvaluateCandidate patchmatch_stereo.cpp:344-418  -> Evaluate / k_evaluate
 //   PatchMatchStereo  patchmatch_stereo.cpp:422-764  -> k_pm_init_cost, k_pm_pass,
-//                                                      k_pm_output + host driver
+//                                                      k_pm_restart, k_pm_output
+//                                                      + host driver
 //   RefineSubpixel    patchmatch_stereo.cpp:770-979  -> k_refine
 //
 // The reference's directional passes read neighbours only from a snapshot of
