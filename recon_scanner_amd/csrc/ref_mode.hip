@@ -698,21 +698,23 @@ __device__ Ncc NccU8Quad(const uint8_t* __restrict__ a, int aw, int ah, const Qu
   return FinishNcc(acc, patch);
 }
 
-// Lanes per listed pixel in k_pm_restart: the pixel's source views are dealt
-// round robin to them (view k to lane k % kRestartLanes, slot k / kRestartLanes).
-constexpr int kRestartLanes = 8;
-constexpr int kRestartSlots = kMaxSrc / kRestartLanes;
+// Lanes per listed pixel in k_pm_restart: the pixel's source views are
+// evaluated in rounds of kRestartLanes, view base + j on lane j.
+constexpr int kRestartLanes = 4;
 
 // The restart step of a pass (patchmatch_stereo.cpp:707-719) for the pixels
-// k_pm_pass listed, against the state it wrote: EvaluateCandidate with its
-// views computed by a group of kRestartLanes lanes and combined by the
-// group's first lane in view order, as EvaluateWith does.  Any grid of whole
-// groups: each group strides over the list up to the count the pass left.
+// k_pm_pass listed, against the state it wrote: EvaluateCandidate with each
+// round of views computed by a group of kRestartLanes lanes and accumulated
+// in view order by every lane of the group (EvaluateWith's order).  Between
+// rounds the early-rejection bound of EvaluateWith against the stored cost
+// stops a candidate that cannot win (a random restart rarely can).  Any grid
+// of whole groups: each group strides over the list up to the count the
+// pass left.
 __global__ void k_pm_restart(RefProblem p, const uint32_t* __restrict__ list,
                              const unsigned int* __restrict__ count, const uint32_t* __restrict__ tab,
                              int K, double inv_min, double inv_max, double* __restrict__ inv,
                              double* __restrict__ cost, int* __restrict__ sup) {
-  static_assert(64 % kRestartLanes == 0 && kMaxSrc % kRestartLanes == 0, "groups within a wave");
+  static_assert(64 % kRestartLanes == 0, "groups within a wave");
   const size_t P = static_cast<size_t>(p.rw) * p.rh;
   const unsigned int n = *count;
   const unsigned int groups = gridDim.x * blockDim.x / kRestartLanes;
@@ -726,6 +728,7 @@ __global__ void k_pm_restart(RefProblem p, const uint32_t* __restrict__ list,
     PixelToRay(p.cams[0], u, v, ray);
     const double uv = static_cast<double>(tab[(K + 1) * P + idx]) / 4294967296.0;
     const double cand = inv_min + uv * (inv_max - inv_min);
+    const double best = cost[idx];
     // EvaluateWith's opening tests and world point
     bool go = !(cand <= 1e-9);
     double pw[3] = {0.0, 0.0, 0.0};
@@ -737,51 +740,54 @@ __global__ void k_pm_restart(RefProblem p, const uint32_t* __restrict__ list,
         CamToWorld(p.cams[0], pc, pw);
       }
     }
-    // this lane's views
-    int valid_bits = 0, weak_bits = 0;
-    double nccv[kRestartSlots];
-#pragma unroll
-    for (int sl = 0; sl < kRestartSlots; ++sl) {
-      nccv[sl] = 0.0;
-      const int k = j + sl * kRestartLanes;
-      if (!go || k >= p.nsrc) continue;
-      const recon_mvs_camera& sc = p.cams[1 + k];
-      double ps[3];
-      WorldToCam(sc, pw, ps);
-      if (ps[2] <= 1e-6) continue;
-      double su = 0.0, sv = 0.0;
-      if (!Project(sc, ps, &su, &sv)) continue;
-      const Ncc nn = NccU8Quad(p.ref, p.rw, p.rh, p.srcq[k], p.sw[k], p.sh[k], u, v, su, sv, p.patch);
-      if (!nn.valid) continue;
-      valid_bits |= 1 << sl;
-      if (nn.weak) weak_bits |= 1 << sl;
-      nccv[sl] = nn.ncc;
-    }
-    // combined in view order k = 0, 1, ... (every lane of the group runs the
-    // shuffles; the first lane keeps the sums)
     Cost r{1e30, 0.0, 0.0, 0, 0};
-#pragma unroll
-    for (int sl = 0; sl < kRestartSlots; ++sl) {
+    bool rejected = false;
+    for (int base = 0; go && base < p.nsrc; base += kRestartLanes) {
+      if (base > 0) {
+        const double rem = static_cast<double>(p.nsrc - base);
+        if (1.0 - (r.ncc_sum + rem) / (r.weight_sum + rem) > best + kRejectMargin) {
+          rejected = true;
+          break;
+        }
+      }
+      // this lane's view
+      const int k = base + j;
+      int valid = 0, weak = 0;
+      double nv = 0.0;
+      if (k < p.nsrc) {
+        const recon_mvs_camera& sc = p.cams[1 + k];
+        double ps[3];
+        WorldToCam(sc, pw, ps);
+        double su = 0.0, sv = 0.0;
+        if (ps[2] > 1e-6 && Project(sc, ps, &su, &sv)) {
+          const Ncc nn = NccU8Quad(p.ref, p.rw, p.rh, p.srcq[k], p.sw[k], p.sh[k], u, v, su, sv, p.patch);
+          if (nn.valid) {
+            valid = 1;
+            weak = nn.weak ? 1 : 0;
+            nv = nn.ncc;
+          }
+        }
+      }
+      // the round's views in order, on every lane of the group
       for (int jj = 0; jj < kRestartLanes; ++jj) {
-        const int k = jj + sl * kRestartLanes;
-        if (k >= p.nsrc) break;
-        const int vb = __shfl(valid_bits, leader + jj);
-        const int wb = __shfl(weak_bits, leader + jj);
-        const double nv = __shfl(nccv[sl], leader + jj);
-        if (!((vb >> sl) & 1)) continue;
+        if (base + jj >= p.nsrc) break;
+        const int vb = __shfl(valid, leader + jj);
+        const int wb = __shfl(weak, leader + jj);
+        const double nvj = __shfl(nv, leader + jj);
+        if (!vb) continue;
         ++r.valid_views;
-        if ((wb >> sl) & 1) ++r.weak;
-        r.ncc_sum += 1.0 * nv;
+        if (wb) ++r.weak;
+        r.ncc_sum += 1.0 * nvj;
         r.weight_sum += 1.0;
       }
     }
-    if (j != 0 || !go) continue;
+    if (j != 0 || !go || rejected) continue;
     if (r.weight_sum > 0.0) {
       const double mean = r.ncc_sum / r.weight_sum;
       r.cost = 1.0 - mean;
       if (r.cost > 3.0 * p.photo_threshold) r.cost = 1e30;
     }
-    if (r.cost < cost[idx]) {
+    if (r.cost < best) {
       cost[idx] = r.cost;
       inv[idx] = cand;
       sup[idx] = r.valid_views;
