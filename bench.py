@@ -55,9 +55,9 @@ def parse():
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1440)
     ap.add_argument("--iterations", type=int, default=5)
-    ap.add_argument("--storage", default=None, choices=["f32", "f16", "bf16"],
-                    help="image storage (default: f16, which holds 8-bit images exactly; f32 "
-                         "for --images float)")
+    ap.add_argument("--storage", default=None, choices=["f32", "f16", "bf16", "u8"],
+                    help="image storage (default: u8 for --images u8, which holds them exactly; "
+                         "f32 for --images float; f16 for --workload batch)")
     ap.add_argument("--images", default="u8", choices=["float", "u8"],
                     help="plane-mode input images: the renderer's images quantized to 8 bits "
                          "(what a camera and the reference deliver), or its float images")
@@ -389,9 +389,9 @@ def main():
         return run_reference(args)
     if args.workload == "copy":
         return run_copy(args)
-    # 8-bit values are exact in FP16: the depth maps are bit-identical to
-    # FP32 storage (a GPU test holds this), at half the image bytes
-    args.storage = args.storage or ("f16" if args.images == "u8" else "f32")
+    # 8-bit images stored as 8 bits: the depth maps are bit-identical to FP32
+    # storage (a GPU test holds this), at a quarter of the image bytes
+    args.storage = args.storage or ("u8" if args.images == "u8" else "f32")
     from recon_scanner_amd import plane, scenes
 
     t0 = time.time()

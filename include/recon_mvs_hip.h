@@ -16,8 +16,9 @@
  *     500 = no depth, 502 = GPU failure;
  *   - all buffers are caller-owned host memory unless a function says
  *     otherwise; images are row-major with stride = width (mvs.h:130-150);
- *   - one context drives one GPU on one HIP stream; a context is not safe
- *     for concurrent use from several threads (recon_c_api.h:10).
+ *   - one context drives one GPU on one HIP stream (plus an upload stream
+ *     ordered against it by events); a context is not safe for concurrent
+ *     use from several threads (recon_c_api.h:10).
  */
 #ifndef RECON_MVS_HIP_H_
 #define RECON_MVS_HIP_H_
@@ -357,8 +358,12 @@ typedef struct recon_mvs_plane_params {
 RECON_MVS_API void recon_mvs_plane_params_default(recon_mvs_plane_params* p);
 
 /*
- * Uploads n views into HBM, stored as `storage_format` (F32, F16 or BF16),
- * replacing any previous set.  Depth-map slots are per view index.
+ * Uploads n views into HBM, stored as `storage_format` (F32, F16, BF16 or
+ * U8), replacing any previous set.  Depth-map slots are per view index.  U8
+ * storage takes 8-bit views only (RECON_MVS_PIXEL_U8 or BGRA8 frames) and
+ * holds them exactly, as FP16 and BF16 do: results are bit-identical to F32
+ * storage for such views.  F16 / BF16 / U8 storage is compiled for the
+ * 11x11 stride-2 window (F16 / BF16 also for 7x7).
  */
 RECON_MVS_API int32_t recon_mvs_hip_set_views(recon_mvs_hip_context* ctx, int32_t n,
                                               const recon_mvs_view* views,

@@ -143,10 +143,11 @@ extern "C" RECON_MVS_API int32_t recon_mvs_hip_run_mvs_stage(
       jobs.push_back(j);
     }
     if (!jobs.empty()) {
-      // 8-bit keyframes are exact in FP16: the results equal FP32 storage's
-      // bit for bit at half the bytes.  The (11, 2) window has its sweep,
-      // batch and polish kernels in FP16; other windows polish in FP32 only.
-      const int32_t storage = (pp.patch_size == 11 && pp.window_stride == 2) ? RECON_MVS_PIXEL_F16
+      // 8-bit keyframes stored as 8 bits: the results equal FP32 storage's
+      // bit for bit at a quarter of the bytes.  The (11, 2) window has its
+      // sweep, batch and polish kernels in every storage format; other
+      // windows polish in FP32 only.
+      const int32_t storage = (pp.patch_size == 11 && pp.window_stride == 2) ? RECON_MVS_PIXEL_U8
                                                                              : RECON_MVS_PIXEL_F32;
       st = recon_mvs_hip_set_views(ctx, static_cast<int32_t>(views.size()), views.data(), storage);
       recon_mvs_hip_batch* b = nullptr;

@@ -536,6 +536,7 @@ int32_t LaunchAdj(recon_mvs_hip_context* ctx, const recon_mvs_plane_params* p, A
   if (ps == 11 && st == 2) {
     if (fmt == RECON_MVS_PIXEL_F32) LaunchAdjT<5, 2, RECON_MVS_PIXEL_F32>(k, *g, grid, ctx->stream);
     else if (fmt == RECON_MVS_PIXEL_F16) LaunchAdjT<5, 2, RECON_MVS_PIXEL_F16>(k, *g, grid, ctx->stream);
+    else if (fmt == RECON_MVS_PIXEL_U8) LaunchAdjT<5, 2, RECON_MVS_PIXEL_U8>(k, *g, grid, ctx->stream);
     else LaunchAdjT<5, 2, RECON_MVS_PIXEL_BF16>(k, *g, grid, ctx->stream);
   } else if (ps == 7 && st == 1 && fmt == RECON_MVS_PIXEL_F32) {
     LaunchAdjT<3, 1, RECON_MVS_PIXEL_F32>(k, *g, grid, ctx->stream);

@@ -91,13 +91,14 @@ struct PlaneArgs {
 // Resident images are stored as row pairs: element (y, x) holds
 // (I[y][x], I[y+1][x]) (zero beyond the image), so the four bilinear taps of
 // a sample at (x, y) are the 2 x 2 elements (y, x), (y, x + 1): one load of
-// 16 B (FP32) or 8 B (FP16 / BF16).  A stride-2 window touches a dense 12 x 12
+// 16 B (FP32), 8 B (FP16 / BF16) or 4 B (8-bit, RECON_MVS_PIXEL_U8 storage:
+// exact for 8-bit views; its 4-byte loads are 2-byte aligned).  A stride-2 window touches a dense 12 x 12
 // pixel block; as row pairs that is 6 runs of 12 elements instead of 12 runs
 // of 12 pixels, about half the cache lines for scattered (unconverged)
 // hypotheses.  Twice the bytes of a plain image.
 template <int FMT>
 constexpr int ElemBytes() {
-  return FMT == RECON_MVS_PIXEL_F32 ? 8 : 4;
+  return FMT == RECON_MVS_PIXEL_F32 ? 8 : FMT == RECON_MVS_PIXEL_U8 ? 2 : 4;
 }
 
 // Pixel (y, x) of a row-pair image, i = y * pitch + x.
@@ -105,6 +106,8 @@ template <int FMT>
 __device__ __forceinline__ float Pix(const void* base, int i) {
   if constexpr (FMT == RECON_MVS_PIXEL_F32) {
     return static_cast<const float*>(base)[2 * i];
+  } else if constexpr (FMT == RECON_MVS_PIXEL_U8) {
+    return static_cast<float>(static_cast<const uint8_t*>(base)[2 * i]);
   } else if constexpr (FMT == RECON_MVS_PIXEL_F16) {
     return static_cast<float>(static_cast<const _Float16*>(base)[2 * i]);
   } else {
@@ -122,6 +125,10 @@ __device__ __forceinline__ float4 BufQuad(__amdgpu_buffer_rsrc_t rsrc, int off) 
     const auto v = __builtin_amdgcn_raw_buffer_load_b128(rsrc, off, 0, 0);
     return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]),
                        __uint_as_float(v[3]));
+  } else if constexpr (FMT == RECON_MVS_PIXEL_U8) {
+    const uint32_t u = __builtin_amdgcn_raw_buffer_load_b32(rsrc, off, 0, 0);
+    return make_float4(static_cast<float>(u & 0xFFu), static_cast<float>((u >> 8) & 0xFFu),
+                       static_cast<float>((u >> 16) & 0xFFu), static_cast<float>(u >> 24));
   } else {
     const auto u = __builtin_amdgcn_raw_buffer_load_b64(rsrc, off, 0, 0);
     if constexpr (FMT == RECON_MVS_PIXEL_F16) {

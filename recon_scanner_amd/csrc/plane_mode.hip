@@ -598,6 +598,11 @@ __global__ void k_store_view(const void* __restrict__ src, int src_format, int s
     _Float16* o = static_cast<_Float16*>(dst) + 2 * static_cast<size_t>(i);
     o[0] = static_cast<_Float16>(v0);
     o[1] = static_cast<_Float16>(v1);
+  } else if constexpr (FMT == RECON_MVS_PIXEL_U8) {
+    // integer values 0..255 (set_views admits only 8-bit views)
+    uint8_t* o = static_cast<uint8_t*>(dst) + 2 * static_cast<size_t>(i);
+    o[0] = static_cast<uint8_t>(v0);
+    o[1] = static_cast<uint8_t>(v1);
   } else {
     uint16_t* o = static_cast<uint16_t*>(dst) + 2 * static_cast<size_t>(i);
     o[0] = ToBf16(v0);
@@ -980,7 +985,8 @@ template <int HALF, int STRIDE>
 bool LaunchFmt(int fmt, int schedule, KernelId id, const PlaneArgs& a, hipStream_t s, float f0, int i0,
                int n, const float* hyps, float* cost, int* nvalid, float* view_ncc, const BatchLaunch* bl,
                int cap, const PhaseLaunch* pl) {
-  // FP16 / BF16 storage is compiled for the 11x11 stride-2 and 7x7 windows
+  // FP16 / BF16 storage is compiled for the 11x11 stride-2 and 7x7 windows,
+  // 8-bit storage for the 11x11 stride-2 window
   constexpr bool kAllFormats = (HALF == 5 && STRIDE == 2) || (HALF == 3 && STRIDE == 1);
   switch (fmt) {
     case RECON_MVS_PIXEL_F32: return LaunchT<HALF, STRIDE, RECON_MVS_PIXEL_F32>(schedule, id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cap, pl);
@@ -989,6 +995,9 @@ bool LaunchFmt(int fmt, int schedule, KernelId id, const PlaneArgs& a, hipStream
       return false;
     case RECON_MVS_PIXEL_BF16:
       if constexpr (kAllFormats) return LaunchT<HALF, STRIDE, RECON_MVS_PIXEL_BF16>(schedule, id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cap, pl);
+      return false;
+    case RECON_MVS_PIXEL_U8:  // 8-bit storage: the 11x11 stride-2 window
+      if constexpr (HALF == 5 && STRIDE == 2) return LaunchT<HALF, STRIDE, RECON_MVS_PIXEL_U8>(schedule, id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cap, pl);
       return false;
     default: return false;
   }
@@ -1479,8 +1488,8 @@ RECON_MVS_API int32_t recon_mvs_hip_set_views(recon_mvs_hip_context* ctx, int32_
   if (!ctx) return RECON_MVS_ERR_INVALID_ARGUMENT;
   if (n < 0 || (n > 0 && !views)) return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "bad views");
   if (storage_format != RECON_MVS_PIXEL_F32 && storage_format != RECON_MVS_PIXEL_F16 &&
-      storage_format != RECON_MVS_PIXEL_BF16)
-    return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "storage format must be F32, F16 or BF16");
+      storage_format != RECON_MVS_PIXEL_BF16 && storage_format != RECON_MVS_PIXEL_U8)
+    return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "storage format must be F32, F16, BF16 or U8");
   for (int i = 0; i < n; ++i) {
     const recon_mvs_view& v = views[i];
     if (!v.pixels || v.camera.width <= 0 || v.camera.height <= 0 ||
@@ -1490,6 +1499,10 @@ RECON_MVS_API int32_t recon_mvs_hip_set_views(recon_mvs_hip_context* ctx, int32_
     if (v.pixel_format == RECON_MVS_PIXEL_BGRA8 ? (v.row_stride != 0 && v.row_stride < 4 * v.camera.width)
                                                 : v.row_stride != 0)
       return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "bad row_stride");
+    // 8-bit storage holds 8-bit views (gray or B,G,R,A frames) exactly, and
+    // nothing else
+    if (storage_format == RECON_MVS_PIXEL_U8 && v.pixel_format == RECON_MVS_PIXEL_F32)
+      return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "U8 storage needs 8-bit views");
     // sample addresses are formed in FP32: exact below 2^24 elements
     if (static_cast<int64_t>((v.camera.width + 1 + 63) & ~63) * v.camera.height >= (int64_t{1} << 24))
       return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT,
@@ -1498,7 +1511,7 @@ RECON_MVS_API int32_t recon_mvs_hip_set_views(recon_mvs_hip_context* ctx, int32_
   RMVS_HIP(ctx, hipStreamSynchronize(ctx->stream));
   DestroyPlaneState(ctx);
   ctx->storage_format = storage_format;
-  const size_t esz = storage_format == RECON_MVS_PIXEL_F32 ? 4 : 2;
+  const size_t esz = storage_format == RECON_MVS_PIXEL_F32 ? 4 : storage_format == RECON_MVS_PIXEL_U8 ? 1 : 2;
   for (int i = 0; i < n; ++i) {
     const recon_mvs_view& in = views[i];
     auto* v = new ResidentView();
@@ -1524,8 +1537,10 @@ RECON_MVS_API int32_t recon_mvs_hip_set_views(recon_mvs_hip_context* ctx, int32_
       hipLaunchKernelGGL(k_store_view<RECON_MVS_PIXEL_F32>, g, b, 0, ctx->stream, staging, in.pixel_format, src_stride, v->w, v->h, v->pitch, v->pixels);
     else if (storage_format == RECON_MVS_PIXEL_F16)
       hipLaunchKernelGGL(k_store_view<RECON_MVS_PIXEL_F16>, g, b, 0, ctx->stream, staging, in.pixel_format, src_stride, v->w, v->h, v->pitch, v->pixels);
-    else
+    else if (storage_format == RECON_MVS_PIXEL_BF16)
       hipLaunchKernelGGL(k_store_view<RECON_MVS_PIXEL_BF16>, g, b, 0, ctx->stream, staging, in.pixel_format, src_stride, v->w, v->h, v->pitch, v->pixels);
+    else
+      hipLaunchKernelGGL(k_store_view<RECON_MVS_PIXEL_U8>, g, b, 0, ctx->stream, staging, in.pixel_format, src_stride, v->w, v->h, v->pitch, v->pixels);
     RMVS_LAUNCH_CHECK(ctx);
     // staging is reused by the next view: finish this one first
     RMVS_HIP(ctx, hipStreamSynchronize(ctx->stream));

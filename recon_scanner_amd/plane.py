@@ -19,7 +19,7 @@ import numpy as np
 from . import _native as N
 from . import mvs
 
-FORMATS = {"f32": N.PIXEL_F32, "f16": N.PIXEL_F16, "bf16": N.PIXEL_BF16}
+FORMATS = {"f32": N.PIXEL_F32, "f16": N.PIXEL_F16, "bf16": N.PIXEL_BF16, "u8": N.PIXEL_U8}
 KERNEL_CLASSES = ("sweep", "init", "refine", "consistency", "score", "eval", "batch", "adjoint")
 
 

@@ -341,8 +341,8 @@ def test_view_phased_sweeps_bit_identical(storage, top_k):
 
 
 def test_u8_images_half_storage_bit_identical():
-    """8-bit images stored as FP16 or BF16 (both hold the integers 0..255
-    exactly) give the FP32-storage results bit for bit: planes, costs,
+    """8-bit images stored as FP16, BF16 or 8 bits (all hold the integers
+    0..255 exactly) give the FP32-storage results bit for bit: planes, costs,
     supports, confidences, after refinement and polish, and the consistency
     filter over them.  The bench
     stores its 8-bit images as FP16 on this ground."""
@@ -353,7 +353,7 @@ def test_u8_images_half_storage_bit_identical():
     p = plane.default_params(patch_size=11, window_stride=2, top_k=4, iterations=3,
                              min_depth_m=0.25, max_depth_m=0.60, value_scale=1.0)
     res = {}
-    for storage in ("f32", "f16", "bf16"):
+    for storage in ("f32", "f16", "bf16", "u8"):
         pm = plane.PlaneMVS(sc.cams, u8, storage=storage)
         for r in views:
             srcs = [v for v in views if v != r]
@@ -363,11 +363,14 @@ def test_u8_images_half_storage_bit_identical():
                 pm.polish(r, srcs, p, steps=5)
         pm.geo_consistency(views, 0.01)
         res[storage] = [pm.depth_map(r, geo=True) for r in views]
-    for storage in ("f16", "bf16"):
+    for storage in ("f16", "bf16", "u8"):
         for a, b in zip(res["f32"], res[storage]):
             for k in a:
                 assert a[k].tobytes() == b[k].tobytes(), (storage, k)
     assert np.isfinite(res["f32"][0]["depth"]).mean() > 0.9
+    # 8-bit storage refuses views that are not 8-bit
+    with pytest.raises(Exception):
+        plane.PlaneMVS(sc.cams, sc.images, storage="u8")
 
 
 def test_tuned_plan_bit_identical():
