@@ -304,7 +304,7 @@ def test_seeded_init_bit_exact_and_converges_faster():
     assert np.array_equal(pm.depth_map(0)["depth"], dr, equal_nan=True)
 
 
-@pytest.mark.parametrize("storage,top_k", [("f32", 4), ("f16", 4), ("f32", 0)])
+@pytest.mark.parametrize("storage,top_k", [("f32", 4), ("f16", 4), ("f32", 0), ("u8", 4)])
 def test_view_phased_sweeps_bit_identical(storage, top_k):
     """The view-phased half-sweep (one launch per source view with its own
     tile shape, candidate state in memory between launches, early exit
@@ -314,7 +314,8 @@ def test_view_phased_sweeps_bit_identical(storage, top_k):
     say("view-phased vs fused sweeps, sphere_planes 480x360 11x11/2", storage, "top_k", top_k)
     sc = scenes.sphere_planes(width=480, height=360, focal=625.0, supersample=1)
     srcs = list(range(1, 9))
-    pm = plane.PlaneMVS(sc.cams, sc.images, storage=storage)
+    imgs = [sc.u8(i) for i in range(sc.num_views)] if storage == "u8" else sc.images
+    pm = plane.PlaneMVS(sc.cams, imgs, storage=storage)
     p = plane.default_params(patch_size=11, window_stride=2, top_k=top_k, iterations=3,
                              min_depth_m=0.25, max_depth_m=0.60, value_scale=1.0, tile_schedule=2)
     outs = []
