@@ -245,6 +245,18 @@ def test_bgra_frames_gray_conversion_bit_exact():
     for k in ("depth", "normals", "cost"):
         assert np.array_equal(np.nan_to_num(da[k], nan=-1), np.nan_to_num(db[k], nan=-1)), k
     assert np.isfinite(da["depth"]).mean() > 0.9
+    # BGRA frames into 8-bit storage (the 11x11 stride-2 window), against the
+    # gray views in FP32 storage
+    p2 = plane.default_params(patch_size=11, window_stride=2, top_k=4, iterations=2,
+                              min_depth_m=0.25, max_depth_m=0.45, value_scale=1.0, tile_schedule=0)
+    c = plane.PlaneMVS(sc.cams, frames, storage="u8")
+    c.patchmatch(0, [1, 2, 3, 4], p2)
+    dc = c.depth_map(0)
+    d = plane.PlaneMVS(sc.cams, grays, storage="f32")
+    d.patchmatch(0, [1, 2, 3, 4], p2)
+    dd = d.depth_map(0)
+    for k in ("depth", "normals", "cost", "support"):
+        assert dc[k].tobytes() == dd[k].tobytes(), k
 
 
 def test_seeded_init_bit_exact_and_converges_faster():
