@@ -209,7 +209,6 @@ def run_adjoint(args):
     """Cost and gradients (per-pixel plane, 6 source poses) of a 960x720 depth
     map whose sources are misaligned by 0.5 deg / 1 mm (seeded), then the
     10-step gradient polish of a fresh PatchMatch result."""
-    import oracle
     from recon_scanner_amd import plane, scenes
     sc = scenes.sphere_planes(width=960, height=720, focal=1250.0, n_views=7, supersample=2)
     rng = np.random.default_rng(5)
@@ -217,7 +216,7 @@ def run_adjoint(args):
     for k in range(1, 7):
         ax, tr = rng.normal(size=3), rng.normal(size=3)
         d = np.concatenate([np.radians(0.5) * ax / np.linalg.norm(ax), 1e-3 * tr / np.linalg.norm(tr)])
-        cams[k] = oracle.perturb_camera(sc.cams[k], d)
+        cams[k] = scenes.perturb_camera(sc.cams[k], d)
     srcs = list(range(1, 7))
     p = plane.default_params(patch_size=11, window_stride=2, top_k=4, iterations=args.iterations,
                              min_depth_m=sc.min_depth, max_depth_m=sc.max_depth, value_scale=1.0,

@@ -118,6 +118,13 @@ def _declare_plane(lib):
     lib.orc_plane_cost_grad.restype = C.c_int
     lib.orc_rng_uniform.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]
     lib.orc_rng_uniform.restype = C.c_float
+    lib.orc_plane_refine.argtypes = [C.c_int, _P(C.c_void_p), _cam_p, C.c_int, C.c_int, C.c_void_p,
+                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.orc_plane_refine.restype = C.c_int
+    lib.orc_refine_1d_closed_form.argtypes = [C.c_int, C.c_double, C.c_double, C.c_float,
+                                              C.c_float, C.c_int, C.c_float, C.c_float,
+                                              C.c_void_p, C.c_void_p]
+    lib.orc_refine_1d_closed_form.restype = C.c_int
 
 
 def reference_available() -> bool:
@@ -349,6 +356,34 @@ def plane_patchmatch(images, cams, ref, srcs, params, init_planes=None):
                                   planes.ctypes.data, cost.ctypes.data, sup.ctypes.data)
     assert ok
     return planes, cost, sup
+
+
+def plane_refine(images, cams, ref, srcs, params, planes, cost, support):
+    """FP64 plane-mode sub-pixel refinement (the search of the GPU's
+    k_plane_refine) of a depth map: planes HxWx4 float32, cost HxW (>= 1e29:
+    invalid), support HxW uint8.  Returns refined copies (planes, cost, support)."""
+    lib = restated()
+    imgs, ptrs, cm = _views(images, cams)
+    srcs = np.ascontiguousarray(srcs, np.int32)
+    planes = np.array(planes, np.float32, order="C")
+    cost = np.array(cost, np.float64, order="C")
+    support = np.array(support, np.uint8, order="C")
+    ok = lib.orc_plane_refine(len(imgs), ptrs, cm, ref, len(srcs), srcs.ctypes.data,
+                              C.byref(params), planes.ctypes.data, cost.ctypes.data,
+                              support.ctypes.data)
+    assert ok
+    return planes, cost, support
+
+
+def refine_1d_closed_form(kind, m, A, inv0, inv_per_px, iterations, inv_min, inv_max):
+    """The refinement search on a closed-form cost (0: A (x - m)^2, 1: A |x - m|,
+    2: 1 - exp(-((x - m) / A)^2)).  Returns (best x as float32, its cost,
+    number of cost evaluations)."""
+    bi = C.c_float()
+    bc = C.c_double()
+    n = restated().orc_refine_1d_closed_form(kind, m, A, inv0, inv_per_px, iterations, inv_min,
+                                             inv_max, C.byref(bi), C.byref(bc))
+    return np.float32(bi.value), bc.value, n
 
 
 def seed_cells(w, h, seeds, grid=32):

@@ -772,3 +772,185 @@ void orc_geo_consistency(int nmaps, const float* const* depth, const recon_mvs_c
     }
   }
 }
+
+/* ---- sub-pixel refinement along the ray (plane mode) ----
+   The search of the GPU's k_plane_refine for one pixel with its normal held:
+   a 5-point least-squares parabola in inverse depth (samples inv0 + (k - 2)
+   step, step = inv_per_px / 2, inv_per_px = 1 / (f B), the reference's step,
+   patchmatch_stereo.cpp:840-841), whose vertex -- scaled by the step, where
+   the reference adds it in index units (line 898) -- is evaluated when it
+   lies within the samples; then golden section over best +- inv_per_px
+   (docs/05_单目管线.md:1694-1700), and the best of the costs seen.
+   Candidate inverse depths are formed with the GPU's FP32 operations; costs
+   and the parabola fit are FP64.  The search is written over a cost callback
+   so that tests can run it on closed-form functions. */
+typedef double (*refine_cost_fn)(const void* ctx, float inv, int* nv);
+
+static void refine_1d(refine_cost_fn f, const void* fc, float inv0, double c0, int s0,
+                      float inv_per_px, int iterations, float inv_min, float inv_max,
+                      float* best_inv, double* best_cost, int* best_nv) {
+  float bi = inv0;
+  double bc = c0;
+  int bs = s0;
+  const float step = 0.5f * inv_per_px;
+  double fv[5];
+  int fit_ok = 1;
+  for (int k = 0; k < 5; ++k) {
+    if (k == 2) {
+      fv[k] = c0;
+      continue;
+    }
+    int nv = 0;
+    fv[k] = f(fc, inv0 + (float)(k - 2) * step, &nv);
+    fit_ok = fit_ok && fv[k] < ORC_COST_INF;
+  }
+  if (fit_ok) {
+    /* normal equations of c(x) = a x^2 + b x + c at x = -2..2 */
+    const double aa = (2.0 * fv[0] - fv[1] - 2.0 * fv[2] - fv[3] + 2.0 * fv[4]) / 14.0;
+    const double bb = (-2.0 * fv[0] - fv[1] + fv[3] + 2.0 * fv[4]) / 10.0;
+    if (aa > 1e-12) {
+      const float vtx = (float)(-bb / (2.0 * aa));
+      if (vtx >= -2.0f && vtx <= 2.0f) {
+        const float cand = inv0 + vtx * step;
+        int nv = 0;
+        const double cc = f(fc, cand, &nv);
+        if (cc < bc) {
+          bc = cc;
+          bi = cand;
+          bs = nv;
+        }
+      }
+    }
+  }
+  float lo = fmaxf(bi - inv_per_px, inv_min), hi = fminf(bi + inv_per_px, inv_max);
+  if (hi > lo && iterations > 0) {
+    const float ip = 0.6180339887498949f;
+    float c1 = hi - ip * (hi - lo), c2 = lo + ip * (hi - lo);
+    int n1 = 0, n2 = 0;
+    double f1 = f(fc, c1, &n1), f2 = f(fc, c2, &n2);
+    for (int k = 0; k < iterations; ++k) {
+      if (f1 < f2) {
+        hi = c2;
+        c2 = c1;
+        f2 = f1;
+        n2 = n1;
+        c1 = hi - ip * (hi - lo);
+        f1 = f(fc, c1, &n1);
+      } else {
+        lo = c1;
+        c1 = c2;
+        f1 = f2;
+        n1 = n2;
+        c2 = lo + ip * (hi - lo);
+        f2 = f(fc, c2, &n2);
+      }
+    }
+    if (f1 < bc) {
+      bc = f1;
+      bi = c1;
+      bs = n1;
+    }
+    if (f2 < bc) {
+      bc = f2;
+      bi = c2;
+      bs = n2;
+    }
+  }
+  *best_inv = bi;
+  *best_cost = bc;
+  *best_nv = bs;
+}
+
+typedef struct {
+  const plane_args* a;
+  int x, y;
+  float nx, ny, nz;
+} pixel_refine_ctx;
+
+static double pixel_refine_cost(const void* ctx, float inv, int* nv) {
+  const pixel_refine_ctx* c = (const pixel_refine_ctx*)ctx;
+  *nv = 0;
+  if (!(inv >= c->a->inv_min && inv <= c->a->inv_max)) return ORC_COST_INF;
+  hyp h = {1.0f / inv, c->nx, c->ny, c->nz};
+  return plane_cost(c->a, c->x, c->y, h, nv, NULL);
+}
+
+/* Exported: plane-mode sub-pixel refinement of a depth map in place.
+   planes P x 4 (d, nx, ny, nz), cost P (the cost of each plane; >= 1e29:
+   invalid, left alone), support P.  inv_per_px = 1 / (f B) in FP32 with f
+   the mean reference focal length and B the mean reference-source baseline
+   (patchmatch_stereo.cpp:472-479, 840). */
+int orc_plane_refine(int nviews, const float* const* imgs, const recon_mvs_camera* cams, int ref,
+                     int nsrc, const int32_t* srcs, const orc_plane_params* p, float* planes,
+                     double* cost, uint8_t* support) {
+  (void)nviews;
+  plane_args a;
+  if (!make_args(&a, imgs, cams, ref, nsrc, srcs, p, 0.5)) return 0;
+  double rc[3];
+  orc_center(&cams[ref], rc);
+  double bsum = 0.0;
+  for (int k = 0; k < nsrc; ++k) {
+    double sc[3];
+    orc_center(&cams[srcs[k]], sc);
+    const double d[3] = {sc[0] - rc[0], sc[1] - rc[1], sc[2] - rc[2]};
+    bsum += orc_norm3(d);
+  }
+  const double baseline = bsum / (double)nsrc;
+  if (!(baseline > 1e-9)) return 0;
+  const double focal = 0.5 * (cams[ref].fx + cams[ref].fy);
+  const float inv_per_px = (float)(1.0 / (focal * baseline));
+  const int P = a.w * a.h;
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int idx = 0; idx < P; ++idx) {
+    if (!(cost[idx] < ORC_COST_INF)) continue;
+    float* pl = planes + 4 * idx;
+    pixel_refine_ctx c = {&a, idx % a.w, idx / a.w, pl[1], pl[2], pl[3]};
+    float bi;
+    double bc;
+    int bs;
+    refine_1d(pixel_refine_cost, &c, 1.0f / pl[0], cost[idx], support[idx], inv_per_px,
+              p->refine_iterations, a.inv_min, a.inv_max, &bi, &bc, &bs);
+    pl[0] = 1.0f / bi;
+    cost[idx] = bc;
+    support[idx] = (uint8_t)(bs < 255 ? bs : 255);
+  }
+  return 1;
+}
+
+/* Closed-form costs for pinning refine_1d: 0 = A (x - m)^2 (a parabola: the
+   fit recovers its vertex), 1 = A |x - m| (unimodal, not smooth: only the
+   golden section finds it), 2 = 1 - exp(-((x - m) / A)^2) (unimodal, smooth,
+   not quadratic); infinite outside [lo, hi], as a pixel's cost is outside the
+   inverse-depth range. */
+typedef struct {
+  int kind;
+  double m, A;
+  float lo, hi;
+  int evals;
+} closed_form_ctx;
+
+static double closed_form_cost(const void* ctx, float inv, int* nv) {
+  closed_form_ctx* c = (closed_form_ctx*)ctx;
+  ++c->evals;
+  *nv = 0;
+  if (!(inv >= c->lo && inv <= c->hi)) return ORC_COST_INF;
+  *nv = 1;
+  const double x = (double)inv - c->m;
+  if (c->kind == 0) return c->A * x * x;
+  if (c->kind == 1) return c->A * fabs(x);
+  return 1.0 - exp(-(x / c->A) * (x / c->A));
+}
+
+/* Exported: refine_1d on a closed-form cost; returns the number of cost
+   evaluations. */
+int orc_refine_1d_closed_form(int kind, double m, double A, float inv0, float inv_per_px,
+                              int iterations, float inv_min, float inv_max, float* best_inv,
+                              double* best_cost) {
+  closed_form_ctx c = {kind, m, A, inv_min, inv_max, 0};
+  int nv0 = 0, bs = 0;
+  const double c0 = closed_form_cost(&c, inv0, &nv0);
+  c.evals = 0;
+  refine_1d(closed_form_cost, &c, inv0, c0, nv0, inv_per_px, iterations, inv_min, inv_max,
+            best_inv, best_cost, &bs);
+  return c.evals;
+}

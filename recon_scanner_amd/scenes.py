@@ -202,6 +202,29 @@ def orbit_batch(num_refs=48, num_cams=64, seed=0, min_res=(640, 480), max_res=(1
     return sc, jobs
 
 
+def perturb_camera(cam: N.Camera, delta) -> N.Camera:
+    """A misaligned copy of `cam` for pose-error workloads: its world->camera
+    transform moved on the left by the rotation vector delta[:3] (Rodrigues'
+    formula) and the translation delta[3:], Y -> Exp(w) Y + tau -- the pose
+    perturbation the library's adjoint differentiates."""
+    import copy
+    w = np.asarray(delta[:3], np.float64)
+    th = float(np.linalg.norm(w))
+    K = np.array([[0.0, -w[2], w[1]], [w[2], 0.0, -w[0]], [-w[1], w[0], 0.0]])
+    if th < 1e-12:
+        E = np.eye(3) + K + 0.5 * K @ K
+    else:
+        E = np.eye(3) + (math.sin(th) / th) * K + ((1.0 - math.cos(th)) / (th * th)) * (K @ K)
+    c = copy.copy(cam)
+    R2 = E @ cam.R_np()
+    t2 = E @ cam.t_np() + np.asarray(delta[3:], np.float64)
+    for i in range(9):
+        c.R[i] = float(R2.reshape(-1)[i])
+    for i in range(3):
+        c.t[i] = float(t2[i])
+    return c
+
+
 def reference_plane(num_views=5, baseline_m=0.016):
     """tests/unit/test_mvs.cpp:112-204 restated: 160x120 uint8, f=400, plane at 0.35 m."""
     lib = N.scene_lib()

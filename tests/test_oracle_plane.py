@@ -190,3 +190,134 @@ def test_geo_consistency_exact_maps_agree():
     bad = [maps[0] * 1.03] + maps[1:]
     gb = oracle.geo_consistency(bad, sc.cams, 0.01)
     assert np.mean(gb[0][fin] >= 2) < 0.05
+
+
+@pytest.mark.parametrize("top_k", [1, 2, 4, 0])
+def test_topk_aggregation_brute_force(top_k):
+    """The aggregated cost is the mean of the k smallest per-view costs
+    1 - NCC over the valid views (docs/05_单目管线.md:1596-1599; k = 0: every
+    valid view, the reduction to EvaluateCandidate's mean), rejected above
+    3 x photo_threshold.  Recomputed here by sorting the oracle's own per-view
+    NCCs; hypotheses near the image border and steep planes leave some views
+    invalid, so the k-of-valid rule (fewer valid views than k) is exercised
+    too.  A wrong k, divisor, sort order or rejection test fails this."""
+    sc = scenes.sphere_planes(width=160, height=120, focal=208.0, n_views=9, supersample=1)
+    srcs = list(range(1, 9))
+    rng = np.random.default_rng(11)
+    n = 1500
+    xs = np.concatenate([rng.integers(0, 160, n - 300), rng.integers(0, 6, 150),
+                         rng.integers(154, 160, 150)])
+    ys = rng.integers(0, 120, n)
+    nrm = rng.normal(size=(n, 3))
+    nrm[:, 2] = -np.abs(nrm[:, 2]) - 0.05
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    d = rng.uniform(0.26, 0.58, n)
+    t = sc.depths[0][ys, xs]
+    half = n // 2
+    d[:half] = np.where(t[:half] > 0, t[:half], d[:half])
+    nrm[:half] = [0.0, 0.0, -1.0]
+    hyps = np.concatenate([xs[:, None], ys[:, None], d[:, None], nrm], 1).astype(np.float32)
+    thr = 0.3
+    p = params(patch_size=11, window_stride=2, top_k=top_k, value_scale=1.0, min_depth_m=0.25,
+               max_depth_m=0.60, photo_threshold=thr)
+    cost, nv, vn = oracle.plane_eval(sc.images, sc.cams, 0, srcs, p, hyps)
+    k = top_k if top_k else len(srcs)
+    nvalid_seen = set()
+    rejected = 0
+    for q in range(n):
+        c = np.sort(1.0 - vn[q][np.isfinite(vn[q])])
+        assert nv[q] == len(c)
+        nvalid_seen.add(len(c))
+        if len(c) == 0:
+            assert cost[q] >= 1e29
+            continue
+        m = min(k, len(c))
+        want = sum(float(v) for v in c[:m]) / m
+        if want > 3.0 * thr:
+            assert cost[q] >= 1e29, (q, want)
+            rejected += 1
+        else:
+            assert abs(cost[q] - want) <= 1e-12, (q, cost[q], want)
+    # both regimes of the k-of-valid rule and the rejection occur
+    assert min(nvalid_seen) < k <= max(nvalid_seen) or k == len(srcs)
+    assert len(nvalid_seen) >= 3
+    assert rejected > 0
+
+
+@pytest.mark.parametrize("delta", [0.75, -1.25, 1.875, -2.0, 0.0])
+def test_refine_parabola_vertex_exact(delta):
+    """The 5-point least-squares parabola through samples of an exact
+    quadratic A (x - m)^2 is that quadratic, so its vertex -- scaled by the
+    sample step (inv_per_px / 2) -- lands on m.  Dyadic inv0, step and m make
+    every sample position and the vertex exact in FP32: the search returns m
+    bit for bit with golden section off.  The reference's index-unit vertex
+    (patchmatch_stereo.cpp:898, cand = inv0 + vertex) or a wrong sign of the
+    linear term would land elsewhere."""
+    inv0, ipp = np.float32(3.0), np.float32(2.0 ** -8)
+    m = float(inv0) + delta * float(ipp) / 2.0
+    best, cost, n = oracle.refine_1d_closed_form(0, m, 5.0, inv0, ipp, 0, 0.5, 10.0)
+    if delta == 0.0:
+        # the vertex is the start: no better cost, the start is kept
+        assert best == inv0 and cost == 0.0 and n == 5
+    else:
+        assert best == np.float32(m) and cost == 0.0
+        assert n == 5   # 4 fit samples + the vertex
+    # a vertex beyond the samples (|x| > 2 steps) is not evaluated, and the
+    # fit samples themselves are not candidates: the start is kept
+    far = float(inv0) + 2.5 * float(ipp) / 2.0
+    best, cost, n = oracle.refine_1d_closed_form(0, far, 5.0, inv0, ipp, 0, 0.5, 10.0)
+    assert n == 4 and best == inv0 and cost == 5.0 * (far - float(inv0)) ** 2
+
+
+@pytest.mark.parametrize("kind", [1, 2])
+@pytest.mark.parametrize("iters", [1, 2, 4, 8, 12])
+def test_refine_golden_section_bracket(kind, iters):
+    """Golden section (Kiefer) on a unimodal cost over best +- inv_per_px
+    keeps the minimiser inside a bracket that shrinks by 0.618 per step: the
+    result is within inv_per_px * 2 * 0.618^N of the minimiser m, with 2 + N
+    evaluations after the fit's 4 (+1 for an in-range vertex).  |x - m| (kind
+    1) is not smooth, so the parabola cannot find it; the Gaussian dip (kind
+    2) is smooth but not quadratic."""
+    inv0, ipp = np.float32(2.5), np.float32(0.004)
+    rng = np.random.default_rng(kind * 100 + iters)
+    for _ in range(40):
+        m = float(inv0) + rng.uniform(-0.95, 0.95) * float(ipp)
+        best, cost, n = oracle.refine_1d_closed_form(kind, m, 0.01 if kind == 2 else 3.0, inv0,
+                                                     ipp, iters, 0.5, 10.0)
+        bound = 2.0 * float(ipp) * 0.6180339887 ** iters
+        assert abs(float(best) - m) <= bound * 1.001 + 4e-7, (m, best, bound)
+        assert n - iters - 2 in (4, 5)
+    # a minimiser beyond inv_max: the fit reaches out of range (no vertex),
+    # the section is clamped to [inv_min, inv_max] and ends at the edge
+    hi = np.float32(float(inv0) + 0.25 * float(ipp))
+    best, _, n = oracle.refine_1d_closed_form(kind, float(inv0) + 0.5 * float(ipp),
+                                              3.0 if kind == 1 else 0.01, inv0, ipp, iters, 0.5, hi)
+    assert n == 4 + 2 + iters
+    assert float(hi) - 1.25 * float(ipp) * 0.6180339887 ** iters - 4e-7 <= float(best) <= float(hi)
+
+
+def test_refine_lowers_costs_on_a_depth_map():
+    """On the oracle's own PatchMatch result the refinement never raises a
+    pixel's cost, keeps invalid pixels as they are, moves depths by at most
+    inv_per_px in inverse depth (the golden-section range) and lowers the
+    median depth error against the analytic truth."""
+    sc = scenes.sphere_arc(width=80, height=60, focal=125.0)
+    srcs = [1, 2, 3, 4]
+    p = params(patch_size=7, window_stride=1, top_k=2, iterations=2, value_scale=1.0)
+    planes, cost, sup = oracle.plane_patchmatch(sc.images, sc.cams, 0, srcs, p)
+    rp, rc, rs = oracle.plane_refine(sc.images, sc.cams, 0, srcs, p, planes, cost, sup)
+    valid = cost < 1e29
+    assert np.all(rc[valid] <= cost[valid])
+    assert np.array_equal(rp[~valid], planes[~valid]) and np.array_equal(rc[~valid], cost[~valid])
+    assert np.array_equal(rp[..., 1:], planes[..., 1:])          # normals held
+    c = sc.cams
+    base = np.mean([np.linalg.norm(c[k].center() - c[0].center()) for k in srcs])
+    ipp = 1.0 / (0.5 * (c[0].fx + c[0].fy) * base)
+    dinv = np.abs(1.0 / rp[..., 0][valid] - 1.0 / planes[..., 0][valid])
+    assert dinv.max() <= 2.0 * ipp * 1.0001
+    assert np.mean(rc[valid] < cost[valid]) > 0.5
+    truth = sc.depths[0]
+    m = valid & (truth > 0)
+    e0 = np.median(np.abs(planes[..., 0][m] - truth[m]))
+    e1 = np.median(np.abs(rp[..., 0][m] - truth[m]))
+    assert e1 < e0

@@ -156,20 +156,32 @@ def test_refine_keeps_accuracy():
     assert eb <= ea * 1.05
 
 
-def test_batch_matches_per_view_runs_bit_exact():
-    """Batched persistent launch (uneven resolutions, 4-10 sources, FP16
-    storage) == per-view plane_patchmatch with the same parameters."""
-    say("batch: orbit of 12 cameras, 6 refs, uneven sizes, f16")
+@pytest.mark.parametrize("storage", ["f16", "u8"])
+def test_batch_matches_per_view_runs_bit_exact(storage):
+    """Batched persistent launch (uneven resolutions, 4-10 sources) ==
+    per-view plane_patchmatch with the same parameters, over 3 graph replays
+    with eager work between them.  FP16 storage of the float images, and
+    8-bit storage of 8-bit images (what the stage uploads): the latter also
+    equals per-view runs on the same images in FP32 storage bit for bit."""
+    say("batch: orbit of 12 cameras, 6 refs, uneven sizes,", storage)
     sc, jobs = scenes.orbit_batch(num_refs=6, num_cams=12, seed=3, min_res=(160, 120),
                                   max_res=(320, 240), src_range=(4, 6))
     p = plane.default_params(patch_size=11, window_stride=2, top_k=4, iterations=2,
                              min_depth_m=sc.min_depth, max_depth_m=sc.max_depth, value_scale=1.0,
                              tile_schedule=1)
-    pm = plane.PlaneMVS(sc.cams, sc.images, storage="f16")
+    imgs = sc.images if storage == "f16" else [sc.u8(i) for i in range(sc.num_views)]
+    pm = plane.PlaneMVS(sc.cams, imgs, storage=storage)
     singles = {}
     for r, srcs in jobs:
         pm.patchmatch(r, srcs, p)
         singles[r] = pm.depth_map(r)
+    if storage == "u8":
+        pf = plane.PlaneMVS(sc.cams, imgs, storage="f32")
+        for r, srcs in jobs:
+            pf.patchmatch(r, srcs, p)
+            f = pf.depth_map(r)
+            for k in ("depth", "normals", "cost", "support"):
+                assert f[k].tobytes() == singles[r][k].tobytes(), (r, k)
     b = pm.batch(jobs, p)
     for rep in range(3):   # graph replays, eager work between them
         b.run()
@@ -177,7 +189,7 @@ def test_batch_matches_per_view_runs_bit_exact():
         for r, _ in jobs:
             g = pm.depth_map(r)
             for k in ("depth", "normals", "cost", "support"):
-                assert np.array_equal(np.nan_to_num(g[k], nan=-1), np.nan_to_num(singles[r][k], nan=-1)), (rep, r, k)
+                assert g[k].tobytes() == singles[r][k].tobytes(), (rep, r, k)
     b.close()
 
 
@@ -357,8 +369,8 @@ def test_u8_images_half_storage_bit_identical():
     """8-bit images stored as FP16, BF16 or 8 bits (all hold the integers
     0..255 exactly) give the FP32-storage results bit for bit: planes, costs,
     supports, confidences, after refinement and polish, and the consistency
-    filter over them.  The bench
-    stores its 8-bit images as FP16 on this ground."""
+    filter over them.  8-bit views are stored as 8 bits by default (the
+    depth-map bench and the stage) on this ground."""
     say("u8 images: f32 vs f16 vs bf16 storage, sphere_planes 480x360 11x11/2")
     sc = scenes.sphere_planes(width=480, height=360, focal=625.0, supersample=1, n_views=5)
     u8 = [sc.u8(i) for i in range(sc.num_views)]
@@ -388,13 +400,19 @@ def test_u8_images_half_storage_bit_identical():
 
 def test_tuned_plan_bit_identical():
     """tile_schedule = -1 with occupancy_cap = 0 (the tuned schedule and
-    per-phase occupancy caps) on 8-bit images in FP16: the results equal
-    every fixed schedule's bit for bit."""
-    say("tuned plan vs fixed schedules, sphere_planes 480x360 u8/f16, 3 iterations")
+    per-phase occupancy caps) on 8-bit images in 8-bit storage (the
+    depth-map bench's configuration) and in FP16: the results equal every
+    fixed schedule's bit for bit."""
+    say("tuned plan vs fixed schedules, sphere_planes 480x360 u8 images, 3 iterations")
     sc = scenes.sphere_planes(width=480, height=360, focal=625.0, supersample=1)
     u8 = [sc.u8(i) for i in range(sc.num_views)]
     srcs = list(range(1, 9))
-    pm = plane.PlaneMVS(sc.cams, u8, storage="f16")
+    for storage in ("u8", "f16"):
+        _tuned_plan_case(sc, u8, srcs, storage)
+
+
+def _tuned_plan_case(sc, u8, srcs, storage):
+    pm = plane.PlaneMVS(sc.cams, u8, storage=storage)
     base = plane.default_params(patch_size=11, window_stride=2, top_k=4, iterations=3,
                                 min_depth_m=0.25, max_depth_m=0.60, value_scale=1.0)
     outs = []
