@@ -152,6 +152,38 @@ RECON_MVS_API int32_t recon_mvs_hip_destroy(recon_mvs_hip_context* ctx);
 RECON_MVS_API const char* recon_mvs_hip_last_error(const recon_mvs_hip_context* ctx);
 /* Blocks until all work queued on the context's stream has finished. */
 RECON_MVS_API int32_t recon_mvs_hip_synchronize(recon_mvs_hip_context* ctx);
+/*
+ * Status of the last call on ctx that failed (0 if none): e.g. 502 after a
+ * GPU failure, which the C++ entry points turn into recon::MvsGpuError.
+ */
+RECON_MVS_API int32_t recon_mvs_hip_last_status(const recon_mvs_hip_context* ctx);
+
+/*
+ * Cooperative cancellation, the library's side of recon_cancel /
+ * recon_is_cancelled (recon_c_api.h:601-608; Orchestrator::CheckCancel,
+ * orchestrator.cpp:249, checked per keyframe at orchestrator.cpp:989).  GPU
+ * work is not preempted: the flag is checked between dispatch batches
+ * (docs/01_接口契约.md §3.11) -- per keyframe in recon_mvs_hip_run_mvs_stage,
+ * per directional pass in recon_mvs_hip_patchmatch_stereo, at the start of
+ * every call of the plane-mode entry points, and by the persistent batch
+ * kernel between its work units (one tile), so a running batch drains within
+ * milliseconds.  A call that sees the flag returns RECON_MVS_ERR_CANCELLED
+ * (5) after the work already queued has finished; its outputs are undefined.
+ *
+ * recon_mvs_hip_cancel sets the flag; it is thread-safe and may be called
+ * from any thread while another thread is inside a call on ctx (the only
+ * exception to the one-thread rule).  The flag stays set, as recon_cancel's
+ * does, until recon_mvs_hip_clear_cancel.  A callback (may be NULL) is polled
+ * at the same check points; a nonzero answer sets the flag -- e.g. a function
+ * returning the app context's cancel flag, so recon_cancel reaches running
+ * GPU work without another call.
+ */
+typedef int32_t (*recon_mvs_cancel_fn)(void* user);
+RECON_MVS_API int32_t recon_mvs_hip_cancel(recon_mvs_hip_context* ctx);
+RECON_MVS_API int32_t recon_mvs_hip_is_cancelled(recon_mvs_hip_context* ctx);
+RECON_MVS_API int32_t recon_mvs_hip_clear_cancel(recon_mvs_hip_context* ctx);
+RECON_MVS_API int32_t recon_mvs_hip_set_cancel_callback(recon_mvs_hip_context* ctx,
+                                                        recon_mvs_cancel_fn fn, void* user);
 
 /* ------------------------------------------------------------------------- */
 /* Reference mode: the reference's own algorithm, FP64, reproduced on the GPU */

@@ -139,6 +139,20 @@ RECON_MVS_API int32_t recon_mvs_hip_create(int32_t device, recon_mvs_hip_context
     (void)recon_mvs_hip_destroy(ctx);
     return RECON_MVS_ERR_GPU_FAILURE;
   }
+  // the cancel flag: coherent pinned host memory, read by kernels through its
+  // device mapping
+  if (hipHostMalloc(reinterpret_cast<void**>(&ctx->cancel_host), 64,
+                    hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
+    ctx->cancel_host = nullptr;
+    (void)recon_mvs_hip_destroy(ctx);
+    return RECON_MVS_ERR_GPU_FAILURE;
+  }
+  *ctx->cancel_host = 0;
+  if (hipHostGetDevicePointer(reinterpret_cast<void**>(&ctx->cancel_dev), ctx->cancel_host, 0) !=
+      hipSuccess) {
+    (void)recon_mvs_hip_destroy(ctx);
+    return RECON_MVS_ERR_GPU_FAILURE;
+  }
   for (int i = 0; i < 2; ++i) {
     if (hipEventCreateWithFlags(&ctx->pinned_ready[i], hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->tab_free[i], hipEventDisableTiming) != hipSuccess) {
@@ -169,8 +183,38 @@ RECON_MVS_API int32_t recon_mvs_hip_destroy(recon_mvs_hip_context* ctx) {
   for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
   if (ctx->copy_stream) (void)hipStreamDestroy(ctx->copy_stream);
   (void)hipStreamDestroy(ctx->stream);
+  if (ctx->cancel_host) (void)hipHostFree(ctx->cancel_host);
   delete ctx;
   return RECON_MVS_OK;
+}
+
+RECON_MVS_API int32_t recon_mvs_hip_cancel(recon_mvs_hip_context* ctx) {
+  if (!ctx) return RECON_MVS_ERR_INVALID_ARGUMENT;
+  __atomic_store_n(ctx->cancel_host, 1, __ATOMIC_RELEASE);
+  return RECON_MVS_OK;
+}
+
+RECON_MVS_API int32_t recon_mvs_hip_is_cancelled(recon_mvs_hip_context* ctx) {
+  if (!ctx) return 0;
+  return CancelRequested(ctx) ? 1 : 0;
+}
+
+RECON_MVS_API int32_t recon_mvs_hip_clear_cancel(recon_mvs_hip_context* ctx) {
+  if (!ctx) return RECON_MVS_ERR_INVALID_ARGUMENT;
+  __atomic_store_n(ctx->cancel_host, 0, __ATOMIC_RELEASE);
+  return RECON_MVS_OK;
+}
+
+RECON_MVS_API int32_t recon_mvs_hip_set_cancel_callback(recon_mvs_hip_context* ctx,
+                                                        recon_mvs_cancel_fn fn, void* user) {
+  if (!ctx) return RECON_MVS_ERR_INVALID_ARGUMENT;
+  ctx->cancel_fn = fn;
+  ctx->cancel_user = user;
+  return RECON_MVS_OK;
+}
+
+RECON_MVS_API int32_t recon_mvs_hip_last_status(const recon_mvs_hip_context* ctx) {
+  return ctx ? ctx->last_status : RECON_MVS_ERR_INVALID_ARGUMENT;
 }
 
 RECON_MVS_API const char* recon_mvs_hip_last_error(const recon_mvs_hip_context* ctx) {

@@ -5,6 +5,7 @@
 // the compute runs on HIP device 0.  Also the camera methods of
 // include/recon/camera.h (core/src/util/camera.cpp:33-162).
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -14,6 +15,7 @@
 #include "recon/camera.h"
 #include "recon/mvs.h"
 #include "recon/mvs_cache.h"
+#include "recon/mvs_gpu.h"
 #include "recon/mvs_stage.h"
 #include "recon_mvs_hip.h"
 
@@ -73,17 +75,42 @@ recon_mvs_config ToConfig(const MvsConfig& m) {
 std::mutex g_mu;
 recon_mvs_hip_context* g_ctx = nullptr;
 
+// Cancellation of the recon:: entry points: CancelMvs (any thread) and the
+// caller's callback, polled through the context's own callback at the
+// library's check points.
+std::atomic<bool> g_cancel{false};
+std::atomic<int (*)(void*)> g_cancel_fn{nullptr};
+std::atomic<void*> g_cancel_user{nullptr};
+
+int32_t PollCancel(void*) {
+  if (g_cancel.load(std::memory_order_acquire)) return 1;
+  int (*fn)(void*) = g_cancel_fn.load(std::memory_order_acquire);
+  return (fn && fn(g_cancel_user.load(std::memory_order_acquire))) ? 1 : 0;
+}
+
 recon_mvs_hip_context* Ctx() {
   if (!g_ctx) {
     const int32_t st = recon_mvs_hip_create(0, &g_ctx);
-    if (st != RECON_MVS_OK) throw MvsGpuError(RECON_MVS_ERR_GPU_FAILURE, "recon_mvs_hip_create failed");
+    if (st != RECON_MVS_OK) {
+      g_ctx = nullptr;
+      throw MvsGpuError(RECON_MVS_ERR_GPU_FAILURE, "recon_mvs_hip_create failed");
+    }
+    (void)recon_mvs_hip_set_cancel_callback(g_ctx, PollCancel, nullptr);
   }
   return g_ctx;
 }
 
-void Check(int32_t st, const char* what) {
-  if (st == RECON_MVS_OK) return;
+thread_local int g_last_status = 0;
+
+// True when the call succeeded; false when it was cancelled (the caller
+// returns the reference's "nothing produced" value); throws MvsGpuError
+// otherwise.  Records the status for LastMvsGpuStatus.
+bool Check(int32_t st, const char* what) {
+  if (st == RECON_MVS_OK) return true;
+  g_last_status = st;
+  if (st == RECON_MVS_ERR_CANCELLED) return false;
   const int code = st == RECON_MVS_ERR_OUT_OF_MEMORY ? st : RECON_MVS_ERR_GPU_FAILURE;
+  g_last_status = code;
   throw MvsGpuError(code, std::string(what) + ": " + recon_mvs_hip_last_error(g_ctx));
 }
 
@@ -160,6 +187,25 @@ bool CameraIntrinsics::ProjectPoint(const Vec3d& p, double* u, double* v) const 
   return rmvs::Project(IntrOnly(*this), q, u, v);
 }
 
+// ---- mvs_gpu.h: status and cancellation ----
+int LastMvsGpuStatus() { return g_last_status; }
+void ClearMvsGpuStatus() { g_last_status = 0; }
+
+void SetMvsCancelCallback(int (*fn)(void* user), void* user) {
+  g_cancel_user.store(user, std::memory_order_release);
+  g_cancel_fn.store(fn, std::memory_order_release);
+}
+
+void CancelMvs() { g_cancel.store(true, std::memory_order_release); }
+
+void ClearMvsCancel() {
+  g_cancel.store(false, std::memory_order_release);
+  std::lock_guard<std::mutex> lock(g_mu);
+  if (g_ctx) (void)recon_mvs_hip_clear_cancel(g_ctx);
+}
+
+bool MvsCancelled() { return PollCancel(nullptr) != 0; }
+
 // ---- mvs.h entry points ----
 std::vector<int> SelectSourceViews(const CameraPose& ref_pose, const std::vector<CameraPose>& poses,
                                    const CameraIntrinsics& intrinsics, const MvsConfig& config) {
@@ -214,11 +260,14 @@ uint32_t PatchMatchStereo(const uint8_t* ref_gray, int ref_w, int ref_h,
     for (const auto& s : *seed_points) seeds.insert(seeds.end(), s.begin(), s.end());
   uint32_t filled = 0;
   std::lock_guard<std::mutex> lock(g_mu);
-  Check(recon_mvs_hip_patchmatch_stereo(Ctx(), &p, &cfg, seeds.empty() ? nullptr : seeds.data(),
-                                        static_cast<int32_t>(seeds.size() / 3),
-                                        out_depth->depth.data(), out_depth->support_count.data(),
-                                        out_depth->confidence.data(), &filled),
-        "PatchMatchStereo");
+  if (!Check(recon_mvs_hip_patchmatch_stereo(Ctx(), &p, &cfg, seeds.empty() ? nullptr : seeds.data(),
+                                             static_cast<int32_t>(seeds.size() / 3),
+                                             out_depth->depth.data(), out_depth->support_count.data(),
+                                             out_depth->confidence.data(), &filled),
+             "PatchMatchStereo")) {
+    out_depth->Allocate(ref_w, ref_h);  // cancelled: nothing produced
+    return 0;
+  }
   return filled;
 }
 
@@ -242,9 +291,10 @@ uint32_t RefineSubpixel(const uint8_t* ref_gray, int ref_w, int ref_h,
   const recon_mvs_config cfg = ToConfig(config);
   uint32_t refined = 0;
   std::lock_guard<std::mutex> lock(g_mu);
-  Check(recon_mvs_hip_refine_subpixel(Ctx(), &p, &cfg, depth->depth.data(),
-                                      depth->confidence.data(), &refined),
-        "RefineSubpixel");
+  if (!Check(recon_mvs_hip_refine_subpixel(Ctx(), &p, &cfg, depth->depth.data(),
+                                           depth->confidence.data(), &refined),
+             "RefineSubpixel"))
+    return 0;  // cancelled before the refinement ran: the map is unchanged
   return refined;
 }
 
@@ -310,10 +360,13 @@ uint32_t RunMvsStage(const std::vector<const uint8_t*>& gray, const std::vector<
   int32_t produced = 0;
   int64_t npts = 0;
   std::lock_guard<std::mutex> lock(g_mu);
-  Check(recon_mvs_hip_run_mvs_stage(Ctx(), static_cast<int32_t>(n), gray.data(), cams.data(), &cfg,
-                                    static_cast<int32_t>(mode), nullptr, dptr.data(), sptr.data(), cptr.data(),
-                                    &produced, &npts),
-        "RunMvsStage");
+  if (!Check(recon_mvs_hip_run_mvs_stage(Ctx(), static_cast<int32_t>(n), gray.data(), cams.data(),
+                                         &cfg, static_cast<int32_t>(mode), nullptr, dptr.data(),
+                                         sptr.data(), cptr.data(), &produced, &npts),
+             "RunMvsStage")) {
+    *out = MvsStageResult();  // cancelled: nothing produced
+    return 0;
+  }
   for (size_t k = 0; k < n; ++k) {
     DepthMap& d = out->depth_maps[k];
     bool any = false;

@@ -80,6 +80,15 @@ struct recon_mvs_hip_context {
   int device = 0;
   hipStream_t stream = nullptr;
   std::string error;
+  int32_t last_status = 0;  // status of the last failed call (0: none)
+  // cooperative cancellation: a flag in pinned host memory mapped into the
+  // device's address space (the persistent batch kernel polls it), set by
+  // recon_mvs_hip_cancel from any thread, or latched from the caller's
+  // callback at the library's check points
+  int32_t* cancel_host = nullptr;
+  int32_t* cancel_dev = nullptr;
+  recon_mvs_cancel_fn cancel_fn = nullptr;
+  void* cancel_user = nullptr;
   rmvs::DevBuf slots[rmvs::kSlotCount];
   void* pinned[2] = {nullptr, nullptr};
   size_t pinned_bytes[2] = {0, 0};
@@ -113,8 +122,33 @@ struct recon_mvs_hip_context {
 namespace rmvs {
 
 inline int32_t Fail(recon_mvs_hip_context* ctx, int32_t code, const std::string& msg) {
-  if (ctx) ctx->error = msg;
+  if (ctx) {
+    ctx->error = msg;
+    ctx->last_status = code;
+  }
   return code;
+}
+
+// Whether cancellation has been requested: the context's flag, or the
+// caller's callback (a yes is latched into the flag, so the device sees it
+// too).  Checked between dispatch batches: per keyframe, per directional
+// pass, per batch launch (docs/01_接口契约.md §3.11: GPU work is not
+// preempted; the flag is checked between batches).
+inline bool CancelRequested(recon_mvs_hip_context* ctx) {
+  if (__atomic_load_n(ctx->cancel_host, __ATOMIC_ACQUIRE)) return true;
+  if (ctx->cancel_fn && ctx->cancel_fn(ctx->cancel_user)) {
+    __atomic_store_n(ctx->cancel_host, 1, __ATOMIC_RELEASE);
+    return true;
+  }
+  return false;
+}
+
+// Waits for the work already queued on the context's streams, then reports
+// the cancellation.
+inline int32_t Cancelled(recon_mvs_hip_context* ctx) {
+  (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->copy_stream) (void)hipStreamSynchronize(ctx->copy_stream);
+  return Fail(ctx, RECON_MVS_ERR_CANCELLED, "cancelled");
 }
 
 #define RMVS_HIP(ctx, expr)                                                    \

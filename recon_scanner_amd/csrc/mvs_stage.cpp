@@ -77,8 +77,11 @@ extern "C" RECON_MVS_API int32_t recon_mvs_hip_run_mvs_stage(
     sup[r].assign(P, 0);
   };
   int32_t st = RECON_MVS_OK;
+  if (rmvs::CancelRequested(ctx)) return rmvs::Cancelled(ctx);
   if (mode == RECON_MVS_STAGE_REFERENCE) {
     for (int r = 0; r < n && st == RECON_MVS_OK; ++r) {
+      // the reference's check before every keyframe (orchestrator.cpp:989)
+      if (rmvs::CancelRequested(ctx)) return rmvs::Cancelled(ctx);
       if (!gray[r]) continue;
       const std::vector<int> kept = KeptSources(n, r, gray, cams, cfg);
       if (kept.empty()) continue;
@@ -155,6 +158,8 @@ extern "C" RECON_MVS_API int32_t recon_mvs_hip_run_mvs_stage(
         st = recon_mvs_hip_batch_create(ctx, static_cast<int32_t>(jobs.size()), jobs.data(), &pp, &b);
       if (st == RECON_MVS_OK) st = recon_mvs_hip_batch_run(ctx, b);
       if (b) recon_mvs_hip_batch_destroy(ctx, b);
+      // a cancel during the batch drained it early: its maps are partial
+      if (st == RECON_MVS_OK && rmvs::CancelRequested(ctx)) return rmvs::Cancelled(ctx);
       // the sub-pixel step (RefineSubpixel's place, orchestrator.cpp:1033-1036):
       // refinement along each ray, then the 10-step gradient polish where its
       // window is compiled
@@ -163,6 +168,7 @@ extern "C" RECON_MVS_API int32_t recon_mvs_hip_run_mvs_stage(
                           (pp.patch_size == 5 && pp.window_stride == 1);
       for (const auto& j : jobs) {
         if (st != RECON_MVS_OK) break;
+        if (rmvs::CancelRequested(ctx)) return rmvs::Cancelled(ctx);
         st = recon_mvs_hip_plane_refine(ctx, j.ref_view, j.num_src, j.src_views, &pp);
         if (st == RECON_MVS_OK && polish)
           st = recon_mvs_hip_plane_polish(ctx, j.ref_view, j.num_src, j.src_views, &pp, 10);
@@ -173,6 +179,7 @@ extern "C" RECON_MVS_API int32_t recon_mvs_hip_run_mvs_stage(
       for (const auto& j : jobs) has_map[frame_of[j.ref_view]] = 1;
       for (int r = 0; r < n && st == RECON_MVS_OK; ++r) {
         if (!has_map[r]) continue;
+        if (rmvs::CancelRequested(ctx)) return rmvs::Cancelled(ctx);
         std::vector<int32_t> list = {view_of[r]};
         for (int si : kept_of[r])
           if (has_map[si] && list.size() < 16) list.push_back(view_of[si]);

@@ -628,6 +628,7 @@ RECON_MVS_API int32_t recon_mvs_hip_plane_polish(recon_mvs_hip_context* ctx, int
                                                  int32_t num_src, const int32_t* src_views,
                                                  const recon_mvs_plane_params* params, int32_t steps) {
   if (!ctx) return RECON_MVS_ERR_INVALID_ARGUMENT;
+  if (rmvs::CancelRequested(ctx)) return rmvs::Cancelled(ctx);
   if (steps < 0 || steps > 1000) return rmvs::Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "steps out of [0, 1000]");
   AdjArgs g;
   int32_t st = MakeAdjArgs(ctx, ref_view, num_src, src_views, params, nullptr, &g);

@@ -145,14 +145,15 @@ __device__ __forceinline__ float4 BufQuad(__amdgpu_buffer_rsrc_t rsrc, int off) 
 
 // Reference window of one pixel: a pointer to its top-left sample in LDS
 // (row step rs floats, column step STRIDE), NaN marking samples outside the
-// image, plus the mean and the centred sums over the samples inside.
+// image; the count and mean of the samples inside, and their sums centred on
+// that mean (centred sums keep the FP32 variance free of cancellation).
 template <int HALF, int STRIDE>
 struct RefWindow {
   static constexpr int NS = 2 * HALF / STRIDE + 1;
   static constexpr int S = NS * NS;
   const float* base;  // LDS values are pre-shifted by `shift`
   int rs;
-  float sum_r, sum_r2, n, mean, shift;
+  float sum_c, sum_c2, n, mean, shift;
   bool full;
   __device__ __forceinline__ float at(int i, int j) const { return base[j * rs + i * STRIDE]; }
 };
@@ -175,21 +176,96 @@ __device__ __forceinline__ void MakeRefWindow(RefWindow<HALF, STRIDE>& win, cons
     }
   }
   const float mean = cnt > 0.0f ? s / cnt : 0.0f;
-  float sr = 0.0f, sr2 = 0.0f;
+  float sc = 0.0f, sc2 = 0.0f;
   for (int j = 0; j < NS; ++j) {
 #pragma unroll
     for (int i = 0; i < NS; ++i) {
       const float v0 = win.at(i, j);
-      const float v = (v0 == v0) ? v0 : 0.0f;
-      sr += v;
-      sr2 = fmaf(v, v, sr2);
+      const float c = (v0 == v0) ? v0 - mean : 0.0f;
+      sc += c;
+      sc2 = fmaf(c, c, sc2);
     }
   }
-  win.sum_r = sr;
-  win.sum_r2 = sr2;
+  win.sum_c = sc;
+  win.sum_c2 = sc2;
   win.n = cnt;
   win.mean = mean;
   win.full = (cnt == static_cast<float>(RefWindow<HALF, STRIDE>::S));
+}
+
+// ---------------------------------------------------------------------------
+// ALU helpers for the window loops: single instructions the compiler does not
+// select from plain code (written as one-instruction asm with register
+// operands only; no memory access)
+// ---------------------------------------------------------------------------
+
+// floor(x) as an integer (v_cvt_flr_i32_f32; the compiler would emit
+// v_floor_f32 + v_cvt_i32_f32).
+__device__ __forceinline__ int FloorInt(float x) {
+  int r;
+  asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+  return r;
+}
+
+// a * b + c for a, b in signed 24 bits (v_mad_i32_i24, full rate; a 32-bit
+// multiply is the quarter-rate v_mul_lo_u32).
+__device__ __forceinline__ int MadI24(int a, int b, int c) {
+  int r;
+  asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+  return r;
+}
+
+// The two horizontal interpolations of an FP16 row-pair quad, read straight
+// from the packed halves by mixed-precision FMAs (v_fma_mix_f32: FP16 inputs
+// convert exactly, each FMA rounds once in FP32).  u0 = (top-left, bottom-left),
+// u1 = (top-right, bottom-right); top = x00 + fx (x10 - x00), bot likewise:
+// the values FP32 arithmetic on the converted taps gives, in 4 instructions
+// instead of 8.
+__device__ __forceinline__ void F16Lerps(uint32_t u0, uint32_t u1, float fx, float* top, float* bot) {
+  float d0, d1, t, b;
+  asm("v_fma_mix_f32 %0, %1, -1.0, %2 op_sel_hi:[1,0,1]" : "=v"(d0) : "v"(u0), "v"(u1));
+  asm("v_fma_mix_f32 %0, %1, -1.0, %2 op_sel:[1,0,1] op_sel_hi:[1,0,1]" : "=v"(d1) : "v"(u0), "v"(u1));
+  asm("v_fma_mix_f32 %0, %1, %2, %3 op_sel_hi:[0,0,1]" : "=v"(t) : "v"(fx), "v"(d0), "v"(u0));
+  asm("v_fma_mix_f32 %0, %1, %2, %3 op_sel:[0,0,1] op_sel_hi:[0,0,1]" : "=v"(b) : "v"(fx), "v"(d1), "v"(u0));
+  *top = t;
+  *bot = b;
+}
+
+// Raw texels of one sample (see BufQuad): FP16 stays packed for F16Lerps.
+template <int FMT>
+struct Texel {
+  float4 q;
+};
+template <>
+struct Texel<RECON_MVS_PIXEL_F16> {
+  uint32_t u0, u1;
+};
+
+template <int FMT>
+__device__ __forceinline__ Texel<FMT> LoadTexel(__amdgpu_buffer_rsrc_t rsrc, int off) {
+  Texel<FMT> t;
+  if constexpr (FMT == RECON_MVS_PIXEL_F16) {
+    const auto u = __builtin_amdgcn_raw_buffer_load_b64(rsrc, off, 0, 0);
+    t.u0 = u[0];
+    t.u1 = u[1];
+  } else {
+    t.q = BufQuad<FMT>(rsrc, off);
+  }
+  return t;
+}
+
+// Bilinear value at fractions (fx, fy) of a texel quad: the same operations
+// (and roundings) in every storage format.
+template <int FMT>
+__device__ __forceinline__ float Bilinear(const Texel<FMT>& t, float fx, float fy) {
+  float top, bot;
+  if constexpr (FMT == RECON_MVS_PIXEL_F16) {
+    F16Lerps(t.u0, t.u1, fx, &top, &bot);
+  } else {
+    top = fmaf(fx, t.q.z - t.q.x, t.q.x);
+    bot = fmaf(fx, t.q.w - t.q.y, t.q.y);
+  }
+  return fmaf(fy, bot - top, top);
 }
 
 struct Hyp {
@@ -248,12 +324,30 @@ __device__ __forceinline__ void ResetCostAcc(CostAcc& acc) {
 
 // View v's NCC of the hypothesis (NaN when the view is invalid); a valid
 // view's cost 1 - NCC is added to `acc`.
+//
+// Sample positions are offsets from the window centre's projection (fu, fv)
+// (FP64-composed warp terms g, h2 relative to that centre):
+//   t = (f, g) + (g0. dx + g0. dy, g1. dx + g1. dy) / (1 + h20 dx + h21 dy).
+// A lane is "fast" when its whole window lies inside the source with a
+// 1e-4 px margin (a rigorous bound on the warp) and inside the reference: its
+// samples need no masks.  When every lane of the wave is fast the window runs
+// a loop without masks; otherwise a loop that checks each sample, in which
+// fast lanes perform exactly the operations of the mask-free loop -- a
+// pixel's result does not depend on which pixels share its wave (tile
+// schedules, view-phased sweeps and batches agree bit for bit).
+//
+// Sums: source values are shifted by the window's first sample (fast lanes)
+// or by the reference window's mean (others), reference values are those of
+// the LDS tile and their centred sums; every sum stays near its own scale, so
+// the FP32 variances and covariance do not cancel.
 template <int HALF, int STRIDE, int FMT>
 __device__ __forceinline__ float ViewNcc(const PlaneArgs& a, int v, const HypGeom& hg,
                                          const RefWindow<HALF, STRIDE>& win, bool use_topk,
                                          CostAcc& acc) {
   constexpr int NS = RefWindow<HALF, STRIDE>::NS;
   constexpr int S = RefWindow<HALF, STRIDE>::S;
+  constexpr int kEB = ElemBytes<FMT>();
+  constexpr int kLogEB = kEB == 8 ? 3 : kEB == 4 ? 2 : 1;
   const double qx = hg.qx, qy = hg.qy, D = hg.D, m0 = hg.m0, m1 = hg.m1, m2 = hg.m2;
   float* topk = acc.topk;
   float& csum = acc.csum;
@@ -278,9 +372,6 @@ __device__ __forceinline__ float ViewNcc(const PlaneArgs& a, int v, const HypGeo
       const double fsu = floor(su), fsv = floor(sv);
       const int iu = static_cast<int>(fsu), iv = static_cast<int>(fsv);
       const float fu = static_cast<float>(su - fsu), fv = static_cast<float>(sv - fsv);
-      // Offset of a sample from the centre projection, for a window offset
-      // (dx, dy): du = (g00 dx + g01 dy) / (1 + h20 dx + h21 dy), and dv
-      // likewise with g10, g11; g = (H_row - u_c H_2) / hc2 in FP64.
       const float g00 = static_cast<float>((H[0] - uc * H[6]) * ih);
       const float g01 = static_cast<float>((H[1] - uc * H[7]) * ih);
       const float g10 = static_cast<float>((H[3] - vc * H[6]) * ih);
@@ -289,14 +380,13 @@ __device__ __forceinline__ float ViewNcc(const PlaneArgs& a, int v, const HypGeo
       const int w = a.src_w[v], hgt = a.src_h[v], pitch = a.src_pitch[v];
       const float lox = static_cast<float>(-iu), hix = static_cast<float>(w - 1 - iu);
       const float loy = static_cast<float>(-iv), hiy = static_cast<float>(hgt - 1 - iv);
-      constexpr int kEB = ElemBytes<FMT>();
       const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
           const_cast<void*>(a.src_img[v]), 0, pitch * hgt * kEB, 0x00020000);
-      // the window origin's row/column in the source, as a float index
-      // offset (exact: image sizes stay below 2^24 pixels)
-      const float fpitch = static_cast<float>(pitch);
-      const float obase = static_cast<float>(iv) * fpitch + static_cast<float>(iu);
-      const float shift = win.shift;
+      // byte offset of the window-centre element (iv, iu) and of one element
+      // row (images stay below 2^24 elements: both fit in 32 bits, the row
+      // step in 24)
+      const int obase = (iv * pitch + iu) * kEB;
+      const int pitch_eb = pitch * kEB;
       // A window entirely beyond one image edge cannot give a valid view:
       // skip it.  Over the window |g0. dx + g0. dy| <= e0 (e1 for v) and
       // the denominator is >= 1 - e2, so |du| <= e0 / (1 - e2).
@@ -308,20 +398,17 @@ __device__ __forceinline__ float ViewNcc(const PlaneArgs& a, int v, const HypGeo
       // e2 < 0.5 keeps every sample's denominator positive
       const bool den_ok = e2 < 0.5f;
       const bool outside = den_ok && (fu + bu < lox || fu - bu > hix || fv + bv < loy || fv - bv > hiy);
-      // Sums over the source samples; the reference sums start from the
-      // whole window and lose the reference values of excluded samples.
-      float ss = 0.0f, sss = 0.0f, srs = 0.0f, xr = 0.0f, xrr = 0.0f, xcnt = 0.0f;
-      // A window whose projection lies inside the source image with a
-      // 1e-4 px margin by the bound above (and whose reference window is
-      // whole) has every sample valid: such windows skip the per-sample
-      // masks.  The test is per window, so the branch below is taken once
-      // per (hypothesis, view), not per row.
       const float mlox = lox + 1e-4f, mhix = hix - 1e-4f, mloy = loy + 1e-4f, mhiy = hiy - 1e-4f;
-      const bool win_in = den_ok & win.full & (fu - bu >= mlox) & (fu + bu <= mhix) &
-                          (fv - bv >= mloy) & (fv + bv <= mhiy);
+      const bool fast = den_ok & win.full & (fu - bu >= mlox) & (fu + bu <= mhix) &
+                        (fv - bv >= mloy) & (fv + bv <= mhiy);
+      // source sums (ss, sss, srs: values shifted by k_src) and the
+      // reference values of excluded samples, centred (xr, xrr), with their
+      // count xcnt
+      float ss = 0.0f, sss = 0.0f, srs = 0.0f, xr = 0.0f, xrr = 0.0f, xcnt = 0.0f;
+      float k_src = win.shift + win.mean;
       if (!outside) {
-        if (__ballot(!win_in) == 0) {
-          // every lane's window is inside: no per-sample masks
+        if (__ballot(!fast) == 0) {
+          // every lane is fast: no per-sample masks
 #pragma unroll 1
           for (int j = 0; j < NS; ++j) {
             const float dy = static_cast<float>(-HALF + j * STRIDE);
@@ -334,26 +421,26 @@ __device__ __forceinline__ float ViewNcc(const PlaneArgs& a, int v, const HypGeo
               const float r = __builtin_amdgcn_rcpf(fmaf(h20, dx, b2));
               const float tx = fmaf(fmaf(g00, dx, b0), r, fu);
               const float ty = fmaf(fmaf(g10, dx, b1), r, fv);
-              const float flx = floorf(tx), fly = floorf(ty);
-              fxs[i] = tx - flx;
-              fys[i] = ty - fly;
-              off[i] = static_cast<int>(fmaf(fly, fpitch, flx + obase)) * kEB;
+              fxs[i] = __builtin_amdgcn_fractf(tx);
+              fys[i] = __builtin_amdgcn_fractf(ty);
+              off[i] = MadI24(FloorInt(ty), pitch_eb, (FloorInt(tx) << kLogEB) + obase);
             }
-            float4 q[NS];
+            Texel<FMT> q[NS];
 #pragma unroll
-            for (int i = 0; i < NS; ++i) q[i] = BufQuad<FMT>(rsrc, off[i]);
+            for (int i = 0; i < NS; ++i) q[i] = LoadTexel<FMT>(rsrc, off[i]);
 #pragma unroll
             for (int i = 0; i < NS; ++i) {
-              const float top = fmaf(fxs[i], q[i].z - q[i].x, q[i].x);
-              const float bot = fmaf(fxs[i], q[i].w - q[i].y, q[i].y);
-              const float s = fmaf(fys[i], bot - top, top) - shift;
+              const float b = Bilinear<FMT>(q[i], fxs[i], fys[i]);
+              if (i == 0 && j == 0) k_src = b;
+              const float s = b - k_src;
               ss += s;
               sss = fmaf(s, s, sss);
               srs = fmaf(win.at(i, j), s, srs);
             }
           }
         } else {
-          // some window crosses an edge: each sample is checked
+          // some lane is not fast: each sample is checked; fast lanes take
+          // the operations of the loop above
 #pragma unroll 1
           for (int j = 0; j < NS; ++j) {
             const float dy = static_cast<float>(-HALF + j * STRIDE);
@@ -371,25 +458,23 @@ __device__ __forceinline__ float ViewNcc(const PlaneArgs& a, int v, const HypGeo
             float fxs[NS], fys[NS];
 #pragma unroll
             for (int i = 0; i < NS; ++i) {
-              const float flx = floorf(txs[i]), fly = floorf(tys[i]);
-              fxs[i] = txs[i] - flx;
-              fys[i] = tys[i] - fly;
-              off[i] = static_cast<int>(fmaf(fly, fpitch, flx + obase)) * kEB;
+              fxs[i] = __builtin_amdgcn_fractf(txs[i]);
+              fys[i] = __builtin_amdgcn_fractf(tys[i]);
+              off[i] = MadI24(FloorInt(tys[i]), pitch_eb, (FloorInt(txs[i]) << kLogEB) + obase);
             }
-            float4 q[NS];
+            Texel<FMT> q[NS];
 #pragma unroll
-            for (int i = 0; i < NS; ++i) q[i] = BufQuad<FMT>(rsrc, off[i]);
+            for (int i = 0; i < NS; ++i) q[i] = LoadTexel<FMT>(rsrc, off[i]);
 #pragma unroll
             for (int i = 0; i < NS; ++i) {
-              const float top = fmaf(fxs[i], q[i].z - q[i].x, q[i].x);
-              const float bot = fmaf(fxs[i], q[i].w - q[i].y, q[i].y);
-              const float sv = fmaf(fys[i], bot - top, top) - shift;
+              const float b = Bilinear<FMT>(q[i], fxs[i], fys[i]);
+              if (i == 0 && j == 0 && fast) k_src = b;
               const float rv = win.at(i, j);
               const bool rin = rv == rv;  // NaN: outside the reference image
-              const bool ok = win_in | (rin & (dens[i] > 0.0f) & (txs[i] >= lox) &
-                                        (txs[i] <= hix) & (tys[i] >= loy) & (tys[i] <= hiy));
-              const float s = ok ? sv : 0.0f;
-              const float x = (ok || !rin) ? 0.0f : rv;
+              const bool ok = fast | (rin & (dens[i] > 0.0f) & (txs[i] >= lox) & (txs[i] <= hix) &
+                                      (tys[i] >= loy) & (tys[i] <= hiy));
+              const float s = ok ? b - k_src : 0.0f;
+              const float x = (ok || !rin) ? 0.0f : rv - win.mean;
               ss += s;
               sss = fmaf(s, s, sss);
               srs = fmaf(ok ? rv : 0.0f, s, srs);
@@ -403,13 +488,13 @@ __device__ __forceinline__ float ViewNcc(const PlaneArgs& a, int v, const HypGeo
         xcnt = win.n;
       }
       const float cnt = win.n - xcnt;
-      const float sr = win.sum_r - xr, srr = win.sum_r2 - xrr;
       if (cnt >= static_cast<float>(S / 2) && cnt > 0.0f) {
         const float in = __builtin_amdgcn_rcpf(cnt);
-        const float mr = sr * in, ms = ss * in;
-        const float vr = fmaf(srr, in, -mr * mr);
+        const float mrc = (win.sum_c - xr) * in;       // reference mean - win.mean
+        const float ms = ss * in;
+        const float vr = fmaf(win.sum_c2 - xrr, in, -mrc * mrc);
         const float vs = fmaf(sss, in, -ms * ms);
-        const float cv = fmaf(srs, in, -mr * ms);
+        const float cv = fmaf(srs, in, -(mrc + win.mean) * ms);
         const float d2 = vr * vs;
         if (d2 >= a.eps2 && d2 > 0.0f) {
           float ncc = cv * __builtin_amdgcn_rsqf(d2);

@@ -376,16 +376,24 @@ __global__ __launch_bounds__(256) void k_phase_final(PlaneArgs a, PhaseState st,
 // Batched, persistent form: one launch processes (job, tile) units of many
 // reference views of any size from a work queue.  phase < 0: random init;
 // otherwise the half-sweep of colour phase & 1 of iteration phase >> 1.  Every
-// workgroup takes units until the queue is exhausted, then exits.
+// workgroup takes units until the queue is exhausted, then exits.  `cancel`
+// (the context's cancel flag in coherent host memory; may be null) is read
+// by thread 0 each time it takes a unit and obeyed at its next take (the read
+// overlaps the unit's work): once the flag is set, every workgroup finds the
+// queue exhausted after at most one more unit.
 template <int HALF, int STRIDE, int FMT, int TW, int TH>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STRIDE == 2 ? 3 : 1)))
-void k_plane_batch(const PlaneArgs* __restrict__ args,
-                                                     const int2* __restrict__ units, int num_units,
-                                                     unsigned int* __restrict__ counter, int phase) {
+void k_plane_batch(const PlaneArgs* __restrict__ args, const int2* __restrict__ units, int num_units,
+                   unsigned int* __restrict__ counter, int phase, const int32_t* cancel) {
   __shared__ RefTile<HALF, FMT, TW, TH> tile;
   __shared__ int s_unit;
+  int stop = 0;  // thread 0: the flag as read at the previous take
   for (;;) {
-    if (threadIdx.x == 0) s_unit = static_cast<int>(atomicAdd(counter, 1u));
+    if (threadIdx.x == 0) {
+      const int u = static_cast<int>(atomicAdd(counter, 1u));
+      s_unit = stop ? num_units : u;
+      if (cancel) stop = __hip_atomic_load(cancel, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
     __syncthreads();
     const int u = __builtin_amdgcn_readfirstlane(s_unit);
     if (u >= num_units) return;
@@ -848,6 +856,7 @@ struct BatchLaunch {
   int num_units;
   unsigned int* counter;
   int phase;
+  const int32_t* cancel;
 };
 
 // Arguments of one part of a view-phased half-sweep.
@@ -925,7 +934,8 @@ void LaunchTile(KernelId id, PlaneArgs a, hipStream_t s, float f0, int i0, int n
   if (id == kBatch) {
     hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, s, bl->counter);
     hipLaunchKernelGGL((k_plane_batch<HALF, STRIDE, FMT, TW, TH>), dim3(BatchGrid<HALF, STRIDE, FMT, TW, TH>(cap)),
-                       dim3(256), pad, s, bl->args, bl->units, bl->num_units, bl->counter, bl->phase);
+                       dim3(256), pad, s, bl->args, bl->units, bl->num_units, bl->counter, bl->phase,
+                       bl->cancel);
     return;
   }
   a.tiles_x = (a.w + TW - 1) / TW;
@@ -1394,6 +1404,7 @@ int32_t RunPlanePatchmatch(recon_mvs_hip_context* ctx, int32_t ref_view, int32_t
                            const int32_t* src_views, const recon_mvs_plane_params* params,
                            const double* seed_points, int32_t num_seeds) {
   if (!ctx) return RECON_MVS_ERR_INVALID_ARGUMENT;
+  if (CancelRequested(ctx)) return Cancelled(ctx);
   if (num_seeds < 0 || (num_seeds > 0 && !seed_points))
     return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "bad seed points");
   PlaneArgs a;
@@ -1586,6 +1597,7 @@ RECON_MVS_API int32_t recon_mvs_hip_plane_refine(recon_mvs_hip_context* ctx, int
                                                  int32_t num_src, const int32_t* src_views,
                                                  const recon_mvs_plane_params* params) {
   if (!ctx) return RECON_MVS_ERR_INVALID_ARGUMENT;
+  if (CancelRequested(ctx)) return Cancelled(ctx);
   PlaneArgs a;
   int32_t st = MakeArgs(ctx, ref_view, num_src, src_views, params, &a);
   if (st != RECON_MVS_OK) return st;
@@ -1637,6 +1649,7 @@ RECON_MVS_API int32_t recon_mvs_hip_geo_consistency(recon_mvs_hip_context* ctx, 
                                                     int32_t min_views) {
   (void)min_views;
   if (!ctx) return RECON_MVS_ERR_INVALID_ARGUMENT;
+  if (CancelRequested(ctx)) return Cancelled(ctx);
   if (num_maps < 1 || num_maps > kMaxGeo || !views)
     return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "num_maps out of [1, 16]");
   GeoArgs g;
@@ -1858,7 +1871,7 @@ RECON_MVS_API int32_t recon_mvs_hip_batch_create(recon_mvs_hip_context* ctx, int
   }
   int32_t st = RECON_MVS_OK;
   for (int phase = -1; phase < 2 * params->iterations && st == RECON_MVS_OK; ++phase) {
-    BatchLaunch bl{b->d_args, b->d_units, b->num_units, b->d_counter, phase};
+    BatchLaunch bl{b->d_args, b->d_units, b->num_units, b->d_counter, phase, ctx->cancel_dev};
     st = Launch(ctx, params, kBatch, args[0], 0.0f, 0, 0, nullptr, nullptr, nullptr, nullptr, &bl, sched,
                 caps[phase + 1]);
   }
@@ -1879,6 +1892,7 @@ RECON_MVS_API int32_t recon_mvs_hip_batch_create(recon_mvs_hip_context* ctx, int
 RECON_MVS_API int32_t recon_mvs_hip_batch_run(recon_mvs_hip_context* ctx, recon_mvs_hip_batch* b) {
   if (!ctx) return RECON_MVS_ERR_INVALID_ARGUMENT;
   if (!b) return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "null batch");
+  if (CancelRequested(ctx)) return Cancelled(ctx);
   hipEvent_t eb, ee;
   TimerBegin(ctx, kBatch, &eb, &ee);
   RMVS_HIP(ctx, hipGraphLaunch(b->exec, ctx->stream));

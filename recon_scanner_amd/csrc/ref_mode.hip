@@ -1345,6 +1345,7 @@ RECON_MVS_API int32_t recon_mvs_hip_patchmatch_stereo(recon_mvs_hip_context* ctx
   int pass = 0;
   for (int iter = 0; iter < config->iterations; ++iter) {
     for (int dir = 0; dir < dirs; ++dir, ++pass) {
+      if (CancelRequested(ctx)) return Cancelled(ctx);
       const int b = pass & 1;
       uint32_t* host = static_cast<uint32_t*>(EnsurePinned(ctx, b, tab_bytes));
       if (!host) return Fail(ctx, RECON_MVS_ERR_OUT_OF_MEMORY, "pinned allocation failed");
@@ -1399,6 +1400,7 @@ RECON_MVS_API int32_t recon_mvs_hip_refine_subpixel(recon_mvs_hip_context* ctx,
                                                     float* confidence, uint32_t* refined) {
   int32_t st = ValidateProblem(ctx, problem, config);
   if (st != RECON_MVS_OK) return st;
+  if (CancelRequested(ctx)) return Cancelled(ctx);
   if (!depth || !confidence || !refined) return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "null output");
   const int fit_points = std::max(3, config->subpixel_fit_points);
   if (fit_points > kMaxFitPoints)

@@ -129,6 +129,11 @@ def lib():
         L.recon_mvs_hip_last_error.argtypes = [ctxp]
         L.recon_mvs_hip_last_error.restype = C.c_char_p
         L.recon_mvs_hip_synchronize.argtypes = [ctxp]
+        L.recon_mvs_hip_last_status.argtypes = [ctxp]
+        L.recon_mvs_hip_cancel.argtypes = [ctxp]
+        L.recon_mvs_hip_is_cancelled.argtypes = [ctxp]
+        L.recon_mvs_hip_clear_cancel.argtypes = [ctxp]
+        L.recon_mvs_hip_set_cancel_callback.argtypes = [ctxp, CANCEL_FN, C.c_void_p]
         L.recon_mvs_hip_timing.argtypes = [ctxp, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p]
         L.recon_mvs_config_default.argtypes = [P(N.Config)]
         L.recon_mvs_config_default.restype = None
@@ -158,6 +163,39 @@ def lib():
                                                P(C.c_int32), C.c_void_p, C.c_void_p, C.c_void_p]
         _lib = L
     return _lib
+
+
+CANCEL_FN = C.CFUNCTYPE(C.c_int32, C.c_void_p)
+
+
+def Cancel():
+    """recon_mvs_hip_cancel on the process-wide context: thread-safe; running
+    and later calls return status 5 (cancelled) at their next check point."""
+    check(lib().recon_mvs_hip_cancel(context()), "cancel")
+
+
+def IsCancelled() -> bool:
+    return bool(lib().recon_mvs_hip_is_cancelled(context()))
+
+
+def ClearCancel():
+    check(lib().recon_mvs_hip_clear_cancel(context()), "clear_cancel")
+
+
+_cancel_cb = None
+
+
+def SetCancelCallback(fn):
+    """fn() -> bool polled at the library's check points (None removes it)."""
+    global _cancel_cb
+    _cancel_cb = None if fn is None else CANCEL_FN(lambda _u: 1 if fn() else 0)
+    check(lib().recon_mvs_hip_set_cancel_callback(context(), _cancel_cb or CANCEL_FN(), None),
+          "set_cancel_callback")
+
+
+def LastStatus() -> int:
+    """Status of the last failed call on the process-wide context (0: none)."""
+    return int(lib().recon_mvs_hip_last_status(context()))
 
 
 def context():
