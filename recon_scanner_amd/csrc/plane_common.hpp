@@ -336,10 +336,10 @@ __device__ __forceinline__ void ResetCostAcc(CostAcc& acc) {
 // pixel's result does not depend on which pixels share its wave (tile
 // schedules, view-phased sweeps and batches agree bit for bit).
 //
-// Sums: source values are shifted by the window's first sample (fast lanes)
-// or by the reference window's mean (others), reference values are those of
-// the LDS tile and their centred sums; every sum stays near its own scale, so
-// the FP32 variances and covariance do not cancel.
+// Sums: source values are shifted by the window's first valid sample,
+// reference values are those of the LDS tile and their centred sums; every
+// sum stays near its own scale, so the FP32 variances and covariance do not
+// cancel.
 template <int HALF, int STRIDE, int FMT>
 __device__ __forceinline__ float ViewNcc(const PlaneArgs& a, int v, const HypGeom& hg,
                                          const RefWindow<HALF, STRIDE>& win, bool use_topk,
@@ -405,7 +405,7 @@ __device__ __forceinline__ float ViewNcc(const PlaneArgs& a, int v, const HypGeo
       // reference values of excluded samples, centred (xr, xrr), with their
       // count xcnt
       float ss = 0.0f, sss = 0.0f, srs = 0.0f, xr = 0.0f, xrr = 0.0f, xcnt = 0.0f;
-      float k_src = win.shift + win.mean;
+      float k_src = 0.0f;
       if (!outside) {
         if (__ballot(!fast) == 0) {
           // every lane is fast: no per-sample masks
@@ -441,6 +441,7 @@ __device__ __forceinline__ float ViewNcc(const PlaneArgs& a, int v, const HypGeo
         } else {
           // some lane is not fast: each sample is checked; fast lanes take
           // the operations of the loop above
+          bool have_k = false;
 #pragma unroll 1
           for (int j = 0; j < NS; ++j) {
             const float dy = static_cast<float>(-HALF + j * STRIDE);
@@ -468,11 +469,14 @@ __device__ __forceinline__ float ViewNcc(const PlaneArgs& a, int v, const HypGeo
 #pragma unroll
             for (int i = 0; i < NS; ++i) {
               const float b = Bilinear<FMT>(q[i], fxs[i], fys[i]);
-              if (i == 0 && j == 0 && fast) k_src = b;
               const float rv = win.at(i, j);
               const bool rin = rv == rv;  // NaN: outside the reference image
               const bool ok = fast | (rin & (dens[i] > 0.0f) & (txs[i] >= lox) & (txs[i] <= hix) &
                                       (tys[i] >= loy) & (tys[i] <= hiy));
+              // the shift: the first valid sample (a fast lane's first sample,
+              // as in the loop above)
+              if (ok && !have_k) k_src = b;
+              have_k = have_k || ok;
               const float s = ok ? b - k_src : 0.0f;
               const float x = (ok || !rin) ? 0.0f : rv - win.mean;
               ss += s;

@@ -1,9 +1,11 @@
 """GPU parity of plane mode (FP32 on the GPU) against the FP64 oracle.
 
 Tolerances (stated per test):
-  * per-view NCC of a hypothesis: |NCC_gpu - NCC_oracle| <= 1e-5 (FP32 warp
-    offsets and sums vs FP64 exact homography), on views valid on both sides;
-  * aggregated cost: |c_gpu - c_oracle| <= 1e-5 * max(1, |c|);
+  * per-view NCC of a hypothesis: |NCC_gpu - NCC_oracle| <= 1e-5 for every
+    view valid on both sides (FP32 warp offsets and sums vs FP64 exact
+    homography; measured maximum 3.7e-6);
+  * aggregated cost: |c_gpu - c_oracle| <= 1e-5 * max(1, |c|) for every
+    hypothesis with the same valid views;
   * random init planes: bit-exact (both sides generate them with the same FP32
     operations from the same counter RNG);
   * final depth after the red-black sweeps: within 0.1 % of the oracle's on
@@ -59,12 +61,18 @@ def compare_eval(sc, srcs, p, storage, n=4000, seed=0):
     dn = np.abs(gv[both] - ov[both])
     say(storage, "views valid both", int(both.sum()), "validity agreement", agree,
         "max |dNCC|", float(dn.max()), "p99.9", float(np.quantile(dn, 0.999)))
+    d_all = np.where(both, np.abs(gv - ov), 0.0)
+    for q in np.argsort(d_all.max(1))[::-1][:3]:
+        v = int(np.argmax(d_all[q]))
+        say("  worst: hyp", hyps[q].tolist(), "view", v, "gpu", float(gv[q, v]), "oracle", float(ov[q, v]),
+            "nvalid", int(gn[q]), int(on[q]))
     assert agree > 0.999
     assert both.sum() > n
-    assert np.quantile(dn, 0.999) <= 1e-5 and dn.max() <= 1e-4
+    assert dn.max() <= 1e-5
     fin = (gc < 1e29) & (oc < 1e29) & (gn == on)
     dc = np.abs(gc[fin] - oc[fin]) / np.maximum(1.0, np.abs(oc[fin]))
-    assert np.quantile(dc, 0.999) <= 1e-5
+    say(storage, "aggregated cost: max relative |dc|", float(dc.max()))
+    assert dc.max() <= 1e-5
     return gc, oc
 
 
