@@ -542,6 +542,28 @@ RECON_MVS_API int32_t recon_mvs_hip_plane_cost_grad(recon_mvs_hip_context* ctx, 
                                                     double* grad_pose);
 
 /*
+ * recon_mvs_hip_plane_cost_grad without any device allocation, for callers
+ * that cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc); it has no counterpart in the reference.
+ * Same kernels and results (bit for bit) as recon_mvs_hip_plane_cost_grad,
+ * but d_weights (P floats, may be NULL), d_cost (P floats), d_grad_plane
+ * (P x 4 floats, 16-byte aligned) and d_grad_pose (6*num_src doubles) are
+ * DEVICE pointers (any may be NULL); pose_delta stays a host array (read on
+ * the host).  Asynchronous on the context's stream (recon_mvs_hip_synchronize
+ * before reading the outputs).  The per-tile pose partials live in a fixed
+ * 2 MiB device array per GPU: maps needing more (tiles of 32 x 16 pixels x
+ * num_src x 6 doubles, e.g. 1920x1440 with more than 8 sources) are refused
+ * with RECON_MVS_ERR_INVALID_ARGUMENT when d_grad_pose is requested.  All
+ * calls on one GPU share that array: calls that overlap on one GPU (two
+ * contexts, two threads) race.  The views, the plane state of ref_view and
+ * the output buffers must exist beforehand.
+ */
+RECON_MVS_API int32_t recon_mvs_hip_plane_cost_grad_noalloc(
+    recon_mvs_hip_context* ctx, int32_t ref_view, int32_t num_src, const int32_t* src_views,
+    const recon_mvs_plane_params* params, const double* pose_delta, const float* d_weights,
+    float* d_cost, float* d_grad_plane, double* d_grad_pose);
+
+/*
  * Gradient polish of the depth-map slot of ref_view (no counterpart in the
  * reference): `steps` steps (10 is the intended count); even steps move each
  * pixel's inverse depth, odd steps its normal, against the cost gradient by a

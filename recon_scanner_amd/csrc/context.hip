@@ -82,6 +82,9 @@ void TimerCollect(recon_mvs_hip_context* ctx) {
 using namespace rmvs;
 
 void DestroyPlaneState(recon_mvs_hip_context* ctx);  // plane_mode.hip
+namespace rmvs {
+int32_t AdjointScratchInit(int device);  // plane_adjoint.hip
+}
 
 extern "C" {
 
@@ -129,6 +132,11 @@ RECON_MVS_API int32_t recon_mvs_hip_create(int32_t device, recon_mvs_hip_context
   auto* ctx = new (std::nothrow) recon_mvs_hip_context();
   if (!ctx) return RECON_MVS_ERR_OUT_OF_MEMORY;
   ctx->device = device;
+  // the device address of the allocation-free adjoint's scratch, once per device
+  if (rmvs::AdjointScratchInit(device) != RECON_MVS_OK) {
+    delete ctx;
+    return RECON_MVS_ERR_GPU_FAILURE;
+  }
   // both streams at the default priority
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
     delete ctx;

@@ -30,6 +30,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "mvs_common.hpp"
@@ -38,6 +40,14 @@
 namespace {
 
 constexpr int kAdjTW = 32, kAdjTH = 16;  // tile of the adjoint kernels (256 threads)
+
+// Scratch of recon_mvs_hip_plane_cost_grad_noalloc: the per-tile FP64 pose
+// partials (tiles x num_src x 6 doubles) in a fixed 2 MiB device array -- a
+// 1920x1440 map (5400 tiles of 32 x 16) against 8 sources.  Each GPU the
+// library runs on holds its own copy until the process ends.  Host code uses
+// the address AdjointScratchInit obtained for the device, never this name.
+constexpr size_t kNoAllocScratchBytes = size_t{2} << 20;
+static __device__ __align__(256) double scratch[kNoAllocScratchBytes / sizeof(double)];
 constexpr int kTimerAdjoint = 7;
 
 struct AdjArgs {
@@ -522,7 +532,7 @@ void LaunchAdjT(AdjKernel k, const AdjArgs& g, dim3 grid, hipStream_t s) {
 // Window shapes of the adjoint: (11, 2) in every storage format, (7, 1) and
 // (5, 1) in FP32.
 int32_t LaunchAdj(recon_mvs_hip_context* ctx, const recon_mvs_plane_params* p, AdjKernel k,
-                  AdjArgs* g) {
+                  AdjArgs* g, bool timed = true) {
   PlaneArgs& a = g->a;
   a.tiles_x = (a.w + kAdjTW - 1) / kAdjTW;
   a.tiles_y = (a.h + kAdjTH - 1) / kAdjTH;
@@ -530,8 +540,8 @@ int32_t LaunchAdj(recon_mvs_hip_context* ctx, const recon_mvs_plane_params* p, A
   a.tiles_per_xcd = (a.num_tiles + 7) / 8;
   const dim3 grid(a.tiles_per_xcd * 8);
   const int fmt = ctx->storage_format, ps = p->patch_size, st = p->window_stride;
-  hipEvent_t eb, ee;
-  rmvs::TimerBegin(ctx, kTimerAdjoint, &eb, &ee);
+  hipEvent_t eb = nullptr, ee = nullptr;
+  if (timed) rmvs::TimerBegin(ctx, kTimerAdjoint, &eb, &ee);
   bool ok = true;
   if (ps == 11 && st == 2) {
     if (fmt == RECON_MVS_PIXEL_F32) LaunchAdjT<5, 2, RECON_MVS_PIXEL_F32>(k, *g, grid, ctx->stream);
@@ -548,10 +558,16 @@ int32_t LaunchAdj(recon_mvs_hip_context* ctx, const recon_mvs_plane_params* p, A
   if (!ok)
     return rmvs::Fail(ctx, RECON_MVS_ERR_NOT_IMPLEMENTED,
                       "adjoint: window (patch_size, window_stride) / storage format not compiled in");
-  rmvs::TimerEnd(ctx, kTimerAdjoint, eb, ee);
+  if (timed) rmvs::TimerEnd(ctx, kTimerAdjoint, eb, ee);
   RMVS_LAUNCH_CHECK(ctx);
   return RECON_MVS_OK;
 }
+
+// Device address of `scratch` per device, fetched once per device by
+// AdjointScratchInit (with that device current).
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxScratchDevices] = {};
 
 int32_t MakeAdjArgs(recon_mvs_hip_context* ctx, int32_t ref, int32_t nsrc, const int32_t* srcs,
                     const recon_mvs_plane_params* p, const double* pose_delta, AdjArgs* g) {
@@ -577,6 +593,21 @@ int32_t MakeAdjArgs(recon_mvs_hip_context* ctx, int32_t ref, int32_t nsrc, const
 }
 
 }  // namespace
+
+namespace rmvs {
+
+int32_t AdjointScratchInit(int device) {
+  if (device < 0 || device >= kMaxScratchDevices) return RECON_MVS_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return RECON_MVS_OK;
+  void* ptr = nullptr;
+  if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr)
+    return RECON_MVS_ERR_GPU_FAILURE;
+  g_scratch_ptr[device] = ptr;
+  return RECON_MVS_OK;
+}
+
+}  // namespace rmvs
 
 extern "C" {
 
@@ -621,6 +652,49 @@ RECON_MVS_API int32_t recon_mvs_hip_plane_cost_grad(recon_mvs_hip_context* ctx, 
   if (grad_plane)
     RMVS_HIP(ctx, hipMemcpyAsync(grad_plane, g.grad_plane, P * sizeof(float4), hipMemcpyDeviceToHost, ctx->stream));
   RMVS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return RECON_MVS_OK;
+}
+
+RECON_MVS_API int32_t recon_mvs_hip_plane_cost_grad_noalloc(
+    recon_mvs_hip_context* ctx, int32_t ref_view, int32_t num_src, const int32_t* src_views,
+    const recon_mvs_plane_params* params, const double* pose_delta, const float* d_weights,
+    float* d_cost, float* d_grad_plane, double* d_grad_pose) {
+  if (!ctx) return RECON_MVS_ERR_INVALID_ARGUMENT;
+  if (reinterpret_cast<uintptr_t>(d_grad_plane) % alignof(float4) != 0)
+    return rmvs::Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "cost_grad_noalloc: d_grad_plane not 16-byte aligned");
+  if (reinterpret_cast<uintptr_t>(d_grad_pose) % alignof(double) != 0)
+    return rmvs::Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "cost_grad_noalloc: d_grad_pose not 8-byte aligned");
+  AdjArgs g;
+  int32_t st = MakeAdjArgs(ctx, ref_view, num_src, src_views, params, pose_delta, &g);
+  if (st != RECON_MVS_OK) return st;
+  const PlaneArgs& a = g.a;
+  const size_t tiles = static_cast<size_t>((a.w + kAdjTW - 1) / kAdjTW) * ((a.h + kAdjTH - 1) / kAdjTH);
+  const size_t need = tiles * static_cast<size_t>(num_src) * 6 * sizeof(double);
+  if (d_grad_pose && need > kNoAllocScratchBytes)
+    return rmvs::Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT,
+                      "cost_grad_noalloc: pose partials (" + std::to_string(tiles) + " tiles x " +
+                          std::to_string(num_src) + " sources x 6 doubles = " + std::to_string(need) +
+                          " bytes) exceed the 2 MiB scratch");
+  double* part = nullptr;
+  if (d_grad_pose) {
+    const int dev = ctx->device;
+    void* ptr = (dev >= 0 && dev < kMaxScratchDevices) ? g_scratch_ptr[dev] : nullptr;
+    if (!ptr) return rmvs::Fail(ctx, RECON_MVS_ERR_GPU_FAILURE, "cost_grad_noalloc: no scratch address");
+    part = static_cast<double*>(ptr);
+  }
+  g.weights = d_weights;
+  g.grad_plane = reinterpret_cast<float4*>(d_grad_plane);
+  g.cost_out = d_cost;
+  g.pose_part = part;
+  // every tile writes all of its partials: nothing in the scratch needs
+  // clearing, and what an earlier call left there is never read
+  st = LaunchAdj(ctx, params, kAdjGrad, &g, false);
+  if (st != RECON_MVS_OK) return st;
+  if (d_grad_pose) {
+    hipLaunchKernelGGL(k_pose_reduce, dim3(1), dim3(128), 0, ctx->stream, part, g.a.num_tiles,
+                       num_src * 6, d_grad_pose);
+    RMVS_LAUNCH_CHECK(ctx);
+  }
   return RECON_MVS_OK;
 }
 
