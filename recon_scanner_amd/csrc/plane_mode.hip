@@ -1522,6 +1522,13 @@ RECON_MVS_API int32_t recon_mvs_hip_set_views(recon_mvs_hip_context* ctx, int32_
   RMVS_HIP(ctx, hipStreamSynchronize(ctx->stream));
   DestroyPlaneState(ctx);
   ctx->storage_format = storage_format;
+  // any failure below leaves the context without views (never a view whose
+  // pixels were not uploaded)
+  auto fail = [&](int32_t code, const std::string& msg) {
+    (void)hipStreamSynchronize(ctx->stream);
+    DestroyPlaneState(ctx);
+    return Fail(ctx, code, msg);
+  };
   const size_t esz = storage_format == RECON_MVS_PIXEL_F32 ? 4 : storage_format == RECON_MVS_PIXEL_U8 ? 1 : 2;
   for (int i = 0; i < n; ++i) {
     const recon_mvs_view& in = views[i];
@@ -1533,15 +1540,18 @@ RECON_MVS_API int32_t recon_mvs_hip_set_views(recon_mvs_hip_context* ctx, int32_
     v->cam = in.camera;
     ctx->views.push_back(v);
     const size_t bytes = static_cast<size_t>(v->pitch) * v->h * 2 * esz;
-    if (hipMalloc(&v->pixels, bytes) != hipSuccess)
-      return Fail(ctx, RECON_MVS_ERR_OUT_OF_MEMORY, "view allocation failed");
+    if (hipMalloc(&v->pixels, bytes) != hipSuccess) {
+      v->pixels = nullptr;
+      return fail(RECON_MVS_ERR_OUT_OF_MEMORY, "view allocation failed");
+    }
     const int src_stride = in.pixel_format == RECON_MVS_PIXEL_BGRA8
                                ? (in.row_stride ? in.row_stride : 4 * v->w)
                                : v->w * (in.pixel_format == RECON_MVS_PIXEL_U8 ? 1 : 4);
     const size_t in_bytes = static_cast<size_t>(src_stride) * v->h;
     void* staging = Ensure(ctx, kSlotQueryImgA, in_bytes);
-    if (!staging) return Fail(ctx, RECON_MVS_ERR_OUT_OF_MEMORY, "staging allocation failed");
-    RMVS_HIP(ctx, hipMemcpyAsync(staging, in.pixels, in_bytes, hipMemcpyHostToDevice, ctx->stream));
+    if (!staging) return fail(RECON_MVS_ERR_OUT_OF_MEMORY, "staging allocation failed");
+    hipError_t e = hipMemcpyAsync(staging, in.pixels, in_bytes, hipMemcpyHostToDevice, ctx->stream);
+    if (e != hipSuccess) return fail(RECON_MVS_ERR_GPU_FAILURE, std::string("view upload: ") + hipGetErrorString(e));
     const int total = v->pitch * v->h;
     const dim3 g((total + 255) / 256), b(256);
     if (storage_format == RECON_MVS_PIXEL_F32)
@@ -1552,9 +1562,11 @@ RECON_MVS_API int32_t recon_mvs_hip_set_views(recon_mvs_hip_context* ctx, int32_
       hipLaunchKernelGGL(k_store_view<RECON_MVS_PIXEL_BF16>, g, b, 0, ctx->stream, staging, in.pixel_format, src_stride, v->w, v->h, v->pitch, v->pixels);
     else
       hipLaunchKernelGGL(k_store_view<RECON_MVS_PIXEL_U8>, g, b, 0, ctx->stream, staging, in.pixel_format, src_stride, v->w, v->h, v->pitch, v->pixels);
-    RMVS_LAUNCH_CHECK(ctx);
+    e = hipGetLastError();
+    if (e != hipSuccess) return fail(RECON_MVS_ERR_GPU_FAILURE, std::string("view store: ") + hipGetErrorString(e));
     // staging is reused by the next view: finish this one first
-    RMVS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return fail(RECON_MVS_ERR_GPU_FAILURE, std::string("view upload: ") + hipGetErrorString(e));
   }
   ctx->states.assign(ctx->views.size(), nullptr);
   return RECON_MVS_OK;

@@ -17,6 +17,7 @@
 #include "recon/camera.h"
 #include "recon/mvs.h"
 #include "recon/mvs_cache.h"
+#include "recon/mvs_gpu.h"
 #include "recon_mvs_hip.h"
 
 extern "C" int32_t recon_scene_reference_plane(int32_t num_views, double baseline_m,
@@ -180,6 +181,33 @@ int main() {
     Expect(recon::PatchMatchStereo(imgs.data(), W, H, src, sw, sh, poses[0], poses, intr, bad,
                                    sources, cfg, nullptr, &d2) == 0,
            "ambiguous intrinsics -> 0");
+  }
+  // GPU controls (recon/mvs_gpu.h): a cancelled PatchMatchStereo produces
+  // nothing (the reference's skip value 0) and reports status 5; a callback
+  // cancels at its own poll; clearing restores normal runs
+  {
+    recon::ClearMvsGpuStatus();
+    recon::CancelMvs();
+    Expect(recon::MvsCancelled(), "CancelMvs sets the cancelled state");
+    recon::DepthMap dc;
+    Expect(recon::PatchMatchStereo(imgs.data(), W, H, src, sw, sh, poses[0], poses, intr, src_intr,
+                                   sources, cfg, nullptr, &dc) == 0 &&
+               recon::LastMvsGpuStatus() == 5,
+           "cancelled PatchMatchStereo -> 0, LastMvsGpuStatus 5");
+    recon::ClearMvsCancel();
+    Expect(!recon::MvsCancelled(), "ClearMvsCancel clears it");
+    int polls = 0;
+    recon::SetMvsCancelCallback([](void* u) { return ++*static_cast<int*>(u) >= 3 ? 1 : 0; }, &polls);
+    Expect(recon::PatchMatchStereo(imgs.data(), W, H, src, sw, sh, poses[0], poses, intr, src_intr,
+                                   sources, cfg, nullptr, &dc) == 0 && polls == 3,
+           "the callback cancels at its third poll (the third directional pass)");
+    recon::SetMvsCancelCallback(nullptr, nullptr);
+    recon::ClearMvsCancel();
+    recon::ClearMvsGpuStatus();
+    Expect(recon::PatchMatchStereo(imgs.data(), W, H, src, sw, sh, poses[0], poses, intr, src_intr,
+                                   sources, cfg, nullptr, &dc) > 0 &&
+               recon::LastMvsGpuStatus() == 0,
+           "after clearing, PatchMatchStereo runs");
   }
   std::printf(g_fail ? "FAILED %d\n" : "ALL PASSED\n", g_fail);
   return g_fail ? 1 : 0;

@@ -253,3 +253,47 @@ def test_ragged_tiles_and_source_sizes(patch):
     assert n == on and n > 0
     assert bits_equal(dm.depth.reshape(67, 101), odr)
     assert bits_equal(dm.confidence.reshape(67, 101), ocr)
+
+
+def test_distorted_cameras_match_reference():
+    """Cameras with radial and tangential distortion (k1, k2, k3, p1, p2 all
+    non-zero, different per view): the device camera arithmetic -- the
+    Newton iterations of RemoveDistortion and ApplyDistortion
+    (camera.cpp:47-129) inside every projection -- reproduces the reference
+    bit for bit: EvaluateCandidate costs against the oracle (itself pinned to
+    the compiled reference with distortion on the host), PatchMatchStereo and
+    RefineSubpixel outputs against the reference's own code compiled from its
+    sources."""
+    say("reference mode with distorted cameras, sphere_arc 96x72")
+    sc = scenes.sphere_arc(width=96, height=72, focal=150.0)
+    u8 = [sc.u8(i) for i in range(5)]
+    dists = [(-0.08, 0.02, 0.003, 0.001, -0.0005), (-0.05, 0.01, 0.0, -0.0008, 0.0006),
+             (0.04, -0.015, 0.002, 0.0005, 0.0004), (-0.1, 0.03, -0.004, 0.0, 0.001),
+             (0.02, 0.0, 0.001, -0.001, -0.0002)]
+    cams = [N.Camera.make(c.fx, c.fy, c.cx, c.cy, c.width, c.height, c.R_np(), c.t_np(), d)
+            for c, d in zip(sc.cams, dists)]
+    cfg = mvs.MvsConfig(patch_size=7, iterations=2, num_threads=3, min_depth_m=0.25,
+                        max_depth_m=0.45)
+    rng = np.random.default_rng(9)
+    q = np.stack([rng.uniform(0, 96, 1500), rng.uniform(0, 72, 1500), rng.uniform(2.3, 3.9, 1500)], 1)
+    g = mvs.EvaluateCandidates(u8[0], u8[1:], cams[0], cams[1:], cfg, q)
+    o = oracle.orc_evaluate(u8[0], u8[1:], cams[0], cams[1:], cfg, q)
+    assert bits_equal(np.array([x.cost for x in g]), np.array([x.cost for x in o]))
+    assert sum(x.cost < 1e29 for x in g) > 500
+    if not oracle.reference_available():
+        pytest.skip("the reference's compiled CPU path (oracle/_ref) is not built")
+    srcs = [1, 2, 3, 4]
+    f, dm = run_pm(u8, cams, cfg, srcs)
+    rf, rd, rs, rc = oracle.ref_patchmatch(u8[0], [u8[i] for i in srcs], cams[0], cams, cams, srcs, cfg)
+    say("filled", f, rf)
+    assert f == rf and f > 0.5 * 96 * 72
+    assert bits_equal(dm.depth.reshape(72, 96), rd)
+    assert bits_equal(dm.support_count.reshape(72, 96), rs)
+    assert bits_equal(dm.confidence.reshape(72, 96), rc)
+    n = mvs.RefineSubpixel(u8[0], 96, 72, [u8[i] for i in srcs], [96] * 4, [72] * 4, pose(cams[0]),
+                           [pose(c) for c in cams], intr(cams[0]), [intr(c) for c in cams], srcs, cfg, dm)
+    rn, rdr, rcr = oracle.ref_refine(u8[0], [u8[i] for i in srcs], cams[0], cams, cams, srcs, cfg, rd, rs, rc)
+    say("refined", n, rn)
+    assert n == rn and n > 0
+    assert bits_equal(dm.depth.reshape(72, 96), rdr)
+    assert bits_equal(dm.confidence.reshape(72, 96), rcr)
