@@ -374,29 +374,50 @@ __global__ __launch_bounds__(256) void k_phase_final(PlaneArgs a, PhaseState st,
 }
 
 // Batched, persistent form: one launch processes (job, tile) units of many
-// reference views of any size from a work queue.  phase < 0: random init;
-// otherwise the half-sweep of colour phase & 1 of iteration phase >> 1.  Every
-// workgroup takes units until the queue is exhausted, then exits.  `cancel`
-// (the context's cancel flag in coherent host memory; may be null) is read
-// by thread 0 each time it takes a unit and obeyed at its next take (the read
-// overlaps the unit's work): once the flag is set, every workgroup finds the
-// queue exhausted after at most one more unit.
+// reference views of any size.  phase < 0: random init; otherwise the
+// half-sweep of colour phase & 1 of iteration phase >> 1.
+//
+// The units are split into one queue per XCD (queue q: units [qoff[q],
+// qoff[q + 1])), each holding a vertical band of every job's tiles, in
+// column order (the layout TileOf gives a single view).  Workgroup ids are
+// dealt round-robin to the XCDs, so workgroup b serves queue b & 7 -- the
+// tiles in flight on an XCD form a compact block whose source footprints
+// stay in that XCD's L2 -- and, once that queue is empty, the others in turn.
+// Every workgroup exits after finding each queue empty once (the counters
+// only grow).  `cancel` (the context's cancel flag in coherent host memory;
+// may be null) is read by thread 0 each time it takes a unit and obeyed at
+// its next take (the read overlaps the unit's work): once the flag is set,
+// every workgroup stops after at most one more unit.
+constexpr int kXcds = 8;
+constexpr int kCounterStride = 32;  // queue counters 128 bytes apart
+
 template <int HALF, int STRIDE, int FMT, int TW, int TH>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STRIDE == 2 ? 3 : 1)))
-void k_plane_batch(const PlaneArgs* __restrict__ args, const int2* __restrict__ units, int num_units,
-                   unsigned int* __restrict__ counter, int phase, const int32_t* cancel) {
+void k_plane_batch(const PlaneArgs* __restrict__ args, const int2* __restrict__ units,
+                   const int* __restrict__ qoff, unsigned int* __restrict__ counters, int phase,
+                   const int32_t* cancel) {
   __shared__ RefTile<HALF, FMT, TW, TH> tile;
   __shared__ int s_unit;
-  int stop = 0;  // thread 0: the flag as read at the previous take
+  int stop = 0;    // thread 0: the flag as read at the previous take
+  int tried = 0;   // thread 0: queues found empty (own queue first)
   for (;;) {
     if (threadIdx.x == 0) {
-      const int u = static_cast<int>(atomicAdd(counter, 1u));
-      s_unit = stop ? num_units : u;
+      int u = -1;
+      while (!stop && tried < kXcds) {
+        const int q = (static_cast<int>(blockIdx.x) + tried) & (kXcds - 1);
+        const int k = static_cast<int>(atomicAdd(&counters[q * kCounterStride], 1u));
+        if (qoff[q] + k < qoff[q + 1]) {
+          u = qoff[q] + k;
+          break;
+        }
+        ++tried;
+      }
+      s_unit = u;
       if (cancel) stop = __hip_atomic_load(cancel, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
     __syncthreads();
     const int u = __builtin_amdgcn_readfirstlane(s_unit);
-    if (u >= num_units) return;
+    if (u < 0) return;
     const int2 un = units[u];
     const int job = __builtin_amdgcn_readfirstlane(un.x);
     const int t = __builtin_amdgcn_readfirstlane(un.y);
@@ -408,8 +429,10 @@ void k_plane_batch(const PlaneArgs* __restrict__ args, const int2* __restrict__ 
   }
 }
 
-__global__ void k_reset_counter(unsigned int* c) {
-  if (threadIdx.x == 0) *c = 0u;
+// Clears the batch's queue counters (a kernel, not a memset: the launches
+// are captured into a graph).
+__global__ void k_reset_counters(unsigned int* c) {
+  if (threadIdx.x < kXcds) c[threadIdx.x * kCounterStride] = 0u;
 }
 
 // Scores a given plane state (both colours).
@@ -853,8 +876,8 @@ enum KernelId {
 struct BatchLaunch {
   const PlaneArgs* args;
   const int2* units;
-  int num_units;
-  unsigned int* counter;
+  const int* qoff;
+  unsigned int* counters;
   int phase;
   const int32_t* cancel;
 };
@@ -932,9 +955,9 @@ void LaunchTile(KernelId id, PlaneArgs a, hipStream_t s, float f0, int i0, int n
     else if (id == kBatch) pad = CapBytes(cap, lds_batch);
   }
   if (id == kBatch) {
-    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, s, bl->counter);
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, s, bl->counters);
     hipLaunchKernelGGL((k_plane_batch<HALF, STRIDE, FMT, TW, TH>), dim3(BatchGrid<HALF, STRIDE, FMT, TW, TH>(cap)),
-                       dim3(256), pad, s, bl->args, bl->units, bl->num_units, bl->counter, bl->phase,
+                       dim3(256), pad, s, bl->args, bl->units, bl->qoff, bl->counters, bl->phase,
                        bl->cancel);
     return;
   }
@@ -1355,7 +1378,8 @@ struct recon_mvs_hip_batch {
   PlaneArgs* d_args = nullptr;
   int2* d_units = nullptr;
   int num_units = 0;
-  unsigned int* d_counter = nullptr;
+  int* d_qoff = nullptr;             // kXcds + 1 queue offsets into d_units
+  unsigned int* d_counter = nullptr; // kXcds queue counters, kCounterStride apart
   hipGraphExec_t exec = nullptr;
 };
 
@@ -1840,19 +1864,30 @@ RECON_MVS_API int32_t recon_mvs_hip_batch_create(recon_mvs_hip_context* ctx, int
   }
   int tw, th;
   ScheduleTile(sched, &tw, &th);
-  // work units, most views first (largest units leave the queue first)
+  // work units, most views first (largest units leave the queues first);
+  // per XCD queue a vertical band of every job: tile c in column order goes
+  // to queue c / tiles_per_xcd (TileOf's layout)
   std::vector<int> order(num_jobs);
   for (int j = 0; j < num_jobs; ++j) order[j] = j;
   std::stable_sort(order.begin(), order.end(),
                    [&](int x, int y) { return args[x].nsrc > args[y].nsrc; });
-  std::vector<int2> units;
+  std::vector<std::vector<int2>> queue(kXcds);
   for (int j : order) {
     PlaneArgs& a = args[j];
     a.tiles_x = (a.w + tw - 1) / tw;
     a.tiles_y = (a.h + th - 1) / th;
     a.num_tiles = a.tiles_x * a.tiles_y;
-    a.tiles_per_xcd = (a.num_tiles + 7) / 8;
-    for (int t = 0; t < a.num_tiles; ++t) units.push_back(make_int2(j, t));
+    a.tiles_per_xcd = (a.num_tiles + kXcds - 1) / kXcds;
+    for (int c = 0; c < a.num_tiles; ++c) {
+      const int tx = c / a.tiles_y, ty = c % a.tiles_y;
+      queue[c / a.tiles_per_xcd].push_back(make_int2(j, ty * a.tiles_x + tx));
+    }
+  }
+  std::vector<int2> units;
+  std::vector<int> qoff(kXcds + 1, 0);
+  for (int q = 0; q < kXcds; ++q) {
+    units.insert(units.end(), queue[q].begin(), queue[q].end());
+    qoff[q + 1] = static_cast<int>(units.size());
   }
   auto* b = new recon_mvs_hip_batch();
   b->params = *params;
@@ -1862,16 +1897,19 @@ RECON_MVS_API int32_t recon_mvs_hip_batch_create(recon_mvs_hip_context* ctx, int
   auto cleanup = [&](int32_t code, const char* msg) {
     if (b->d_args) (void)hipFree(b->d_args);
     if (b->d_units) (void)hipFree(b->d_units);
+    if (b->d_qoff) (void)hipFree(b->d_qoff);
     if (b->d_counter) (void)hipFree(b->d_counter);
     delete b;
     return Fail(ctx, code, msg);
   };
   if (hipMalloc(&b->d_args, sizeof(PlaneArgs) * num_jobs) != hipSuccess ||
       hipMalloc(&b->d_units, sizeof(int2) * units.size()) != hipSuccess ||
-      hipMalloc(&b->d_counter, 256) != hipSuccess)
+      hipMalloc(&b->d_qoff, sizeof(int) * qoff.size()) != hipSuccess ||
+      hipMalloc(&b->d_counter, sizeof(unsigned int) * kXcds * kCounterStride) != hipSuccess)
     return cleanup(RECON_MVS_ERR_OUT_OF_MEMORY, "batch allocation failed");
   if (hipMemcpy(b->d_args, args.data(), sizeof(PlaneArgs) * num_jobs, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(b->d_units, units.data(), sizeof(int2) * units.size(), hipMemcpyHostToDevice) != hipSuccess)
+      hipMemcpy(b->d_units, units.data(), sizeof(int2) * units.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->d_qoff, qoff.data(), sizeof(int) * qoff.size(), hipMemcpyHostToDevice) != hipSuccess)
     return cleanup(RECON_MVS_ERR_GPU_FAILURE, "batch upload failed");
   // capture init + all half-sweeps as one graph
   const bool timing = ctx->timing;
@@ -1883,7 +1921,7 @@ RECON_MVS_API int32_t recon_mvs_hip_batch_create(recon_mvs_hip_context* ctx, int
   }
   int32_t st = RECON_MVS_OK;
   for (int phase = -1; phase < 2 * params->iterations && st == RECON_MVS_OK; ++phase) {
-    BatchLaunch bl{b->d_args, b->d_units, b->num_units, b->d_counter, phase, ctx->cancel_dev};
+    BatchLaunch bl{b->d_args, b->d_units, b->d_qoff, b->d_counter, phase, ctx->cancel_dev};
     st = Launch(ctx, params, kBatch, args[0], 0.0f, 0, 0, nullptr, nullptr, nullptr, nullptr, &bl, sched,
                 caps[phase + 1]);
   }
@@ -1922,6 +1960,7 @@ RECON_MVS_API int32_t recon_mvs_hip_batch_destroy(recon_mvs_hip_context* ctx, re
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->d_args) (void)hipFree(b->d_args);
   if (b->d_units) (void)hipFree(b->d_units);
+  if (b->d_qoff) (void)hipFree(b->d_qoff);
   if (b->d_counter) (void)hipFree(b->d_counter);
   delete b;
   return RECON_MVS_OK;
