@@ -1340,6 +1340,64 @@ int32_t ResolveCaps(recon_mvs_hip_context* ctx, const PlaneArgs& a, const recon_
   return RECON_MVS_OK;
 }
 
+// Work units of a batch under tile schedule `sched` (sets each job's tile
+// counts): one queue per XCD holding a vertical band of every job, jobs in
+// `order` -- tile c in column order goes to queue c / tiles_per_xcd
+// (TileOf's layout for a single view).
+void BatchUnits(std::vector<PlaneArgs>* args, const std::vector<int>& order, int sched,
+                std::vector<int2>* units, std::vector<int>* qoff) {
+  int tw, th;
+  ScheduleTile(sched, &tw, &th);
+  std::vector<std::vector<int2>> queue(kXcds);
+  for (int j : order) {
+    PlaneArgs& a = (*args)[j];
+    a.tiles_x = (a.w + tw - 1) / tw;
+    a.tiles_y = (a.h + th - 1) / th;
+    a.num_tiles = a.tiles_x * a.tiles_y;
+    a.tiles_per_xcd = (a.num_tiles + kXcds - 1) / kXcds;
+    for (int c = 0; c < a.num_tiles; ++c) {
+      const int tx = c / a.tiles_y, ty = c % a.tiles_y;
+      queue[c / a.tiles_per_xcd].push_back(make_int2(j, ty * a.tiles_x + tx));
+    }
+  }
+  units->clear();
+  qoff->assign(kXcds + 1, 0);
+  for (int q = 0; q < kXcds; ++q) {
+    units->insert(units->end(), queue[q].begin(), queue[q].end());
+    (*qoff)[q + 1] = static_cast<int>(units->size());
+  }
+}
+
+// Times every phase of a batch (init, then each half-sweep) as eager
+// launches with events between them, phase ph capped at caps[ph] workgroups
+// per CU (0: none); ms[ph] receives each launch's time.
+int32_t TimeBatchPhases(recon_mvs_hip_context* ctx, const recon_mvs_plane_params* p, const PlaneArgs& a0,
+                        BatchLaunch bl, int sched, const std::vector<int>& caps, std::vector<float>* ms) {
+  const int nph = 1 + 2 * p->iterations;
+  std::vector<hipEvent_t> ev(nph + 1, nullptr);
+  int32_t st = RECON_MVS_OK;
+  for (auto& e : ev)
+    if (hipEventCreate(&e) != hipSuccess) st = Fail(ctx, RECON_MVS_ERR_GPU_FAILURE, "event create failed");
+  if (st == RECON_MVS_OK && hipEventRecord(ev[0], ctx->stream) != hipSuccess)
+    st = Fail(ctx, RECON_MVS_ERR_GPU_FAILURE, "event record failed");
+  for (int ph = 0; ph < nph && st == RECON_MVS_OK; ++ph) {
+    bl.phase = ph - 1;
+    st = Launch(ctx, p, kBatch, a0, 0.0f, 0, 0, nullptr, nullptr, nullptr, nullptr, &bl, sched, caps[ph]);
+    if (st == RECON_MVS_OK && hipEventRecord(ev[ph + 1], ctx->stream) != hipSuccess)
+      st = Fail(ctx, RECON_MVS_ERR_GPU_FAILURE, "event record failed");
+  }
+  if (st == RECON_MVS_OK && hipEventSynchronize(ev[nph]) != hipSuccess)
+    st = Fail(ctx, RECON_MVS_ERR_GPU_FAILURE, "batch tuning run failed");
+  ms->assign(nph, 0.0f);
+  for (int ph = 0; ph < nph && st == RECON_MVS_OK; ++ph)
+    if (hipEventElapsedTime(&(*ms)[ph], ev[ph], ev[ph + 1]) != hipSuccess)
+      st = Fail(ctx, RECON_MVS_ERR_GPU_FAILURE, "event timing failed");
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto& e : ev)
+    if (e) (void)hipEventDestroy(e);
+  return st;
+}
+
 }  // namespace
 
 namespace rmvs {
@@ -1847,53 +1905,27 @@ RECON_MVS_API int32_t recon_mvs_hip_batch_create(recon_mvs_hip_context* ctx, int
     args[j].cost = S.cost;
     args[j].support = S.support;
   }
-  // tile schedule and occupancy caps: given, or tuned on the job with the
-  // most pixel-views
-  int big = 0;
-  for (int j = 1; j < num_jobs; ++j)
-    if (static_cast<double>(args[j].w) * args[j].h * args[j].nsrc >
-        static_cast<double>(args[big].w) * args[big].h * args[big].nsrc)
-      big = j;
-  int sched = params->tile_schedule;
-  std::vector<int> caps;
-  {
-    int32_t st = RECON_MVS_OK;
-    if (sched < 0) st = ResolveSchedule(ctx, args[big], params, &sched);
-    if (st == RECON_MVS_OK) st = ResolveCaps(ctx, args[big], params, sched, &caps);
-    if (st != RECON_MVS_OK) return st;
-  }
-  int tw, th;
-  ScheduleTile(sched, &tw, &th);
-  // work units, most views first (largest units leave the queues first);
-  // per XCD queue a vertical band of every job: tile c in column order goes
-  // to queue c / tiles_per_xcd (TileOf's layout)
+  // work units, most views first (largest units leave the queues first)
   std::vector<int> order(num_jobs);
   for (int j = 0; j < num_jobs; ++j) order[j] = j;
   std::stable_sort(order.begin(), order.end(),
                    [&](int x, int y) { return args[x].nsrc > args[y].nsrc; });
-  std::vector<std::vector<int2>> queue(kXcds);
-  for (int j : order) {
-    PlaneArgs& a = args[j];
-    a.tiles_x = (a.w + tw - 1) / tw;
-    a.tiles_y = (a.h + th - 1) / th;
-    a.num_tiles = a.tiles_x * a.tiles_y;
-    a.tiles_per_xcd = (a.num_tiles + kXcds - 1) / kXcds;
-    for (int c = 0; c < a.num_tiles; ++c) {
-      const int tx = c / a.tiles_y, ty = c % a.tiles_y;
-      queue[c / a.tiles_per_xcd].push_back(make_int2(j, ty * a.tiles_x + tx));
-    }
-  }
-  std::vector<int2> units;
-  std::vector<int> qoff(kXcds + 1, 0);
-  for (int q = 0; q < kXcds; ++q) {
-    units.insert(units.end(), queue[q].begin(), queue[q].end());
-    qoff[q + 1] = static_cast<int>(units.size());
+  const int nph = 1 + 2 * params->iterations;
+  // candidate tile schedules: the given one, or every schedule (those not
+  // compiled for this window / format fail to launch and drop out)
+  std::vector<int> scheds;
+  if (params->tile_schedule >= 0) scheds.push_back(params->tile_schedule);
+  else for (int sc = 0; sc < kNumSchedules; ++sc) scheds.push_back(sc);
+  size_t max_units = 0;
+  for (int sc : scheds) {
+    std::vector<int2> u;
+    std::vector<int> q;
+    BatchUnits(&args, order, sc, &u, &q);
+    max_units = std::max(max_units, u.size());
   }
   auto* b = new recon_mvs_hip_batch();
   b->params = *params;
-  b->schedule = sched;
   for (int j = 0; j < num_jobs; ++j) b->refs.push_back(jobs[j].ref_view);
-  b->num_units = static_cast<int>(units.size());
   auto cleanup = [&](int32_t code, const char* msg) {
     if (b->d_args) (void)hipFree(b->d_args);
     if (b->d_units) (void)hipFree(b->d_units);
@@ -1903,14 +1935,78 @@ RECON_MVS_API int32_t recon_mvs_hip_batch_create(recon_mvs_hip_context* ctx, int
     return Fail(ctx, code, msg);
   };
   if (hipMalloc(&b->d_args, sizeof(PlaneArgs) * num_jobs) != hipSuccess ||
-      hipMalloc(&b->d_units, sizeof(int2) * units.size()) != hipSuccess ||
-      hipMalloc(&b->d_qoff, sizeof(int) * qoff.size()) != hipSuccess ||
+      hipMalloc(&b->d_units, sizeof(int2) * max_units) != hipSuccess ||
+      hipMalloc(&b->d_qoff, sizeof(int) * (kXcds + 1)) != hipSuccess ||
       hipMalloc(&b->d_counter, sizeof(unsigned int) * kXcds * kCounterStride) != hipSuccess)
     return cleanup(RECON_MVS_ERR_OUT_OF_MEMORY, "batch allocation failed");
-  if (hipMemcpy(b->d_args, args.data(), sizeof(PlaneArgs) * num_jobs, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(b->d_units, units.data(), sizeof(int2) * units.size(), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(b->d_qoff, qoff.data(), sizeof(int) * qoff.size(), hipMemcpyHostToDevice) != hipSuccess)
-    return cleanup(RECON_MVS_ERR_GPU_FAILURE, "batch upload failed");
+  std::vector<int2> units;
+  std::vector<int> qoff;
+  auto upload = [&](int sc) -> bool {
+    BatchUnits(&args, order, sc, &units, &qoff);
+    return hipMemcpy(b->d_args, args.data(), sizeof(PlaneArgs) * num_jobs, hipMemcpyHostToDevice) == hipSuccess &&
+           hipMemcpy(b->d_units, units.data(), sizeof(int2) * units.size(), hipMemcpyHostToDevice) == hipSuccess &&
+           hipMemcpy(b->d_qoff, qoff.data(), sizeof(int) * qoff.size(), hipMemcpyHostToDevice) == hipSuccess;
+  };
+  // occupancy caps per phase (index 0: init): none (occupancy_cap < 0), the
+  // given cap in the leading capped phases (> 0), or tuned (0): every
+  // candidate schedule's phases are timed on the batch itself uncapped and
+  // at 2 workgroups per CU, each phase keeps its faster cap, and the schedule
+  // with the least total wins.  Results do not depend on either choice.
+  int sched = scheds[0];
+  std::vector<int> caps(nph, 0);
+  if (params->occupancy_cap > 0) {
+    const int ci = params->capped_iterations < 0 ? 1 : params->capped_iterations;
+    for (int ph = 0; ph < std::min(nph, 1 + 2 * ci); ++ph) caps[ph] = params->occupancy_cap;
+  }
+  const bool tune = params->occupancy_cap == 0 || scheds.size() > 1;
+  if (tune) {
+    const bool timing = ctx->timing;
+    ctx->timing = false;
+    double best_total = 1e300;
+    const int trial_caps[2] = {0, 2};
+    for (int sc : scheds) {
+      if (!upload(sc)) {
+        ctx->timing = timing;
+        return cleanup(RECON_MVS_ERR_GPU_FAILURE, "batch upload failed");
+      }
+      BatchLaunch bl{b->d_args, b->d_units, b->d_qoff, b->d_counter, 0, ctx->cancel_dev};
+      std::vector<float> ms, best(nph, 3.0e38f);
+      std::vector<int> cap_of(nph, 0);
+      // a first run loads the code; a schedule not compiled here drops out
+      if (TimeBatchPhases(ctx, params, args[0], bl, sc, std::vector<int>(nph, 0), &ms) != RECON_MVS_OK) {
+        ctx->error.clear();
+        continue;
+      }
+      int32_t st = RECON_MVS_OK;
+      const int ncaps = params->occupancy_cap == 0 ? 2 : 1;
+      for (int t = 0; t < ncaps && st == RECON_MVS_OK; ++t) {
+        std::vector<int> trial = params->occupancy_cap == 0 ? std::vector<int>(nph, trial_caps[t]) : caps;
+        st = TimeBatchPhases(ctx, params, args[0], bl, sc, trial, &ms);
+        for (int ph = 0; ph < nph && st == RECON_MVS_OK; ++ph)
+          if (ms[ph] < best[ph]) {
+            best[ph] = ms[ph];
+            cap_of[ph] = trial[ph];
+          }
+      }
+      if (st != RECON_MVS_OK) {
+        ctx->timing = timing;
+        return cleanup(st, "batch tuning failed");
+      }
+      double total = 0.0;
+      for (float x : best) total += x;
+      if (total < best_total) {
+        best_total = total;
+        sched = sc;
+        caps = cap_of;
+      }
+    }
+    ctx->timing = timing;
+    if (best_total >= 1e300)
+      return cleanup(RECON_MVS_ERR_NOT_IMPLEMENTED, "no tile schedule compiled for this window / format");
+  }
+  if (!upload(sched)) return cleanup(RECON_MVS_ERR_GPU_FAILURE, "batch upload failed");
+  b->schedule = sched;
+  b->num_units = static_cast<int>(units.size());
   // capture init + all half-sweeps as one graph
   const bool timing = ctx->timing;
   ctx->timing = false;
