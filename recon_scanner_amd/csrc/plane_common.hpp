@@ -528,6 +528,7 @@ __device__ __forceinline__ float ViewNcc(const PlaneArgs& a, int v, const HypGeo
 // of the k smallest).
 __device__ __forceinline__ float CostLowerBound(const PlaneArgs& a, int v, const CostAcc& acc) {
   const int u = a.nsrc - 1 - v;  // views still to come
+  if (u >= a.top_k) return 0.0f;
   float lb = 0.0f;
   if (acc.nvalid + u <= a.top_k) {
     // every valid view is aggregated: the mean of the seen costs and u zeros
