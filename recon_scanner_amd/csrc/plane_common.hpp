@@ -525,7 +525,9 @@ __device__ __forceinline__ float ViewNcc(const PlaneArgs& a, int v, const HypGeo
 
 // Lower bound of the final cost after view v of a.nsrc, every view still to
 // come taken as valid with cost 0 (adding such a view never raises the mean
-// of the k smallest).
+// of the k smallest).  It is 0 while top_k or more views are still to come
+// (then either the k smallest can all be those zeros, or no view has been
+// seen), so those views skip it.
 __device__ __forceinline__ float CostLowerBound(const PlaneArgs& a, int v, const CostAcc& acc) {
   const int u = a.nsrc - 1 - v;  // views still to come
   if (u >= a.top_k) return 0.0f;
