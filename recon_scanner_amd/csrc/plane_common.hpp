@@ -68,6 +68,7 @@ struct PlaneArgs {
   int w, h, ref_pitch;
   const void* ref_img;
   double fx, fy, cx, cy;        // reference intrinsics
+  double kinv[4];               // 1/fx, 1/fy, -cx/fx, -cy/fy: the terms of K_r^-1
   float ffx, ffy, fcx, fcy;     // same, FP32
   int nsrc, top_k;
   const void* src_img[kMaxViews];
@@ -296,16 +297,26 @@ __device__ __forceinline__ HypGeom MakeHypGeom(const PlaneArgs& a, int x, int y,
   HypGeom g;
   g.qx = x + 0.5;
   g.qy = y + 0.5;
-  const double rx = (g.qx - a.cx) / a.fx, ry = (g.qy - a.cy) / a.fy;
+  const double rx = fma(g.qx, a.kinv[0], a.kinv[2]), ry = fma(g.qy, a.kinv[1], a.kinv[3]);
   const double nx = h.nx, ny = h.ny, nz = h.nz;
   g.D = h.d;
   const double nd = nx * rx + ny * ry + nz;
   g.front = nd < 0.0;
-  const double rho = g.D * nd;
-  g.m0 = nx / a.fx / rho;
-  g.m1 = ny / a.fy / rho;
-  g.m2 = (nz - nx * a.cx / a.fx - ny * a.cy / a.fy) / rho;
+  const double irho = 1.0 / (g.D * nd);
+  g.m0 = nx * a.kinv[0] * irho;
+  g.m1 = ny * a.kinv[1] * irho;
+  g.m2 = (nz + nx * a.kinv[2] + ny * a.kinv[3]) * irho;
   return g;
+}
+
+// 1/x within an ulp of the rounded quotient: the hardware reciprocal
+// refined by two Newton steps (x finite, normal and nonzero).
+__device__ __forceinline__ double RcpF64(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
 }
 
 // Costs of the views seen so far: count, sum and the sorted k smallest.
@@ -320,6 +331,30 @@ __device__ __forceinline__ void ResetCostAcc(CostAcc& acc) {
   for (int k = 0; k < kMaxTopKNet; ++k) acc.topk[k] = kCostInf;
   acc.csum = 0.0f;
   acc.nvalid = 0;
+}
+
+// Inserts cost c into the sorted k smallest.  Costs are finite and
+// non-negative (1 - NCC with NCC clamped to [-1, 1], or kCostInf), so their
+// bit patterns order as unsigned integers: integer min/max, which need no
+// NaN canonicalisation.  With top_k <= 4 only the first 4 slots are ever
+// read (the rest stay kCostInf), so the network stops there.
+__device__ __forceinline__ void InsertTopK(float* topk, float c, bool four) {
+  uint32_t cu = __float_as_uint(c);
+  if (four) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const uint32_t t = __float_as_uint(topk[k]);
+      topk[k] = __uint_as_float(min(t, cu));
+      cu = max(t, cu);
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < kMaxTopKNet; ++k) {
+      const uint32_t t = __float_as_uint(topk[k]);
+      topk[k] = __uint_as_float(min(t, cu));
+      cu = max(t, cu);
+    }
+  }
 }
 
 // View v's NCC of the hypothesis (NaN when the view is invalid); a valid
@@ -365,7 +400,7 @@ __device__ __forceinline__ float ViewNcc(const PlaneArgs& a, int v, const HypGeo
   const double hc1 = fma(H[3], qx, fma(H[4], qy, H[5]));
   const double hc2 = fma(H[6], qx, fma(H[7], qy, H[8]));
   if (hc2 * D > 1e-6) {
-    const double ih = 1.0 / hc2;
+    const double ih = RcpF64(hc2);
     const double uc = hc0 * ih, vc = hc1 * ih;       // continuous source coords
     const double su = uc - 0.5, sv = vc - 0.5;       // sampler coords (pixel centres at integers)
     if (fabs(su) < 1e7 && fabs(sv) < 1e7) {
@@ -438,6 +473,54 @@ __device__ __forceinline__ float ViewNcc(const PlaneArgs& a, int v, const HypGeo
               srs = fmaf(win.at(i, j), s, srs);
             }
           }
+        } else if (__ballot(!(win.full & den_ok)) == 0) {
+          // some lane is not fast, but every lane's reference window is
+          // whole and every denominator positive: only the source range is
+          // checked per sample; same operations as the general loop below
+          // for such lanes
+          bool have_k = false;
+          int ixcnt = 0;
+#pragma unroll 1
+          for (int j = 0; j < NS; ++j) {
+            const float dy = static_cast<float>(-HALF + j * STRIDE);
+            const float b0 = g01 * dy, b1 = g11 * dy, b2 = fmaf(h21, dy, 1.0f);
+            float txs[NS], tys[NS];
+#pragma unroll
+            for (int i = 0; i < NS; ++i) {
+              const float dx = static_cast<float>(-HALF + i * STRIDE);
+              const float r = __builtin_amdgcn_rcpf(fmaf(h20, dx, b2));
+              txs[i] = fmaf(fmaf(g00, dx, b0), r, fu);
+              tys[i] = fmaf(fmaf(g10, dx, b1), r, fv);
+            }
+            int off[NS];
+            float fxs[NS], fys[NS];
+#pragma unroll
+            for (int i = 0; i < NS; ++i) {
+              fxs[i] = __builtin_amdgcn_fractf(txs[i]);
+              fys[i] = __builtin_amdgcn_fractf(tys[i]);
+              off[i] = MadI24(FloorInt(tys[i]), pitch_eb, (FloorInt(txs[i]) << kLogEB) + obase);
+            }
+            Texel<FMT> q[NS];
+#pragma unroll
+            for (int i = 0; i < NS; ++i) q[i] = LoadTexel<FMT>(rsrc, off[i]);
+#pragma unroll
+            for (int i = 0; i < NS; ++i) {
+              const float b = Bilinear<FMT>(q[i], fxs[i], fys[i]);
+              const float rv = win.at(i, j);
+              const bool ok = fast | ((txs[i] >= lox) & (txs[i] <= hix) & (tys[i] >= loy) & (tys[i] <= hiy));
+              if (ok && !have_k) k_src = b;
+              have_k = have_k || ok;
+              const float s = ok ? b - k_src : 0.0f;
+              const float x = ok ? 0.0f : rv - win.mean;
+              ss += s;
+              sss = fmaf(s, s, sss);
+              srs = fmaf(rv, s, srs);
+              xr += x;
+              xrr = fmaf(x, x, xrr);
+              ixcnt += ok ? 0 : 1;
+            }
+          }
+          xcnt = static_cast<float>(ixcnt);
         } else {
           // some lane is not fast: each sample is checked; fast lanes take
           // the operations of the loop above
@@ -507,15 +590,7 @@ __device__ __forceinline__ float ViewNcc(const PlaneArgs& a, int v, const HypGeo
           float c = 1.0f - ncc;
           ++nvalid;
           csum += c;
-          if (use_topk) {
-            // insert into the sorted k smallest
-#pragma unroll
-            for (int k = 0; k < kMaxTopKNet; ++k) {
-              const float lo = fminf(topk[k], c);
-              c = fmaxf(topk[k], c);
-              topk[k] = lo;
-            }
-          }
+          if (use_topk) InsertTopK(topk, c, a.top_k <= 4);
         }
       }
     }
@@ -527,21 +602,23 @@ __device__ __forceinline__ float ViewNcc(const PlaneArgs& a, int v, const HypGeo
 // come taken as valid with cost 0 (adding such a view never raises the mean
 // of the k smallest).  It is 0 while top_k or more views are still to come
 // (then either the k smallest can all be those zeros, or no view has been
-// seen), so those views skip it.
+// seen), so those views skip it.  The divisions are reciprocal products: their
+// rounding (~2e-7 relative; none for a power-of-two top_k) lies far inside the
+// callers' margin (the bound is compared with the current cost x (1 + 1e-5)).
 __device__ __forceinline__ float CostLowerBound(const PlaneArgs& a, int v, const CostAcc& acc) {
   const int u = a.nsrc - 1 - v;  // views still to come
   if (u >= a.top_k) return 0.0f;
   float lb = 0.0f;
   if (acc.nvalid + u <= a.top_k) {
     // every valid view is aggregated: the mean of the seen costs and u zeros
-    if (acc.nvalid > 0) lb = acc.csum / static_cast<float>(acc.nvalid + u);
+    if (acc.nvalid > 0) lb = acc.csum * __builtin_amdgcn_rcpf(static_cast<float>(acc.nvalid + u));
   } else {
     // the k smallest: u zeros and the (k - u) smallest seen costs
     const int need = a.top_k - u;
     float sk = 0.0f;
 #pragma unroll
     for (int k = 0; k < kMaxTopKNet; ++k) sk += (k < need) ? acc.topk[k] : 0.0f;
-    if (need > 0) lb = sk / static_cast<float>(a.top_k);
+    if (need > 0) lb = sk * __builtin_amdgcn_rcpf(static_cast<float>(a.top_k));
   }
   return lb;
 }

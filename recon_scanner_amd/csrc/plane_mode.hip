@@ -766,6 +766,8 @@ int32_t MakeArgs(recon_mvs_hip_context* ctx, int ref, int nsrc, const int32_t* s
   a->ref_pitch = R.pitch;
   a->ref_img = R.pixels;
   a->fx = R.cam.fx; a->fy = R.cam.fy; a->cx = R.cam.cx; a->cy = R.cam.cy;
+  a->kinv[0] = 1.0 / R.cam.fx; a->kinv[1] = 1.0 / R.cam.fy;
+  a->kinv[2] = -R.cam.cx / R.cam.fx; a->kinv[3] = -R.cam.cy / R.cam.fy;
   a->ffx = static_cast<float>(R.cam.fx); a->ffy = static_cast<float>(R.cam.fy);
   a->fcx = static_cast<float>(R.cam.cx); a->fcy = static_cast<float>(R.cam.cy);
   a->nsrc = nsrc;
