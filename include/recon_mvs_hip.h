@@ -383,6 +383,15 @@ typedef struct recon_mvs_plane_params {
      default: the first iteration for FP32 images, none for FP16 / BF16),
      0 = none.  Results do not depend on it. */
   int32_t view_phased_iterations;
+  /* Leading phases (phase 0 = random init, then the half-sweeps in order)
+     whose hypotheses are evaluated in inverse-depth order: each tile's
+     (pixel, hypothesis) pairs are sorted into 64 bins of inverse depth in LDS
+     and evaluated bin by bin, so a wave's windows sit in one compact region
+     of each source instead of 64 scattered ones (11x11 stride-2 window only;
+     other windows ignore it).  -1 = automatic (tuned per phase wherever the
+     occupancy is tuned, otherwise none), 0 = none.  Results do not depend on
+     it. */
+  int32_t binned_phases;
 } recon_mvs_plane_params;
 
 /* Fills *p with the defaults: 11x11 window, stride 2, top-4, 5 iterations,

@@ -768,12 +768,16 @@ struct RefTile {
       v[ly][lx] = val;
     }
   }
-  // pixel of colour `color` owned by this thread
-  __device__ __forceinline__ void pixel(int x0, int y0, int color, int* x, int* y, int* r) const {
-    *r = threadIdx.x / (TW / 2);
-    const int j = threadIdx.x % (TW / 2);
+  // pixel of colour `color` owned by thread t
+  __device__ __forceinline__ void pixel_of(int t, int x0, int y0, int color, int* x, int* y, int* r) const {
+    *r = t / (TW / 2);
+    const int j = t % (TW / 2);
     *y = y0 + *r;
     *x = x0 + 2 * j + (((x0 + *y) & 1) ^ color);
+  }
+  // pixel of colour `color` owned by this thread
+  __device__ __forceinline__ void pixel(int x0, int y0, int color, int* x, int* y, int* r) const {
+    pixel_of(static_cast<int>(threadIdx.x), x0, y0, color, x, y, r);
   }
   template <int STRIDE>
   __device__ __forceinline__ void window(RefWindow<HALF, STRIDE>& win, int x, int x0, int r) const {

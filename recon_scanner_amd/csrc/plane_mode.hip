@@ -24,6 +24,8 @@
 // contraction (this file is compiled with -ffp-contract=off; the hot loops
 // use fmaf() where a fused multiply-add is wanted), so the FP64 oracle can
 // regenerate exactly the same candidates.
+#include <type_traits>
+
 #include "plane_common.hpp"
 
 // Random init + initial cost of every pixel (both colours) of one tile.
@@ -226,6 +228,273 @@ void k_plane_sweep(PlaneArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Binned evaluation: the same init and half-sweep as InitTile / SweepTile
+// with the tile's hypotheses evaluated in order of inverse depth.
+//
+// In the unconverged phases every pixel's hypotheses lie at random depths, so
+// the 64 lanes of a wave sample 64 windows scattered along as many epipolar
+// segments of each source: each gather touches ~64 cache lines and nothing is
+// reused across lanes.  Here the owning threads first write their pixel's
+// hypotheses (entries) to LDS, a counting sort puts the entries in 64 bins
+// of inverse depth over [inv_min, inv_max], and the workgroup evaluates the
+// sorted list: a wave's 64 consecutive entries lie within a few bins, so
+// their windows in every source sit in one compact region (the tile's extent
+// plus a short piece of the epipolar lines) and share cache lines.  The
+// owning threads then pick each pixel's best entry in candidate order.
+//
+// Results are bit-identical to InitTile / SweepTile: every entry is scored
+// by the same PlaneCost operations (a pixel's result does not depend on
+// which lanes share its wave, see ViewNcc); the early exit uses the pixel's
+// cost from the start of the sweep, a bound no tighter than SweepTile's
+// running best, so it drops only entries SweepTile drops or that lose to the
+// running best anyway; and the choice runs in candidate order with the same
+// strict comparison.
+// ---------------------------------------------------------------------------
+constexpr int kBins = 64;
+constexpr int kBinnedEntries = 2048;  // sweep: 256 pixels x 8 candidates; init: 512 pixels x 1
+
+template <int HALF, int FMT, int TW, int TH>
+struct BinnedLds {
+  RefTile<HALF, FMT, TW, TH> tile;
+  float4 cand[kBinnedEntries];     // hypothesis of entry e
+  float res[kBinnedEntries];       // its cost (kCostInf: rejected)
+  uint8_t nv[kBinnedEntries];      // its valid views
+  uint16_t list[kBinnedEntries];   // entries in bin order
+  float4 wstat[512];               // per pixel slot: the reference window's sum_c, sum_c2, n, mean
+  float start_cost[256];           // sweep: each pixel's cost at the start of the sweep
+  int bin[kBins];                  // counts, then first positions
+  int n;                           // entries in the list
+};
+
+// Bin of inverse depth 1/d in [inv_min, inv_max] (clamped; NaN -> 0).
+__device__ __forceinline__ int InvDepthBin(const PlaneArgs& a, float d) {
+  const float u = (1.0f / d - a.inv_min) * (static_cast<float>(kBins) / (a.inv_max - a.inv_min));
+  return static_cast<int>(fminf(fmaxf(u, 0.0f), static_cast<float>(kBins - 1)));
+}
+
+// Counting sort of the workgroup's entries by bin.  Each thread passes its
+// NE entries as (bin << 16 | entry), -1 for none; on return L.list
+// holds them bin by bin (order inside a bin unspecified) and L.n their count.
+template <int NE, int HALF, int FMT, int TW, int TH>
+__device__ __forceinline__ void BinEntries(BinnedLds<HALF, FMT, TW, TH>& L, const int (&be)[NE]) {
+  const int t = threadIdx.x;
+  if (t < kBins) L.bin[t] = 0;
+  __syncthreads();
+  int rank[NE];
+#pragma unroll
+  for (int k = 0; k < NE; ++k) rank[k] = be[k] >= 0 ? atomicAdd(&L.bin[be[k] >> 16], 1) : 0;
+  __syncthreads();
+  if (t == 0) {
+    int s = 0;
+    for (int b = 0; b < kBins; ++b) {
+      const int c = L.bin[b];
+      L.bin[b] = s;
+      s += c;
+    }
+    L.n = s;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NE; ++k)
+    if (be[k] >= 0) L.list[L.bin[be[k] >> 16] + rank[k]] = static_cast<uint16_t>(be[k] & 0xFFFF);
+  __syncthreads();
+}
+
+// Reference window of pixel slot s from its stored sums.
+template <int HALF, int STRIDE, int FMT, int TW, int TH>
+__device__ __forceinline__ void SlotWindow(const BinnedLds<HALF, FMT, TW, TH>& L, int s, int x, int x0, int r,
+                                           RefWindow<HALF, STRIDE>& win) {
+  using Tile = RefTile<HALF, FMT, TW, TH>;
+  const float4 ws = L.wstat[s];
+  win.base = &L.tile.v[r][x - x0];
+  win.rs = STRIDE * Tile::LW;
+  win.shift = L.tile.shift;
+  win.sum_c = ws.x;
+  win.sum_c2 = ws.y;
+  win.n = ws.z;
+  win.mean = ws.w;
+  win.full = (ws.z == static_cast<float>(RefWindow<HALF, STRIDE>::S));
+}
+
+// Scores the sorted entries.  Entry e belongs to pixel slot e >> shift_bits
+// (slot s: colour s >> 8 in init, thread s & 255); `sweep`: early exit
+// against the slot's start cost.
+template <int HALF, int STRIDE, int FMT, int TW, int TH>
+__device__ __forceinline__ void ScoreEntries(const PlaneArgs& a, BinnedLds<HALF, FMT, TW, TH>& L, int x0,
+                                             int y0, int color, bool sweep) {
+  const int n = L.n;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const int e = L.list[i];
+    const int s = sweep ? (e >> 3) : e;
+    const int col = sweep ? color : (s >> 8);
+    int x, y, r;
+    L.tile.pixel_of(s & 255, x0, y0, col, &x, &y, &r);
+    RefWindow<HALF, STRIDE> win;
+    SlotWindow<HALF, STRIDE, FMT, TW, TH>(L, s, x, x0, r, win);
+    float reject_at = kCostInf;
+    if (sweep) {
+      const float c0 = L.start_cost[s];
+      reject_at = c0 < kCostInf ? fmaf(c0, 1.0f + 1e-5f, 1e-7f) : kCostInf;
+    }
+    const float4 p = L.cand[e];
+    int nv = 0;
+    L.res[e] = PlaneCost<HALF, STRIDE, FMT>(a, x, y, win, Hyp{p.x, p.y, p.z, p.w}, &nv, nullptr, reject_at);
+    L.nv[e] = static_cast<uint8_t>(min(nv, 255));
+  }
+  __syncthreads();
+}
+
+// InitTile with binned evaluation.
+template <int HALF, int STRIDE, int FMT, int TW, int TH>
+__device__ __forceinline__ void InitTileBinned(const PlaneArgs& a, BinnedLds<HALF, FMT, TW, TH>& L, int tx,
+                                               int ty) {
+  const int x0 = tx * TW, y0 = ty * TH;
+  L.tile.load(a, x0, y0);
+  __syncthreads();
+  int be[2] = {-1, -1};
+  for (int color = 0; color < 2; ++color) {
+    int x, y, r;
+    L.tile.pixel(x0, y0, color, &x, &y, &r);
+    if (x >= a.w || y >= a.h) continue;
+    const int s = color * 256 + static_cast<int>(threadIdx.x);
+    RefWindow<HALF, STRIDE> win;
+    L.tile.template window<STRIDE>(win, x, x0, r);
+    L.wstat[s] = make_float4(win.sum_c, win.sum_c2, win.n, win.mean);
+    float ray[3];
+    PixelRay(a, x, y, ray);
+    const int idx = y * a.w + x;
+    const uint32_t key = RngKey(a.seed, 0u, static_cast<uint32_t>(idx));
+    Hyp h = RandomPlane(a, key, 0, ray);
+    if (a.seed_cells) {
+      const float sd = a.seed_cells[min(y >> 5, a.seed_gy - 1) * a.seed_gx + min(x >> 5, a.seed_gx - 1)];
+      if (sd > 0.0f) h = Hyp{sd, 0.0f, 0.0f, -1.0f};
+    }
+    L.cand[s] = make_float4(h.d, h.nx, h.ny, h.nz);
+    be[color] = (InvDepthBin(a, h.d) << 16) | s;
+  }
+  BinEntries(L, be);
+  ScoreEntries<HALF, STRIDE, FMT, TW, TH>(a, L, x0, y0, 0, false);
+  for (int color = 0; color < 2; ++color) {
+    if (be[color] < 0) continue;
+    int x, y, r;
+    L.tile.pixel(x0, y0, color, &x, &y, &r);
+    const int s = be[color] & 0xFFFF;
+    const int idx = y * a.w + x;
+    a.plane[idx] = L.cand[s];
+    a.cost[idx] = L.res[s];
+    a.support[idx] = L.nv[s];
+  }
+}
+
+// SweepTile with binned evaluation.
+template <int HALF, int STRIDE, int FMT, int TW, int TH>
+__device__ __forceinline__ void SweepTileBinned(const PlaneArgs& a, BinnedLds<HALF, FMT, TW, TH>& L, int tx,
+                                                int ty, int color, int iter) {
+  const int x0 = tx * TW, y0 = ty * TH;
+  L.tile.load(a, x0, y0);
+  __syncthreads();
+  const int t = threadIdx.x;
+  int x, y, r;
+  L.tile.pixel(x0, y0, color, &x, &y, &r);
+  const bool in = x < a.w && y < a.h;
+  int be[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) be[c] = -1;
+  int idx = 0;
+  float4 cur = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  float ccur = kCostInf;
+  int scur = 0;
+  if (in) {
+    RefWindow<HALF, STRIDE> win;
+    L.tile.template window<STRIDE>(win, x, x0, r);
+    L.wstat[t] = make_float4(win.sum_c, win.sum_c2, win.n, win.mean);
+    idx = y * a.w + x;
+    float ray[3];
+    PixelRay(a, x, y, ray);
+    cur = a.plane[idx];
+    ccur = a.cost[idx];
+    scur = a.support[idx];
+    L.start_cost[t] = ccur;
+    const uint32_t key = RngKey(a.seed, static_cast<uint32_t>(1 + 2 * iter + color),
+                                static_cast<uint32_t>(idx));
+    const float dscale = ldexpf(a.depth_perturb, -iter);
+    const float nscale = ldexpf(a.normal_perturb, -iter);
+    const float4 base = cur;
+    int src_x[4], src_y[4];
+    PropagationSources(a, x, y, src_x, src_y);
+    float4 pc[4];
+    bool pok[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      Hyp h{0.0f, 0.0f, 0.0f, 0.0f};
+      pok[c] = SweepCandidate(a, c, ray, key, base, dscale, nscale, src_x, src_y, &h);
+      pc[c] = make_float4(h.d, h.nx, h.ny, h.nz);
+    }
+    int dup_mask = 0;
+#pragma unroll
+    for (int c = 1; c < 4; ++c)
+#pragma unroll
+      for (int k = 0; k < c; ++k)
+        if (pok[c] && pok[k] && pc[c].x == pc[k].x && pc[c].y == pc[k].y && pc[c].z == pc[k].z &&
+            pc[c].w == pc[k].w)
+          dup_mask |= 1 << c;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      if ((dup_mask >> c) & 1) continue;
+      Hyp h;
+      if (c < 4) {
+        if (!pok[c]) continue;
+        h = Hyp{pc[c].x, pc[c].y, pc[c].z, pc[c].w};
+      } else if (!SweepCandidate(a, c, ray, key, base, dscale, nscale, src_x, src_y, &h)) {
+        continue;
+      }
+      // PlaneCost scores these kCostInf without any view: never taken
+      if (!(h.d >= a.min_depth && h.d <= a.max_depth) || !MakeHypGeom(a, x, y, h).front) continue;
+      const int e = t * 8 + c;
+      L.cand[e] = make_float4(h.d, h.nx, h.ny, h.nz);
+      be[c] = (InvDepthBin(a, h.d) << 16) | e;
+    }
+  }
+  BinEntries(L, be);
+  ScoreEntries<HALF, STRIDE, FMT, TW, TH>(a, L, x0, y0, color, true);
+  if (!in) return;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    if (be[c] < 0) continue;
+    const int e = t * 8 + c;
+    const float cc = L.res[e];
+    if (cc < ccur) {
+      ccur = cc;
+      cur = L.cand[e];
+      scur = L.nv[e];
+    }
+  }
+  a.plane[idx] = cur;
+  a.cost[idx] = ccur;
+  a.support[idx] = static_cast<uint8_t>(min(scur, 255));
+}
+
+// Single-view launches of the binned forms (LDS: ~60 KB, 2 workgroups per
+// CU; register budget of 2 waves per SIMD).
+template <int HALF, int STRIDE, int FMT, int TW, int TH>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+void k_plane_init_binned(PlaneArgs a) {
+  __shared__ BinnedLds<HALF, FMT, TW, TH> L;
+  int tx, ty;
+  if (!TileOf(a, &tx, &ty)) return;
+  InitTileBinned<HALF, STRIDE, FMT, TW, TH>(a, L, tx, ty);
+}
+
+template <int HALF, int STRIDE, int FMT, int TW, int TH>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+void k_plane_sweep_binned(PlaneArgs a) {
+  __shared__ BinnedLds<HALF, FMT, TW, TH> L;
+  int tx, ty;
+  if (!TileOf(a, &tx, &ty)) return;
+  SweepTileBinned<HALF, STRIDE, FMT, TW, TH>(a, L, tx, ty, a.color, a.iter);
+}
+
+// ---------------------------------------------------------------------------
 // View-phased half-sweep: the same half-sweep as k_plane_sweep in 2 + nsrc
 // launches.  k_phase_prep makes each pixel's 8 candidates, one k_phase_view
 // launch per source view adds that view to every live candidate's
@@ -391,12 +660,15 @@ __global__ __launch_bounds__(256) void k_phase_final(PlaneArgs a, PhaseState st,
 constexpr int kXcds = 8;
 constexpr int kCounterStride = 32;  // queue counters 128 bytes apart
 
-template <int HALF, int STRIDE, int FMT, int TW, int TH>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STRIDE == 2 ? 3 : 1)))
+template <bool BINNED, int HALF, int FMT, int TW, int TH>
+using BatchLds = std::conditional_t<BINNED, BinnedLds<HALF, FMT, TW, TH>, RefTile<HALF, FMT, TW, TH>>;
+
+template <int HALF, int STRIDE, int FMT, int TW, int TH, bool BINNED>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BINNED ? 2 : (STRIDE == 2 ? 3 : 1))))
 void k_plane_batch(const PlaneArgs* __restrict__ args, const int2* __restrict__ units,
                    const int* __restrict__ qoff, unsigned int* __restrict__ counters, int phase,
                    const int32_t* cancel) {
-  __shared__ RefTile<HALF, FMT, TW, TH> tile;
+  __shared__ BatchLds<BINNED, HALF, FMT, TW, TH> lds;
   __shared__ int s_unit;
   int stop = 0;    // thread 0: the flag as read at the previous take
   int tried = 0;   // thread 0: queues found empty (own queue first)
@@ -423,9 +695,14 @@ void k_plane_batch(const PlaneArgs* __restrict__ args, const int2* __restrict__ 
     const int t = __builtin_amdgcn_readfirstlane(un.y);
     const PlaneArgs& a = args[job];
     const int tx = t % a.tiles_x, ty = t / a.tiles_x;
-    if (phase < 0) InitTile<HALF, STRIDE, FMT, TW, TH>(a, tile, tx, ty);
-    else SweepTile<HALF, STRIDE, FMT, TW, TH>(a, tile, tx, ty, phase & 1, phase >> 1);
-    __syncthreads();  // the LDS tile and s_unit are reused by the next unit
+    if constexpr (BINNED) {
+      if (phase < 0) InitTileBinned<HALF, STRIDE, FMT, TW, TH>(a, lds, tx, ty);
+      else SweepTileBinned<HALF, STRIDE, FMT, TW, TH>(a, lds, tx, ty, phase & 1, phase >> 1);
+    } else {
+      if (phase < 0) InitTile<HALF, STRIDE, FMT, TW, TH>(a, lds, tx, ty);
+      else SweepTile<HALF, STRIDE, FMT, TW, TH>(a, lds, tx, ty, phase & 1, phase >> 1);
+    }
+    __syncthreads();  // the LDS and s_unit are reused by the next unit
   }
 }
 
@@ -743,6 +1020,8 @@ int32_t CheckParams(recon_mvs_hip_context* ctx, const recon_mvs_plane_params* p)
     return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "occupancy_cap out of [-1, 64]");
   if (p->view_phased_iterations < -1)
     return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "view_phased_iterations below -1");
+  if (p->binned_phases < -1)
+    return Fail(ctx, RECON_MVS_ERR_INVALID_ARGUMENT, "binned_phases below -1");
   return RECON_MVS_OK;
 }
 
@@ -917,17 +1196,24 @@ inline size_t CapBytes(int cap, size_t stat) {
   return std::min(want, room);
 }
 
+// Phase configuration of a launch: occupancy cap (low byte; 0 = none) plus
+// kBinnedCfg for the binned form of init / half-sweep (11x11 stride-2 window;
+// other windows run the fused form, which computes the same results).
+constexpr int kBinnedCfg = 1 << 8;
+inline int CfgCap(int cfg) { return cfg & 0xFF; }
+inline bool CfgBinned(int cfg) { return (cfg & kBinnedCfg) != 0; }
+
 // Workgroups of a persistent launch: as many as fit on the device at once
 // (with `cap` workgroups per CU at most).
-template <int HALF, int STRIDE, int FMT, int TW, int TH>
+template <int HALF, int STRIDE, int FMT, int TW, int TH, bool BINNED>
 int BatchGrid(int cap) {
   static int nb = -1, cus = 1;
   if (nb < 0) {
     int dev = 0;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_plane_batch<HALF, STRIDE, FMT, TW, TH>, 256, 0) !=
-        hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_plane_batch<HALF, STRIDE, FMT, TW, TH, BINNED>, 256,
+                                                     0) != hipSuccess)
       nb = 1;
   }
   const int per_cu = std::max(1, cap > 0 ? std::min(cap, nb) : nb);
@@ -944,23 +1230,41 @@ inline size_t StaticLds(const void* kernel) {
 
 template <int HALF, int STRIDE, int FMT, int TW, int TH>
 void LaunchTile(KernelId id, PlaneArgs a, hipStream_t s, float f0, int i0, int n, const float* hyps,
-                float* cost, int* nvalid, float* view_ncc, const BatchLaunch* bl, int cap,
+                float* cost, int* nvalid, float* view_ncc, const BatchLaunch* bl, int cfg,
                 const PhaseLaunch* pl) {
+  constexpr bool kHasBinned = (HALF == 5 && STRIDE == 2);
+  const bool binned = kHasBinned && CfgBinned(cfg) && (id == kSweep || id == kInit || id == kBatch);
+  const int cap = CfgCap(cfg);
   // padding for the occupancy cap, from each kernel's own static LDS
   size_t pad = 0;
   if (cap > 0) {
     static const size_t lds_sweep = StaticLds(reinterpret_cast<const void*>(k_plane_sweep<HALF, STRIDE, FMT, TW, TH>));
     static const size_t lds_init = StaticLds(reinterpret_cast<const void*>(k_plane_init<HALF, STRIDE, FMT, TW, TH>));
-    static const size_t lds_batch = StaticLds(reinterpret_cast<const void*>(k_plane_batch<HALF, STRIDE, FMT, TW, TH>));
-    if (id == kSweep) pad = CapBytes(cap, lds_sweep);
+    static const size_t lds_batch = StaticLds(reinterpret_cast<const void*>(k_plane_batch<HALF, STRIDE, FMT, TW, TH, false>));
+    if (binned) {
+      if constexpr (kHasBinned) {
+        static const size_t lds_bsweep = StaticLds(reinterpret_cast<const void*>(k_plane_sweep_binned<HALF, STRIDE, FMT, TW, TH>));
+        static const size_t lds_binit = StaticLds(reinterpret_cast<const void*>(k_plane_init_binned<HALF, STRIDE, FMT, TW, TH>));
+        static const size_t lds_bbatch = StaticLds(reinterpret_cast<const void*>(k_plane_batch<HALF, STRIDE, FMT, TW, TH, true>));
+        pad = CapBytes(cap, id == kSweep ? lds_bsweep : id == kInit ? lds_binit : lds_bbatch);
+      }
+    } else if (id == kSweep) pad = CapBytes(cap, lds_sweep);
     else if (id == kInit) pad = CapBytes(cap, lds_init);
     else if (id == kBatch) pad = CapBytes(cap, lds_batch);
   }
   if (id == kBatch) {
     hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, s, bl->counters);
-    hipLaunchKernelGGL((k_plane_batch<HALF, STRIDE, FMT, TW, TH>), dim3(BatchGrid<HALF, STRIDE, FMT, TW, TH>(cap)),
-                       dim3(256), pad, s, bl->args, bl->units, bl->qoff, bl->counters, bl->phase,
-                       bl->cancel);
+    if constexpr (kHasBinned) {
+      if (binned) {
+        hipLaunchKernelGGL((k_plane_batch<HALF, STRIDE, FMT, TW, TH, true>),
+                           dim3(BatchGrid<HALF, STRIDE, FMT, TW, TH, true>(cap)), dim3(256), pad, s, bl->args,
+                           bl->units, bl->qoff, bl->counters, bl->phase, bl->cancel);
+        return;
+      }
+    }
+    hipLaunchKernelGGL((k_plane_batch<HALF, STRIDE, FMT, TW, TH, false>),
+                       dim3(BatchGrid<HALF, STRIDE, FMT, TW, TH, false>(cap)), dim3(256), pad, s, bl->args,
+                       bl->units, bl->qoff, bl->counters, bl->phase, bl->cancel);
     return;
   }
   a.tiles_x = (a.w + TW - 1) / TW;
@@ -968,6 +1272,13 @@ void LaunchTile(KernelId id, PlaneArgs a, hipStream_t s, float f0, int i0, int n
   a.num_tiles = a.tiles_x * a.tiles_y;
   a.tiles_per_xcd = (a.num_tiles + 7) / 8;
   const dim3 grid(a.tiles_per_xcd * 8), block(256);
+  if constexpr (kHasBinned) {
+    if (binned) {
+      if (id == kSweep) hipLaunchKernelGGL((k_plane_sweep_binned<HALF, STRIDE, FMT, TW, TH>), grid, block, pad, s, a);
+      else hipLaunchKernelGGL((k_plane_init_binned<HALF, STRIDE, FMT, TW, TH>), grid, block, pad, s, a);
+      return;
+    }
+  }
   switch (id) {
     case kSweep: hipLaunchKernelGGL((k_plane_sweep<HALF, STRIDE, FMT, TW, TH>), grid, block, pad, s, a); break;
     case kInit: hipLaunchKernelGGL((k_plane_init<HALF, STRIDE, FMT, TW, TH>), grid, block, pad, s, a); break;
@@ -1000,17 +1311,17 @@ void LaunchTile(KernelId id, PlaneArgs a, hipStream_t s, float f0, int i0, int n
 template <int HALF, int STRIDE, int FMT>
 bool LaunchT(int schedule, KernelId id, const PlaneArgs& a, hipStream_t s, float f0, int i0, int n,
              const float* hyps, float* cost, int* nvalid, float* view_ncc, const BatchLaunch* bl,
-             int cap, const PhaseLaunch* pl) {
+             int cfg, const PhaseLaunch* pl) {
   // the 11x11 stride-2 window is compiled for every tile schedule, the others
   // for the default one
   constexpr bool kAllSchedules = (HALF == 5 && STRIDE == 2);
   switch (schedule) {
-    case 0: LaunchTile<HALF, STRIDE, FMT, 64, 8>(id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cap, pl); return true;
+    case 0: LaunchTile<HALF, STRIDE, FMT, 64, 8>(id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cfg, pl); return true;
     case 1:
-      if constexpr (kAllSchedules) { LaunchTile<HALF, STRIDE, FMT, 32, 16>(id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cap, pl); return true; }
+      if constexpr (kAllSchedules) { LaunchTile<HALF, STRIDE, FMT, 32, 16>(id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cfg, pl); return true; }
       return false;
     case 2:
-      if constexpr (kAllSchedules) { LaunchTile<HALF, STRIDE, FMT, 128, 4>(id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cap, pl); return true; }
+      if constexpr (kAllSchedules) { LaunchTile<HALF, STRIDE, FMT, 128, 4>(id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cfg, pl); return true; }
       return false;
     default: return false;
   }
@@ -1019,44 +1330,46 @@ bool LaunchT(int schedule, KernelId id, const PlaneArgs& a, hipStream_t s, float
 template <int HALF, int STRIDE>
 bool LaunchFmt(int fmt, int schedule, KernelId id, const PlaneArgs& a, hipStream_t s, float f0, int i0,
                int n, const float* hyps, float* cost, int* nvalid, float* view_ncc, const BatchLaunch* bl,
-               int cap, const PhaseLaunch* pl) {
+               int cfg, const PhaseLaunch* pl) {
   // FP16 / BF16 storage is compiled for the 11x11 stride-2 and 7x7 windows,
   // 8-bit storage for the 11x11 stride-2 window
   constexpr bool kAllFormats = (HALF == 5 && STRIDE == 2) || (HALF == 3 && STRIDE == 1);
   switch (fmt) {
-    case RECON_MVS_PIXEL_F32: return LaunchT<HALF, STRIDE, RECON_MVS_PIXEL_F32>(schedule, id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cap, pl);
+    case RECON_MVS_PIXEL_F32: return LaunchT<HALF, STRIDE, RECON_MVS_PIXEL_F32>(schedule, id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cfg, pl);
     case RECON_MVS_PIXEL_F16:
-      if constexpr (kAllFormats) return LaunchT<HALF, STRIDE, RECON_MVS_PIXEL_F16>(schedule, id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cap, pl);
+      if constexpr (kAllFormats) return LaunchT<HALF, STRIDE, RECON_MVS_PIXEL_F16>(schedule, id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cfg, pl);
       return false;
     case RECON_MVS_PIXEL_BF16:
-      if constexpr (kAllFormats) return LaunchT<HALF, STRIDE, RECON_MVS_PIXEL_BF16>(schedule, id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cap, pl);
+      if constexpr (kAllFormats) return LaunchT<HALF, STRIDE, RECON_MVS_PIXEL_BF16>(schedule, id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cfg, pl);
       return false;
     case RECON_MVS_PIXEL_U8:  // 8-bit storage: the 11x11 stride-2 window
-      if constexpr (HALF == 5 && STRIDE == 2) return LaunchT<HALF, STRIDE, RECON_MVS_PIXEL_U8>(schedule, id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cap, pl);
+      if constexpr (HALF == 5 && STRIDE == 2) return LaunchT<HALF, STRIDE, RECON_MVS_PIXEL_U8>(schedule, id, a, s, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cfg, pl);
       return false;
     default: return false;
   }
 }
 
 // Window shapes compiled in: (patch, stride) in {(11,2), (11,1), (7,1), (5,1), (7,2), (9,2)}.
+// cfg: phase configuration (occupancy cap, binned form) of init / sweep /
+// batch launches.
 int32_t Launch(recon_mvs_hip_context* ctx, const recon_mvs_plane_params* p, KernelId id,
                const PlaneArgs& a, float f0 = 0.0f, int i0 = 0, int n = 0,
                const float* hyps = nullptr, float* cost = nullptr, int* nvalid = nullptr,
                float* view_ncc = nullptr, const BatchLaunch* bl = nullptr, int schedule = -1,
-               int cap = 0, const PhaseLaunch* pl = nullptr) {
+               int cfg = 0, const PhaseLaunch* pl = nullptr) {
   const int fmt = ctx->storage_format;
   const int sch = schedule >= 0 ? schedule : p->tile_schedule;
   hipEvent_t eb, ee;
   TimerBegin(ctx, static_cast<int>(id), &eb, &ee);
   bool ok = false;
   const int ps = p->patch_size, st = p->window_stride;
-  if (ps == 11 && st == 2) ok = LaunchFmt<5, 2>(fmt, sch, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cap, pl);
+  if (ps == 11 && st == 2) ok = LaunchFmt<5, 2>(fmt, sch, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cfg, pl);
 #ifndef RMVS_BENCH_WINDOW_ONLY  // (development builds compile the 11x11 stride-2 window only)
-  else if (ps == 11 && st == 1) ok = LaunchFmt<5, 1>(fmt, sch, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cap, pl);
-  else if (ps == 7 && st == 1) ok = LaunchFmt<3, 1>(fmt, sch, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cap, pl);
-  else if (ps == 5 && st == 1) ok = LaunchFmt<2, 1>(fmt, sch, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cap, pl);
-  else if (ps == 7 && st == 2) ok = LaunchFmt<3, 2>(fmt, sch, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cap, pl);
-  else if (ps == 9 && st == 2) ok = LaunchFmt<4, 2>(fmt, sch, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cap, pl);
+  else if (ps == 11 && st == 1) ok = LaunchFmt<5, 1>(fmt, sch, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cfg, pl);
+  else if (ps == 7 && st == 1) ok = LaunchFmt<3, 1>(fmt, sch, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cfg, pl);
+  else if (ps == 5 && st == 1) ok = LaunchFmt<2, 1>(fmt, sch, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cfg, pl);
+  else if (ps == 7 && st == 2) ok = LaunchFmt<3, 2>(fmt, sch, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cfg, pl);
+  else if (ps == 9 && st == 2) ok = LaunchFmt<4, 2>(fmt, sch, id, a, ctx->stream, f0, i0, n, hyps, cost, nvalid, view_ncc, bl, cfg, pl);
 #endif
   if (!ok)
     return Fail(ctx, RECON_MVS_ERR_NOT_IMPLEMENTED,
@@ -1267,7 +1580,8 @@ int32_t ResolveSchedule(recon_mvs_hip_context* ctx, const PlaneArgs& a,
 }
 
 // Times one full PatchMatch run of view `a` (init + every half-sweep) with
-// occupancy cap `cap` in every phase; ms[phase] receives each launch's time.
+// phase configuration `cap` (occupancy cap, binned form) in every phase;
+// ms[phase] receives each launch's time.
 int32_t TimePhases(recon_mvs_hip_context* ctx, PlaneArgs a, const recon_mvs_plane_params* p,
                    int sched, int cap, std::vector<float>* ms) {
   const int nph = 1 + 2 * p->iterations;
@@ -1299,25 +1613,40 @@ int32_t TimePhases(recon_mvs_hip_context* ctx, PlaneArgs a, const recon_mvs_plan
   return st;
 }
 
-// Occupancy cap of every phase of a PatchMatch run (index 0: init; 1 + 2 it +
-// c: half-sweep c of iteration it), as recon_mvs_plane_params.occupancy_cap
-// describes.  Tuning runs the view under caps 0 (none), 1 and 2 and keeps
-// each phase's fastest; the phases compute the same results under any cap.
+// Whether the binned form exists for p's window.
+inline bool HasBinned(const recon_mvs_plane_params* p) { return p->patch_size == 11 && p->window_stride == 2; }
+
+// Configuration (occupancy cap, binned form) of every phase of a PatchMatch
+// run (index 0: init; 1 + 2 it + c: half-sweep c of iteration it), as
+// recon_mvs_plane_params.occupancy_cap and binned_phases describe.  Tuning
+// runs the view under caps 0 (none), 1 and 2 and, with binned_phases = -1,
+// in the binned form, and keeps each phase's fastest; the phases compute the
+// same results under any configuration.
 int32_t ResolveCaps(recon_mvs_hip_context* ctx, const PlaneArgs& a, const recon_mvs_plane_params* p,
                     int sched, std::vector<int>* caps) {
   const int nph = 1 + 2 * p->iterations;
   caps->assign(nph, 0);
-  if (p->occupancy_cap < 0) return RECON_MVS_OK;
+  const bool binnable = HasBinned(p);
+  // forced binned phases keep the binned form whatever the caps say
+  const int forced = binnable && p->binned_phases > 0 ? std::min(nph, p->binned_phases) : 0;
+  auto force = [&]() {
+    for (int ph = 0; ph < forced; ++ph) (*caps)[ph] = kBinnedCfg;
+  };
+  if (p->occupancy_cap < 0) {
+    force();
+    return RECON_MVS_OK;
+  }
   const int ci = p->capped_iterations < 0 ? 1 : p->capped_iterations;
   const int ncapped = std::min(nph, 1 + 2 * ci);
   if (p->occupancy_cap > 0 || p->tile_schedule >= 0) {
     const int cap = p->occupancy_cap > 0 ? p->occupancy_cap
                                          : (ctx->storage_format == RECON_MVS_PIXEL_F32 ? 1 : 0);
     for (int ph = 0; ph < ncapped; ++ph) (*caps)[ph] = cap;
+    force();
     return RECON_MVS_OK;
   }
   const std::string key = TuneKey(ctx, a, p) + "/" + std::to_string(sched) + "/" +
-                          std::to_string(p->iterations);
+                          std::to_string(p->iterations) + "/" + std::to_string(p->binned_phases);
   auto it = ctx->tuned_caps.find(key);
   if (it != ctx->tuned_caps.end()) {
     *caps = it->second;
@@ -1325,19 +1654,21 @@ int32_t ResolveCaps(recon_mvs_hip_context* ctx, const PlaneArgs& a, const recon_
   }
   const bool timing = ctx->timing;
   ctx->timing = false;
-  const int trial_caps[3] = {0, 1, 2};
+  std::vector<int> trials = {0, 1, 2};
+  if (binnable && p->binned_phases < 0) trials.push_back(kBinnedCfg);
   std::vector<float> best(nph, 3.0e38f), ms;
   int32_t st = TimePhases(ctx, a, p, sched, 0, &ms);  // warm-up (code load)
-  for (int t = 0; t < 3 && st == RECON_MVS_OK; ++t) {
-    st = TimePhases(ctx, a, p, sched, trial_caps[t], &ms);
+  for (size_t t = 0; t < trials.size() && st == RECON_MVS_OK; ++t) {
+    st = TimePhases(ctx, a, p, sched, trials[t], &ms);
     for (int ph = 0; ph < nph && st == RECON_MVS_OK; ++ph)
       if (ms[ph] < best[ph]) {
         best[ph] = ms[ph];
-        (*caps)[ph] = trial_caps[t];
+        (*caps)[ph] = trials[t];
       }
   }
   ctx->timing = timing;
   if (st != RECON_MVS_OK) return st;
+  force();
   ctx->tuned_caps[key] = *caps;
   return RECON_MVS_OK;
 }
@@ -1576,6 +1907,7 @@ RECON_MVS_API void recon_mvs_plane_params_default(recon_mvs_plane_params* p) {
   p->occupancy_cap = 0;
   p->capped_iterations = 1;
   p->view_phased_iterations = -1;
+  p->binned_phases = -1;
 }
 
 RECON_MVS_API int32_t recon_mvs_hip_set_views(recon_mvs_hip_context* ctx, int32_t n,
@@ -1949,23 +2281,28 @@ RECON_MVS_API int32_t recon_mvs_hip_batch_create(recon_mvs_hip_context* ctx, int
            hipMemcpy(b->d_units, units.data(), sizeof(int2) * units.size(), hipMemcpyHostToDevice) == hipSuccess &&
            hipMemcpy(b->d_qoff, qoff.data(), sizeof(int) * qoff.size(), hipMemcpyHostToDevice) == hipSuccess;
   };
-  // occupancy caps per phase (index 0: init): none (occupancy_cap < 0), the
-  // given cap in the leading capped phases (> 0), or tuned (0): every
-  // candidate schedule's phases are timed on the batch itself uncapped and
-  // at 2 workgroups per CU, each phase keeps its faster cap, and the schedule
-  // with the least total wins.  Results do not depend on either choice.
+  // configuration per phase (index 0: init): occupancy cap none
+  // (occupancy_cap < 0), the given cap in the leading capped phases (> 0),
+  // or tuned (0): every candidate schedule's phases are timed on the batch
+  // itself uncapped, at 2 workgroups per CU and (binned_phases = -1) in the
+  // binned form, each phase keeps its fastest, and the schedule with the
+  // least total wins; binned_phases > 0 forces the binned form on the
+  // leading phases.  Results do not depend on any of these choices.
   int sched = scheds[0];
   std::vector<int> caps(nph, 0);
   if (params->occupancy_cap > 0) {
     const int ci = params->capped_iterations < 0 ? 1 : params->capped_iterations;
     for (int ph = 0; ph < std::min(nph, 1 + 2 * ci); ++ph) caps[ph] = params->occupancy_cap;
   }
+  const int forced = HasBinned(params) && params->binned_phases > 0 ? std::min(nph, params->binned_phases) : 0;
+  for (int ph = 0; ph < forced; ++ph) caps[ph] = kBinnedCfg;
   const bool tune = params->occupancy_cap == 0 || scheds.size() > 1;
   if (tune) {
     const bool timing = ctx->timing;
     ctx->timing = false;
     double best_total = 1e300;
-    const int trial_caps[2] = {0, 2};
+    std::vector<int> trial_cfgs = {0, 2};
+    if (HasBinned(params) && params->binned_phases < 0) trial_cfgs.push_back(kBinnedCfg);
     for (int sc : scheds) {
       if (!upload(sc)) {
         ctx->timing = timing;
@@ -1980,9 +2317,10 @@ RECON_MVS_API int32_t recon_mvs_hip_batch_create(recon_mvs_hip_context* ctx, int
         continue;
       }
       int32_t st = RECON_MVS_OK;
-      const int ncaps = params->occupancy_cap == 0 ? 2 : 1;
-      for (int t = 0; t < ncaps && st == RECON_MVS_OK; ++t) {
-        std::vector<int> trial = params->occupancy_cap == 0 ? std::vector<int>(nph, trial_caps[t]) : caps;
+      const size_t ncfg = params->occupancy_cap == 0 ? trial_cfgs.size() : 1;
+      for (size_t t = 0; t < ncfg && st == RECON_MVS_OK; ++t) {
+        std::vector<int> trial = params->occupancy_cap == 0 ? std::vector<int>(nph, trial_cfgs[t]) : caps;
+        for (int ph = 0; ph < forced; ++ph) trial[ph] = kBinnedCfg;
         st = TimeBatchPhases(ctx, params, args[0], bl, sc, trial, &ms);
         for (int ph = 0; ph < nph && st == RECON_MVS_OK; ++ph)
           if (ms[ph] < best[ph]) {

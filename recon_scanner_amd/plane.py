@@ -31,7 +31,7 @@ class PlaneParams(C.Structure):
                 ("depth_perturb", C.c_double), ("normal_perturb", C.c_double),
                 ("photo_threshold", C.c_double), ("value_scale", C.c_double),
                 ("occupancy_cap", C.c_int32), ("capped_iterations", C.c_int32),
-                ("view_phased_iterations", C.c_int32)]
+                ("view_phased_iterations", C.c_int32), ("binned_phases", C.c_int32)]
 
     def copy(self, **kw):
         p = PlaneParams()

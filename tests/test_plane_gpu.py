@@ -168,7 +168,8 @@ def test_refine_keeps_accuracy():
 def test_batch_matches_per_view_runs_bit_exact(storage):
     """Batched persistent launch (uneven resolutions, 4-10 sources) ==
     per-view plane_patchmatch with the same parameters, over 3 graph replays
-    with eager work between them.  FP16 storage of the float images, and
+    with eager work between them, with the phases' binned form chosen by
+    the tuner, forced on every phase and switched off.  FP16 storage of the float images, and
     8-bit storage of 8-bit images (what the stage uploads): the latter also
     equals per-view runs on the same images in FP32 storage bit for bit."""
     say("batch: orbit of 12 cameras, 6 refs, uneven sizes,", storage)
@@ -181,7 +182,7 @@ def test_batch_matches_per_view_runs_bit_exact(storage):
     pm = plane.PlaneMVS(sc.cams, imgs, storage=storage)
     singles = {}
     for r, srcs in jobs:
-        pm.patchmatch(r, srcs, p)
+        pm.patchmatch(r, srcs, p.copy(binned_phases=0))
         singles[r] = pm.depth_map(r)
     if storage == "u8":
         pf = plane.PlaneMVS(sc.cams, imgs, storage="f32")
@@ -190,15 +191,17 @@ def test_batch_matches_per_view_runs_bit_exact(storage):
             f = pf.depth_map(r)
             for k in ("depth", "normals", "cost", "support"):
                 assert f[k].tobytes() == singles[r][k].tobytes(), (r, k)
-    b = pm.batch(jobs, p)
-    for rep in range(3):   # graph replays, eager work between them
-        b.run()
-        pm.synchronize()
-        for r, _ in jobs:
-            g = pm.depth_map(r)
-            for k in ("depth", "normals", "cost", "support"):
-                assert g[k].tobytes() == singles[r][k].tobytes(), (rep, r, k)
-    b.close()
+    # tuned plan (binned_phases = -1), every phase binned, none binned
+    for binned in (-1, 1 + 2 * p.iterations, 0):
+        b = pm.batch(jobs, p.copy(binned_phases=binned))
+        for rep in range(3):   # graph replays, eager work between them
+            b.run()
+            pm.synchronize()
+            for r, _ in jobs:
+                g = pm.depth_map(r)
+                for k in ("depth", "normals", "cost", "support"):
+                    assert g[k].tobytes() == singles[r][k].tobytes(), (binned, rep, r, k)
+        b.close()
 
 
 def test_autotune_picks_a_compiled_schedule():
@@ -368,6 +371,47 @@ def test_view_phased_sweeps_bit_identical(storage, top_k):
     seeded = []
     for phased in (0, 1):
         pm.patchmatch(0, srcs, p.copy(view_phased_iterations=phased, iterations=1), seed_points=seeds)
+        seeded.append(pm.depth_map(0))
+    for k in ("depth", "normals", "cost", "support"):
+        assert np.array_equal(seeded[1][k], seeded[0][k], equal_nan=True), k
+
+
+@pytest.mark.parametrize("storage,top_k", [("f32", 4), ("f16", 4), ("u8", 4), ("f16", 0)])
+def test_binned_phases_bit_identical(storage, top_k):
+    """Binned init and half-sweeps (each tile's hypotheses sorted by inverse
+    depth in LDS and evaluated bin by bin; early exit against the cost from
+    the start of the sweep) give the fused kernels' planes, costs and
+    supports bit for bit: on every phase, on the leading two, under every
+    tile schedule, for top-k and all-views aggregation, with random and
+    seeded init, and under the tuned plan."""
+    say("binned vs fused phases, sphere_planes 480x360 11x11/2", storage, "top_k", top_k)
+    sc = scenes.sphere_planes(width=480, height=360, focal=625.0, supersample=1)
+    srcs = list(range(1, 9))
+    imgs = [sc.u8(i) for i in range(sc.num_views)] if storage == "u8" else sc.images
+    pm = plane.PlaneMVS(sc.cams, imgs, storage=storage)
+    p = plane.default_params(patch_size=11, window_stride=2, top_k=top_k, iterations=3,
+                             min_depth_m=0.25, max_depth_m=0.60, value_scale=1.0,
+                             view_phased_iterations=0, occupancy_cap=-1)
+    outs = []
+    for binned, sched in ((0, 0), (7, 0), (2, 0), (7, 1), (7, 2), (0, 2)):
+        pm.patchmatch(0, srcs, p.copy(binned_phases=binned, tile_schedule=sched))
+        outs.append(pm.depth_map(0))
+    # tuned schedule and per-phase configuration (binned among the trials)
+    pm.patchmatch(0, srcs, p.copy(binned_phases=-1, tile_schedule=-1, occupancy_cap=0))
+    outs.append(pm.depth_map(0))
+    for o in outs[1:]:
+        for k in ("depth", "normals", "cost", "support"):
+            assert np.array_equal(o[k], outs[0][k], equal_nan=True), k
+    assert np.isfinite(outs[0]["depth"]).mean() > 0.9
+    rng = np.random.default_rng(5)
+    h, w = sc.depths[0].shape
+    us, vs = rng.uniform(0, w, 400), rng.uniform(0, h, 400)
+    t = sc.depths[0][vs.astype(int), us.astype(int)]
+    seeds = np.stack([us, vs, t], 1)[t > 0]
+    seeded = []
+    for binned in (0, 3):
+        pm.patchmatch(0, srcs, p.copy(binned_phases=binned, iterations=1, tile_schedule=1),
+                      seed_points=seeds)
         seeded.append(pm.depth_map(0))
     for k in ("depth", "normals", "cost", "support"):
         assert np.array_equal(seeded[1][k], seeded[0][k], equal_nan=True), k
