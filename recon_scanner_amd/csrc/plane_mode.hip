@@ -28,6 +28,21 @@
 
 #include "plane_common.hpp"
 
+// Init plane of pixel (x, y): random, or the fronto-parallel plane at the
+// mean depth of its seeded cell.
+__device__ __forceinline__ Hyp InitPlane(const PlaneArgs& a, int x, int y) {
+  float ray[3];
+  PixelRay(a, x, y, ray);
+  const uint32_t key = RngKey(a.seed, 0u, static_cast<uint32_t>(y * a.w + x));
+  Hyp h = RandomPlane(a, key, 0, ray);
+  if (a.seed_cells) {
+    // a seeded cell starts on the fronto-parallel plane at its mean depth
+    const float sd = a.seed_cells[min(y >> 5, a.seed_gy - 1) * a.seed_gx + min(x >> 5, a.seed_gx - 1)];
+    if (sd > 0.0f) h = Hyp{sd, 0.0f, 0.0f, -1.0f};
+  }
+  return h;
+}
+
 // Random init + initial cost of every pixel (both colours) of one tile.
 template <int HALF, int STRIDE, int FMT, int TW, int TH>
 __device__ __forceinline__ void InitTile(const PlaneArgs& a, RefTile<HALF, FMT, TW, TH>& tile,
@@ -41,16 +56,8 @@ __device__ __forceinline__ void InitTile(const PlaneArgs& a, RefTile<HALF, FMT, 
     if (x >= a.w || y >= a.h) continue;
     RefWindow<HALF, STRIDE> win;
     tile.template window<STRIDE>(win, x, x0, r);
-    float ray[3];
-    PixelRay(a, x, y, ray);
     const int idx = y * a.w + x;
-    const uint32_t key = RngKey(a.seed, 0u, static_cast<uint32_t>(idx));
-    Hyp h = RandomPlane(a, key, 0, ray);
-    if (a.seed_cells) {
-      // a seeded cell starts on the fronto-parallel plane at its mean depth
-      const float sd = a.seed_cells[min(y >> 5, a.seed_gy - 1) * a.seed_gx + min(x >> 5, a.seed_gx - 1)];
-      if (sd > 0.0f) h = Hyp{sd, 0.0f, 0.0f, -1.0f};
-    }
+    const Hyp h = InitPlane(a, x, y);
     int nv = 0;
     const float c = PlaneCost<HALF, STRIDE, FMT>(a, x, y, win, h, &nv, nullptr);
     a.plane[idx] = make_float4(h.d, h.nx, h.ny, h.nz);
@@ -253,11 +260,17 @@ void k_plane_sweep(PlaneArgs a) {
 constexpr int kBins = 64;
 constexpr int kBinnedEntries = 2048;  // sweep: 256 pixels x 8 candidates; init: 512 pixels x 1
 
+// LDS of the binned forms (about 48 KB: 3 workgroups per CU).  Entries hold
+// no hypothesis: the propagated ones (candidates 0-3) are kept per pixel, the
+// perturbed and random ones (4-7) and the init planes are regenerated from
+// the counter RNG by the evaluating thread -- the same operations on the same
+// inputs, so the same bits.
 template <int HALF, int FMT, int TW, int TH>
 struct BinnedLds {
   RefTile<HALF, FMT, TW, TH> tile;
-  float4 cand[kBinnedEntries];     // hypothesis of entry e
-  float res[kBinnedEntries];       // its cost (kCostInf: rejected)
+  float4 prop[256 * 4];            // sweep: pixel t's propagated candidate c at [4 t + c]
+  float4 base[256];                // sweep: pixel t's plane at the start of the sweep
+  float res[kBinnedEntries];       // cost of entry e (kCostInf: rejected)
   uint8_t nv[kBinnedEntries];      // its valid views
   uint16_t list[kBinnedEntries];   // entries in bin order
   float4 wstat[512];               // per pixel slot: the reference window's sum_c, sum_c2, n, mean
@@ -273,8 +286,8 @@ __device__ __forceinline__ int InvDepthBin(const PlaneArgs& a, float d) {
 }
 
 // Counting sort of the workgroup's entries by bin.  Each thread passes its
-// NE entries as (bin << 16 | entry), -1 for none; on return L.list
-// holds them bin by bin (order inside a bin unspecified) and L.n their count.
+// NE entries as (bin << 16 | entry), -1 for none; on return L.list holds
+// them bin by bin (order inside a bin unspecified) and L.n their count.
 template <int NE, int HALF, int FMT, int TW, int TH>
 __device__ __forceinline__ void BinEntries(BinnedLds<HALF, FMT, TW, TH>& L, const int (&be)[NE]) {
   const int t = threadIdx.x;
@@ -316,12 +329,26 @@ __device__ __forceinline__ void SlotWindow(const BinnedLds<HALF, FMT, TW, TH>& L
   win.full = (ws.z == static_cast<float>(RefWindow<HALF, STRIDE>::S));
 }
 
-// Scores the sorted entries.  Entry e belongs to pixel slot e >> shift_bits
-// (slot s: colour s >> 8 in init, thread s & 255); `sweep`: early exit
-// against the slot's start cost.
+// Perturbed or random candidate c (4-7) of pixel (x, y) in half-sweep
+// (color, iter), from the pixel's start plane `base` (SweepCandidate's).
+__device__ __forceinline__ bool RngCandidate(const PlaneArgs& a, int c, int x, int y, int color, int iter,
+                                             float4 base, Hyp* h) {
+  float ray[3];
+  PixelRay(a, x, y, ray);
+  const uint32_t key = RngKey(a.seed, static_cast<uint32_t>(1 + 2 * iter + color),
+                              static_cast<uint32_t>(y * a.w + x));
+  const float dscale = ldexpf(a.depth_perturb, -iter);
+  const float nscale = ldexpf(a.normal_perturb, -iter);
+  const int none[4] = {-1, -1, -1, -1};
+  return SweepCandidate(a, c, ray, key, base, dscale, nscale, none, none, h);
+}
+
+// Scores the sorted entries.  Init: entry e = pixel slot e (colour e >> 8,
+// thread e & 255); sweep: entry e = candidate e & 7 of thread e >> 3's pixel,
+// with early exit against its start cost.
 template <int HALF, int STRIDE, int FMT, int TW, int TH>
 __device__ __forceinline__ void ScoreEntries(const PlaneArgs& a, BinnedLds<HALF, FMT, TW, TH>& L, int x0,
-                                             int y0, int color, bool sweep) {
+                                             int y0, int color, int iter, bool sweep) {
   const int n = L.n;
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
     const int e = L.list[i];
@@ -332,13 +359,22 @@ __device__ __forceinline__ void ScoreEntries(const PlaneArgs& a, BinnedLds<HALF,
     RefWindow<HALF, STRIDE> win;
     SlotWindow<HALF, STRIDE, FMT, TW, TH>(L, s, x, x0, r, win);
     float reject_at = kCostInf;
+    Hyp h;
     if (sweep) {
       const float c0 = L.start_cost[s];
       reject_at = c0 < kCostInf ? fmaf(c0, 1.0f + 1e-5f, 1e-7f) : kCostInf;
+      const int c = e & 7;
+      if (c < 4) {
+        const float4 p = L.prop[4 * s + c];
+        h = Hyp{p.x, p.y, p.z, p.w};
+      } else {
+        (void)RngCandidate(a, c, x, y, color, iter, L.base[s], &h);
+      }
+    } else {
+      h = InitPlane(a, x, y);
     }
-    const float4 p = L.cand[e];
     int nv = 0;
-    L.res[e] = PlaneCost<HALF, STRIDE, FMT>(a, x, y, win, Hyp{p.x, p.y, p.z, p.w}, &nv, nullptr, reject_at);
+    L.res[e] = PlaneCost<HALF, STRIDE, FMT>(a, x, y, win, h, &nv, nullptr, reject_at);
     L.nv[e] = static_cast<uint8_t>(min(nv, 255));
   }
   __syncthreads();
@@ -360,27 +396,18 @@ __device__ __forceinline__ void InitTileBinned(const PlaneArgs& a, BinnedLds<HAL
     RefWindow<HALF, STRIDE> win;
     L.tile.template window<STRIDE>(win, x, x0, r);
     L.wstat[s] = make_float4(win.sum_c, win.sum_c2, win.n, win.mean);
-    float ray[3];
-    PixelRay(a, x, y, ray);
-    const int idx = y * a.w + x;
-    const uint32_t key = RngKey(a.seed, 0u, static_cast<uint32_t>(idx));
-    Hyp h = RandomPlane(a, key, 0, ray);
-    if (a.seed_cells) {
-      const float sd = a.seed_cells[min(y >> 5, a.seed_gy - 1) * a.seed_gx + min(x >> 5, a.seed_gx - 1)];
-      if (sd > 0.0f) h = Hyp{sd, 0.0f, 0.0f, -1.0f};
-    }
-    L.cand[s] = make_float4(h.d, h.nx, h.ny, h.nz);
-    be[color] = (InvDepthBin(a, h.d) << 16) | s;
+    be[color] = (InvDepthBin(a, InitPlane(a, x, y).d) << 16) | s;
   }
   BinEntries(L, be);
-  ScoreEntries<HALF, STRIDE, FMT, TW, TH>(a, L, x0, y0, 0, false);
+  ScoreEntries<HALF, STRIDE, FMT, TW, TH>(a, L, x0, y0, 0, 0, false);
   for (int color = 0; color < 2; ++color) {
     if (be[color] < 0) continue;
     int x, y, r;
     L.tile.pixel(x0, y0, color, &x, &y, &r);
     const int s = be[color] & 0xFFFF;
     const int idx = y * a.w + x;
-    a.plane[idx] = L.cand[s];
+    const Hyp h = InitPlane(a, x, y);
+    a.plane[idx] = make_float4(h.d, h.nx, h.ny, h.nz);
     a.cost[idx] = L.res[s];
     a.support[idx] = L.nv[s];
   }
@@ -415,6 +442,7 @@ __device__ __forceinline__ void SweepTileBinned(const PlaneArgs& a, BinnedLds<HA
     ccur = a.cost[idx];
     scur = a.support[idx];
     L.start_cost[t] = ccur;
+    L.base[t] = cur;
     const uint32_t key = RngKey(a.seed, static_cast<uint32_t>(1 + 2 * iter + color),
                                 static_cast<uint32_t>(idx));
     const float dscale = ldexpf(a.depth_perturb, -iter);
@@ -429,6 +457,7 @@ __device__ __forceinline__ void SweepTileBinned(const PlaneArgs& a, BinnedLds<HA
       Hyp h{0.0f, 0.0f, 0.0f, 0.0f};
       pok[c] = SweepCandidate(a, c, ray, key, base, dscale, nscale, src_x, src_y, &h);
       pc[c] = make_float4(h.d, h.nx, h.ny, h.nz);
+      L.prop[4 * t + c] = pc[c];
     }
     int dup_mask = 0;
 #pragma unroll
@@ -450,34 +479,39 @@ __device__ __forceinline__ void SweepTileBinned(const PlaneArgs& a, BinnedLds<HA
       }
       // PlaneCost scores these kCostInf without any view: never taken
       if (!(h.d >= a.min_depth && h.d <= a.max_depth) || !MakeHypGeom(a, x, y, h).front) continue;
-      const int e = t * 8 + c;
-      L.cand[e] = make_float4(h.d, h.nx, h.ny, h.nz);
-      be[c] = (InvDepthBin(a, h.d) << 16) | e;
+      be[c] = (InvDepthBin(a, h.d) << 16) | (t * 8 + c);
     }
   }
   BinEntries(L, be);
-  ScoreEntries<HALF, STRIDE, FMT, TW, TH>(a, L, x0, y0, color, true);
+  ScoreEntries<HALF, STRIDE, FMT, TW, TH>(a, L, x0, y0, color, iter, true);
   if (!in) return;
+  int best = -1;
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
     if (be[c] < 0) continue;
-    const int e = t * 8 + c;
-    const float cc = L.res[e];
+    const float cc = L.res[t * 8 + c];
     if (cc < ccur) {
       ccur = cc;
-      cur = L.cand[e];
-      scur = L.nv[e];
+      scur = L.nv[t * 8 + c];
+      best = c;
     }
+  }
+  if (best >= 4) {
+    Hyp h;
+    (void)RngCandidate(a, best, x, y, color, iter, L.base[t], &h);
+    cur = make_float4(h.d, h.nx, h.ny, h.nz);
+  } else if (best >= 0) {
+    cur = L.prop[4 * t + best];
   }
   a.plane[idx] = cur;
   a.cost[idx] = ccur;
   a.support[idx] = static_cast<uint8_t>(min(scur, 255));
 }
 
-// Single-view launches of the binned forms (LDS: ~60 KB, 2 workgroups per
-// CU; register budget of 2 waves per SIMD).
+// Single-view launches of the binned forms (3 workgroups per CU, as the
+// fused forms).
 template <int HALF, int STRIDE, int FMT, int TW, int TH>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STRIDE == 2 ? 3 : 1)))
 void k_plane_init_binned(PlaneArgs a) {
   __shared__ BinnedLds<HALF, FMT, TW, TH> L;
   int tx, ty;
@@ -486,7 +520,7 @@ void k_plane_init_binned(PlaneArgs a) {
 }
 
 template <int HALF, int STRIDE, int FMT, int TW, int TH>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STRIDE == 2 ? 3 : 1)))
 void k_plane_sweep_binned(PlaneArgs a) {
   __shared__ BinnedLds<HALF, FMT, TW, TH> L;
   int tx, ty;
@@ -664,7 +698,7 @@ template <bool BINNED, int HALF, int FMT, int TW, int TH>
 using BatchLds = std::conditional_t<BINNED, BinnedLds<HALF, FMT, TW, TH>, RefTile<HALF, FMT, TW, TH>>;
 
 template <int HALF, int STRIDE, int FMT, int TW, int TH, bool BINNED>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BINNED ? 2 : (STRIDE == 2 ? 3 : 1))))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STRIDE == 2 ? 3 : 1)))
 void k_plane_batch(const PlaneArgs* __restrict__ args, const int2* __restrict__ units,
                    const int* __restrict__ qoff, unsigned int* __restrict__ counters, int phase,
                    const int32_t* cancel) {
