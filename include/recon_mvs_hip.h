@@ -380,8 +380,8 @@ typedef struct recon_mvs_plane_params {
      along that view's epipolar lines, so the unconverged hypotheses' window
      gathers touch one source image at a time and stay in the L2 (11x11
      stride-2 window only; other windows ignore it).  -1 = automatic (the
-     default: the first iteration for FP32 images, none for FP16 / BF16),
-     0 = none.  Results do not depend on it. */
+     default: none; the first iteration for FP32 images when binned_phases
+     is 0), 0 = none.  Results do not depend on it. */
   int32_t view_phased_iterations;
   /* Leading phases (phase 0 = random init, then the half-sweeps in order)
      whose hypotheses are evaluated in inverse-depth order: each tile's

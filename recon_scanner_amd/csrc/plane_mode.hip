@@ -1236,6 +1236,8 @@ inline size_t CapBytes(int cap, size_t stat) {
 constexpr int kBinnedCfg = 1 << 8;
 inline int CfgCap(int cfg) { return cfg & 0xFF; }
 inline bool CfgBinned(int cfg) { return (cfg & kBinnedCfg) != 0; }
+// Whether the binned form exists for p's window.
+inline bool HasBinned(const recon_mvs_plane_params* p) { return p->patch_size == 11 && p->window_stride == 2; }
 
 // Workgroups of a persistent launch: as many as fit on the device at once
 // (with `cap` workgroups per CU at most).
@@ -1547,8 +1549,10 @@ std::string TuneKey(const recon_mvs_hip_context* ctx, const PlaneArgs& a, const 
 
 // Times random init + two half-sweeps of view `a` under every compiled tile
 // schedule and returns the fastest (the state of the slot is overwritten).
+// The phases run binned where that form may be used, as the run will.
 int32_t Autotune(recon_mvs_hip_context* ctx, const PlaneArgs& a, const recon_mvs_plane_params* p,
                  int* best, double* ms_out) {
+  const int cfg = HasBinned(p) && p->binned_phases != 0 ? kBinnedCfg : 0;
   hipEvent_t e0, e1;
   RMVS_HIP(ctx, hipEventCreate(&e0));
   RMVS_HIP(ctx, hipEventCreate(&e1));
@@ -1561,7 +1565,7 @@ int32_t Autotune(recon_mvs_hip_context* ctx, const PlaneArgs& a, const recon_mvs
     if (ms_out) ms_out[s] = -1.0;
     // warm-up launch (code load), then the timed one
     PlaneArgs b = a;
-    if (Launch(ctx, p, kInit, b, 0.0f, 0, 0, nullptr, nullptr, nullptr, nullptr, nullptr, s) !=
+    if (Launch(ctx, p, kInit, b, 0.0f, 0, 0, nullptr, nullptr, nullptr, nullptr, nullptr, s, cfg) !=
         RECON_MVS_OK) {
       ctx->error.clear();
       continue;
@@ -1570,7 +1574,7 @@ int32_t Autotune(recon_mvs_hip_context* ctx, const PlaneArgs& a, const recon_mvs
     for (int c = 0; c < 2 && st == RECON_MVS_OK; ++c) {
       b.iter = 0;
       b.color = c;
-      st = Launch(ctx, p, kSweep, b, 0.0f, 0, 0, nullptr, nullptr, nullptr, nullptr, nullptr, s);
+      st = Launch(ctx, p, kSweep, b, 0.0f, 0, 0, nullptr, nullptr, nullptr, nullptr, nullptr, s, cfg);
     }
     (void)hipEventRecord(e1, ctx->stream);
     if (st != RECON_MVS_OK) break;
@@ -1647,15 +1651,12 @@ int32_t TimePhases(recon_mvs_hip_context* ctx, PlaneArgs a, const recon_mvs_plan
   return st;
 }
 
-// Whether the binned form exists for p's window.
-inline bool HasBinned(const recon_mvs_plane_params* p) { return p->patch_size == 11 && p->window_stride == 2; }
-
 // Configuration (occupancy cap, binned form) of every phase of a PatchMatch
 // run (index 0: init; 1 + 2 it + c: half-sweep c of iteration it), as
 // recon_mvs_plane_params.occupancy_cap and binned_phases describe.  Tuning
-// runs the view under caps 0 (none), 1 and 2 and, with binned_phases = -1,
-// in the binned form, and keeps each phase's fastest; the phases compute the
-// same results under any configuration.
+// runs the view under caps 0 (none), 1 and 2, fused and (binned_phases = -1)
+// binned, and keeps each phase's fastest; the phases compute the same
+// results under any configuration.
 int32_t ResolveCaps(recon_mvs_hip_context* ctx, const PlaneArgs& a, const recon_mvs_plane_params* p,
                     int sched, std::vector<int>* caps) {
   const int nph = 1 + 2 * p->iterations;
@@ -1689,7 +1690,8 @@ int32_t ResolveCaps(recon_mvs_hip_context* ctx, const PlaneArgs& a, const recon_
   const bool timing = ctx->timing;
   ctx->timing = false;
   std::vector<int> trials = {0, 1, 2};
-  if (binnable && p->binned_phases < 0) trials.push_back(kBinnedCfg);
+  if (binnable && p->binned_phases < 0)
+    for (int c : {kBinnedCfg, kBinnedCfg | 1, kBinnedCfg | 2}) trials.push_back(c);
   std::vector<float> best(nph, 3.0e38f), ms;
   int32_t st = TimePhases(ctx, a, p, sched, 0, &ms);  // warm-up (code load)
   for (size_t t = 0; t < trials.size() && st == RECON_MVS_OK; ++t) {
@@ -1896,13 +1898,15 @@ int32_t RunPlanePatchmatch(recon_mvs_hip_context* ctx, int32_t ref_view, int32_t
               caps[0]);
   if (st != RECON_MVS_OK) return st;
   // view-phased leading iterations (the 11x11 stride-2 window only);
-  // automatic: the first iteration for FP32 images (FP16 / BF16 images are
-  // half the bytes, and their fused sweeps already keep to the L2)
+  // automatic: none, unless the binned form is switched off, then the first
+  // iteration for FP32 images (binned sweeps keep the unconverged gathers in
+  // the caches better; FP16 / BF16 images are half the bytes, and their
+  // fused sweeps already keep to the L2)
   int phased = 0;
   if (params->patch_size == 11 && params->window_stride == 2)
     phased = params->view_phased_iterations >= 0
                  ? params->view_phased_iterations
-                 : (ctx->storage_format == RECON_MVS_PIXEL_F32 ? 1 : 0);
+                 : (params->binned_phases == 0 && ctx->storage_format == RECON_MVS_PIXEL_F32 ? 1 : 0);
   for (int it = 0; it < params->iterations; ++it) {
     for (int color = 0; color < 2; ++color) {
       a.iter = it;
@@ -2318,8 +2322,8 @@ RECON_MVS_API int32_t recon_mvs_hip_batch_create(recon_mvs_hip_context* ctx, int
   // configuration per phase (index 0: init): occupancy cap none
   // (occupancy_cap < 0), the given cap in the leading capped phases (> 0),
   // or tuned (0): every candidate schedule's phases are timed on the batch
-  // itself uncapped, at 2 workgroups per CU and (binned_phases = -1) in the
-  // binned form, each phase keeps its fastest, and the schedule with the
+  // itself uncapped and at 2 workgroups per CU, fused and (binned_phases =
+  // -1) binned, each phase keeps its fastest, and the schedule with the
   // least total wins; binned_phases > 0 forces the binned form on the
   // leading phases.  Results do not depend on any of these choices.
   int sched = scheds[0];
@@ -2336,7 +2340,10 @@ RECON_MVS_API int32_t recon_mvs_hip_batch_create(recon_mvs_hip_context* ctx, int
     ctx->timing = false;
     double best_total = 1e300;
     std::vector<int> trial_cfgs = {0, 2};
-    if (HasBinned(params) && params->binned_phases < 0) trial_cfgs.push_back(kBinnedCfg);
+    if (HasBinned(params) && params->binned_phases < 0) {
+      trial_cfgs.push_back(kBinnedCfg);
+      trial_cfgs.push_back(kBinnedCfg | 2);
+    }
     for (int sc : scheds) {
       if (!upload(sc)) {
         ctx->timing = timing;
