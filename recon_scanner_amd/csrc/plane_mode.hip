@@ -243,11 +243,15 @@ void k_plane_sweep(PlaneArgs a) {
 // segments of each source: each gather touches ~64 cache lines and nothing is
 // reused across lanes.  Here the owning threads first write their pixel's
 // hypotheses (entries) to LDS, a counting sort puts the entries in 64 bins
-// of inverse depth over [inv_min, inv_max], and the workgroup evaluates the
-// sorted list: a wave's 64 consecutive entries lie within a few bins, so
-// their windows in every source sit in one compact region (the tile's extent
-// plus a short piece of the epipolar lines) and share cache lines.  The
-// owning threads then pick each pixel's best entry in candidate order.
+// of inverse depth over [inv_min, inv_max] (the entries of pixels whose
+// reference window is cut by the image border after all the others), and
+// the workgroup evaluates the sorted list: a wave's 64 consecutive entries
+// lie within a few bins, so their windows in every source sit in one compact
+// region (the tile's extent plus a short piece of the epipolar lines) and
+// share cache lines, and the window loops that check reference samples stay
+// in the waves of border pixels.  Evaluating the list also compacts the
+// work: duplicate and invalid candidates leave no idle lanes.  The owning
+// threads then pick each pixel's best entry in candidate order.
 //
 // Results are bit-identical to InitTile / SweepTile: every entry is scored
 // by the same PlaneCost operations (a pixel's result does not depend on
@@ -257,7 +261,7 @@ void k_plane_sweep(PlaneArgs a) {
 // running best anyway; and the choice runs in candidate order with the same
 // strict comparison.
 // ---------------------------------------------------------------------------
-constexpr int kBins = 64;
+constexpr int kBins = 64;                 // inverse-depth bins per pixel class
 constexpr int kBinnedEntries = 2048;  // sweep: 256 pixels x 8 candidates; init: 512 pixels x 1
 
 // LDS of the binned forms (about 48 KB: 3 workgroups per CU).  Entries hold
@@ -275,14 +279,16 @@ struct BinnedLds {
   uint16_t list[kBinnedEntries];   // entries in bin order
   float4 wstat[512];               // per pixel slot: the reference window's sum_c, sum_c2, n, mean
   float start_cost[256];           // sweep: each pixel's cost at the start of the sweep
-  int bin[kBins];                  // counts, then first positions
+  int bin[2 * kBins];              // counts, then first positions (whole windows first, then by depth)
   int n;                           // entries in the list
 };
 
-// Bin of inverse depth 1/d in [inv_min, inv_max] (clamped; NaN -> 0).
-__device__ __forceinline__ int InvDepthBin(const PlaneArgs& a, float d) {
+// Sort bin of a hypothesis at depth d of a pixel whose reference window is
+// whole (`full`) or not: the bin of 1/d in [inv_min, inv_max] (clamped; NaN
+// -> 0), after all kBins of the whole windows when not.
+__device__ __forceinline__ int SortBin(const PlaneArgs& a, float d, bool full) {
   const float u = (1.0f / d - a.inv_min) * (static_cast<float>(kBins) / (a.inv_max - a.inv_min));
-  return static_cast<int>(fminf(fmaxf(u, 0.0f), static_cast<float>(kBins - 1)));
+  return static_cast<int>(fminf(fmaxf(u, 0.0f), static_cast<float>(kBins - 1))) + (full ? 0 : kBins);
 }
 
 // Counting sort of the workgroup's entries by bin.  Each thread passes its
@@ -291,7 +297,7 @@ __device__ __forceinline__ int InvDepthBin(const PlaneArgs& a, float d) {
 template <int NE, int HALF, int FMT, int TW, int TH>
 __device__ __forceinline__ void BinEntries(BinnedLds<HALF, FMT, TW, TH>& L, const int (&be)[NE]) {
   const int t = threadIdx.x;
-  if (t < kBins) L.bin[t] = 0;
+  if (t < 2 * kBins) L.bin[t] = 0;
   __syncthreads();
   int rank[NE];
 #pragma unroll
@@ -299,7 +305,7 @@ __device__ __forceinline__ void BinEntries(BinnedLds<HALF, FMT, TW, TH>& L, cons
   __syncthreads();
   if (t == 0) {
     int s = 0;
-    for (int b = 0; b < kBins; ++b) {
+    for (int b = 0; b < 2 * kBins; ++b) {
       const int c = L.bin[b];
       L.bin[b] = s;
       s += c;
@@ -396,7 +402,7 @@ __device__ __forceinline__ void InitTileBinned(const PlaneArgs& a, BinnedLds<HAL
     RefWindow<HALF, STRIDE> win;
     L.tile.template window<STRIDE>(win, x, x0, r);
     L.wstat[s] = make_float4(win.sum_c, win.sum_c2, win.n, win.mean);
-    be[color] = (InvDepthBin(a, InitPlane(a, x, y).d) << 16) | s;
+    be[color] = (SortBin(a, InitPlane(a, x, y).d, win.full) << 16) | s;
   }
   BinEntries(L, be);
   ScoreEntries<HALF, STRIDE, FMT, TW, TH>(a, L, x0, y0, 0, 0, false);
@@ -479,7 +485,7 @@ __device__ __forceinline__ void SweepTileBinned(const PlaneArgs& a, BinnedLds<HA
       }
       // PlaneCost scores these kCostInf without any view: never taken
       if (!(h.d >= a.min_depth && h.d <= a.max_depth) || !MakeHypGeom(a, x, y, h).front) continue;
-      be[c] = (InvDepthBin(a, h.d) << 16) | (t * 8 + c);
+      be[c] = (SortBin(a, h.d, win.full) << 16) | (t * 8 + c);
     }
   }
   BinEntries(L, be);
