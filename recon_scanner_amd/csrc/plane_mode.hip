@@ -156,7 +156,19 @@ __device__ __forceinline__ bool SweepCandidate(const PlaneArgs& a, int c, const 
   return ok;
 }
 
+// Whether hypothesis h is the plane p bit for bit.
+__device__ __forceinline__ bool SamePlane(const Hyp& h, float4 p) {
+  return h.d == p.x && h.nx == p.y && h.ny == p.z && h.nz == p.w;
+}
+
 // One red-black half-sweep of one tile over the pixels of colour `color`.
+//
+// Candidates that are bit for bit an earlier candidate or the pixel's
+// current plane are skipped: they would score the same cost by the same
+// operations (the current cost is always the full PlaneCost of the current
+// plane: the init's, or an accepted candidate's, and an accepted candidate
+// never took the early exit), which lost or was taken, or equals the
+// current cost and loses the strict comparison.
 template <int HALF, int STRIDE, int FMT, int TW, int TH>
 __device__ __forceinline__ void SweepTile(const PlaneArgs& a, RefTile<HALF, FMT, TW, TH>& tile,
                                           int tx, int ty, int color, int iter) {
@@ -181,8 +193,8 @@ __device__ __forceinline__ void SweepTile(const PlaneArgs& a, RefTile<HALF, FMT,
   const float4 base = cur;
   int src_x[4], src_y[4];
   PropagationSources(a, x, y, src_x, src_y);
-  // propagated candidates equal to an earlier one (bit c): they would score
-  // the same cost again by the same operations, which lost or was taken
+  // propagated candidates equal to an earlier one or to the current plane
+  // (bit c)
   int dup_mask = 0;
   {
     float4 pc[4];
@@ -192,6 +204,7 @@ __device__ __forceinline__ void SweepTile(const PlaneArgs& a, RefTile<HALF, FMT,
       Hyp h{0.0f, 0.0f, 0.0f, 0.0f};
       pok[c] = SweepCandidate(a, c, ray, key, base, dscale, nscale, src_x, src_y, &h);
       pc[c] = make_float4(h.d, h.nx, h.ny, h.nz);
+      if (pok[c] && SamePlane(h, base)) dup_mask |= 1 << c;
     }
 #pragma unroll
     for (int c = 1; c < 4; ++c)
@@ -205,6 +218,7 @@ __device__ __forceinline__ void SweepTile(const PlaneArgs& a, RefTile<HALF, FMT,
     if ((dup_mask >> c) & 1) continue;
     Hyp h;
     if (!SweepCandidate(a, c, ray, key, base, dscale, nscale, src_x, src_y, &h)) continue;
+    if (c >= 4 && SamePlane(h, base)) continue;
     int nv = 0;
     // candidates that provably cannot beat the current cost stop early (the
     // margin covers the rounding of the bound against the full sum)
@@ -465,7 +479,13 @@ __device__ __forceinline__ void SweepTileBinned(const PlaneArgs& a, BinnedLds<HA
       pc[c] = make_float4(h.d, h.nx, h.ny, h.nz);
       L.prop[4 * t + c] = pc[c];
     }
+    // candidates equal to an earlier one or to the current plane are
+    // skipped, as in SweepTile
     int dup_mask = 0;
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      if (pok[c] && pc[c].x == base.x && pc[c].y == base.y && pc[c].z == base.z && pc[c].w == base.w)
+        dup_mask |= 1 << c;
 #pragma unroll
     for (int c = 1; c < 4; ++c)
 #pragma unroll
@@ -480,7 +500,8 @@ __device__ __forceinline__ void SweepTileBinned(const PlaneArgs& a, BinnedLds<HA
       if (c < 4) {
         if (!pok[c]) continue;
         h = Hyp{pc[c].x, pc[c].y, pc[c].z, pc[c].w};
-      } else if (!SweepCandidate(a, c, ray, key, base, dscale, nscale, src_x, src_y, &h)) {
+      } else if (!SweepCandidate(a, c, ray, key, base, dscale, nscale, src_x, src_y, &h) ||
+                 SamePlane(h, base)) {
         continue;
       }
       // PlaneCost scores these kCostInf without any view: never taken
