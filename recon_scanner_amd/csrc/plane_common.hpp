@@ -29,6 +29,7 @@ struct ResidentView {
 struct PlaneState {
   int w = 0, h = 0;
   float4* plane = nullptr;  // (depth, nx, ny, nz)
+  float4* prev = nullptr;   // per pixel, the propagated candidates of its last half-sweep (4)
   float* cost = nullptr;
   uint8_t* support = nullptr;
   uint8_t* geo_support = nullptr;
@@ -76,6 +77,10 @@ struct PlaneArgs {
   double A[kMaxViews][9];       // K_s R_rel K_r^-1 (row-major)
   double b[kMaxViews][3];       // K_s t_rel
   float4* plane;
+  // per pixel 4 planes it has been scored on in an earlier half-sweep of the
+  // run (its propagated candidates; NaN: none), or null: such a plane cannot
+  // beat the pixel's cost, which only falls during a run
+  float4* prev;
   float* cost;
   uint8_t* support;
   const float* seed_cells;      // init: mean seed depth per 32 x 32 cell (0 = none), or null

@@ -43,6 +43,14 @@ __device__ __forceinline__ Hyp InitPlane(const PlaneArgs& a, int x, int y) {
   return h;
 }
 
+// A new run: pixel idx has been scored on no plane yet.
+__device__ __forceinline__ void ClearPrev(const PlaneArgs& a, int idx) {
+  if (!a.prev) return;
+  const float n = __builtin_nanf("");
+#pragma unroll
+  for (int k = 0; k < 4; ++k) a.prev[4 * idx + k] = make_float4(n, n, n, n);
+}
+
 // Random init + initial cost of every pixel (both colours) of one tile.
 template <int HALF, int STRIDE, int FMT, int TW, int TH>
 __device__ __forceinline__ void InitTile(const PlaneArgs& a, RefTile<HALF, FMT, TW, TH>& tile,
@@ -63,6 +71,7 @@ __device__ __forceinline__ void InitTile(const PlaneArgs& a, RefTile<HALF, FMT, 
     a.plane[idx] = make_float4(h.d, h.nx, h.ny, h.nz);
     a.cost[idx] = c;
     a.support[idx] = static_cast<uint8_t>(min(nv, 255));
+    ClearPrev(a, idx);
   }
 }
 
@@ -160,15 +169,46 @@ __device__ __forceinline__ bool SweepCandidate(const PlaneArgs& a, int c, const 
 __device__ __forceinline__ bool SamePlane(const Hyp& h, float4 p) {
   return h.d == p.x && h.nx == p.y && h.ny == p.z && h.nz == p.w;
 }
+__device__ __forceinline__ bool SamePlane4(float4 h, float4 p) {
+  return h.x == p.x && h.y == p.y && h.z == p.z && h.w == p.w;
+}
+
+// Propagated candidates (bits of `ok`) of pixel idx that need no scoring:
+// equal to an earlier candidate, to the current plane `base`, or to a plane
+// the pixel was scored on in an earlier half-sweep of the run (a.prev); then
+// records this half-sweep's propagated candidates in a.prev.
+__device__ __forceinline__ int SkipPropagated(const PlaneArgs& a, int idx, const float4 (&pc)[4],
+                                              const bool (&ok)[4], float4 base) {
+  int skip = 0;
+  float4 old[4];
+  const float n = __builtin_nanf("");
+#pragma unroll
+  for (int k = 0; k < 4; ++k) old[k] = a.prev ? a.prev[4 * idx + k] : make_float4(n, n, n, n);
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    bool same = SamePlane4(pc[c], base);
+#pragma unroll
+    for (int k = 0; k < c; ++k) same = same || (ok[k] && SamePlane4(pc[c], pc[k]));
+#pragma unroll
+    for (int k = 0; k < 4; ++k) same = same || SamePlane4(pc[c], old[k]);
+    if (ok[c] && same) skip |= 1 << c;
+  }
+  if (a.prev)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) a.prev[4 * idx + c] = ok[c] ? pc[c] : make_float4(n, n, n, n);
+  return skip;
+}
 
 // One red-black half-sweep of one tile over the pixels of colour `color`.
 //
-// Candidates that are bit for bit an earlier candidate or the pixel's
-// current plane are skipped: they would score the same cost by the same
-// operations (the current cost is always the full PlaneCost of the current
-// plane: the init's, or an accepted candidate's, and an accepted candidate
-// never took the early exit), which lost or was taken, or equals the
-// current cost and loses the strict comparison.
+// Candidates that are bit for bit an earlier candidate, the pixel's current
+// plane, or (propagated ones) a plane the pixel was scored on in an earlier
+// half-sweep of the run are skipped: they would score the same cost by the
+// same operations, which lost or was taken, or equals the current cost and
+// loses the strict comparison (the current cost is always the full PlaneCost
+// of the current plane: the init's, or an accepted candidate's, and an
+// accepted candidate never took the early exit), or, scored earlier, was
+// not below the pixel's cost then, and that cost only falls during a run.
 template <int HALF, int STRIDE, int FMT, int TW, int TH>
 __device__ __forceinline__ void SweepTile(const PlaneArgs& a, RefTile<HALF, FMT, TW, TH>& tile,
                                           int tx, int ty, int color, int iter) {
@@ -193,8 +233,7 @@ __device__ __forceinline__ void SweepTile(const PlaneArgs& a, RefTile<HALF, FMT,
   const float4 base = cur;
   int src_x[4], src_y[4];
   PropagationSources(a, x, y, src_x, src_y);
-  // propagated candidates equal to an earlier one or to the current plane
-  // (bit c)
+  // propagated candidates that need no scoring (bit c)
   int dup_mask = 0;
   {
     float4 pc[4];
@@ -204,15 +243,8 @@ __device__ __forceinline__ void SweepTile(const PlaneArgs& a, RefTile<HALF, FMT,
       Hyp h{0.0f, 0.0f, 0.0f, 0.0f};
       pok[c] = SweepCandidate(a, c, ray, key, base, dscale, nscale, src_x, src_y, &h);
       pc[c] = make_float4(h.d, h.nx, h.ny, h.nz);
-      if (pok[c] && SamePlane(h, base)) dup_mask |= 1 << c;
     }
-#pragma unroll
-    for (int c = 1; c < 4; ++c)
-#pragma unroll
-      for (int k = 0; k < c; ++k)
-        if (pok[c] && pok[k] && pc[c].x == pc[k].x && pc[c].y == pc[k].y && pc[c].z == pc[k].z &&
-            pc[c].w == pc[k].w)
-          dup_mask |= 1 << c;
+    dup_mask = SkipPropagated(a, idx, pc, pok, base);
   }
   for (int c = 0; c < 8; ++c) {
     if ((dup_mask >> c) & 1) continue;
@@ -430,6 +462,7 @@ __device__ __forceinline__ void InitTileBinned(const PlaneArgs& a, BinnedLds<HAL
     a.plane[idx] = make_float4(h.d, h.nx, h.ny, h.nz);
     a.cost[idx] = L.res[s];
     a.support[idx] = L.nv[s];
+    ClearPrev(a, idx);
   }
 }
 
@@ -479,20 +512,8 @@ __device__ __forceinline__ void SweepTileBinned(const PlaneArgs& a, BinnedLds<HA
       pc[c] = make_float4(h.d, h.nx, h.ny, h.nz);
       L.prop[4 * t + c] = pc[c];
     }
-    // candidates equal to an earlier one or to the current plane are
-    // skipped, as in SweepTile
-    int dup_mask = 0;
-#pragma unroll
-    for (int c = 0; c < 4; ++c)
-      if (pok[c] && pc[c].x == base.x && pc[c].y == base.y && pc[c].z == base.z && pc[c].w == base.w)
-        dup_mask |= 1 << c;
-#pragma unroll
-    for (int c = 1; c < 4; ++c)
-#pragma unroll
-      for (int k = 0; k < c; ++k)
-        if (pok[c] && pok[k] && pc[c].x == pc[k].x && pc[c].y == pc[k].y && pc[c].z == pc[k].z &&
-            pc[c].w == pc[k].w)
-          dup_mask |= 1 << c;
+    // candidates that need no scoring are skipped, as in SweepTile
+    const int dup_mask = SkipPropagated(a, idx, pc, pok, base);
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
       if ((dup_mask >> c) & 1) continue;
@@ -1057,6 +1078,7 @@ void FreeView(ResidentView* v) {
 void FreeState(PlaneState* s) {
   if (!s) return;
   if (s->plane) (void)hipFree(s->plane);
+  if (s->prev) (void)hipFree(s->prev);
   if (s->cost) (void)hipFree(s->cost);
   if (s->support) (void)hipFree(s->support);
   if (s->geo_support) (void)hipFree(s->geo_support);
@@ -1199,6 +1221,7 @@ int32_t EnsureState(recon_mvs_hip_context* ctx, int view) {
   s->h = V.h;
   const size_t P = static_cast<size_t>(V.w) * V.h;
   if (hipMalloc(&s->plane, P * sizeof(float4)) != hipSuccess ||
+      hipMalloc(&s->prev, 4 * P * sizeof(float4)) != hipSuccess ||
       hipMalloc(&s->cost, P * sizeof(float)) != hipSuccess ||
       hipMalloc(&s->support, P) != hipSuccess || hipMalloc(&s->geo_support, P) != hipSuccess) {
     FreeState(s);
@@ -1893,6 +1916,7 @@ int32_t RunPlanePatchmatch(recon_mvs_hip_context* ctx, int32_t ref_view, int32_t
   PlaneState& S = *ctx->states[ref_view];
   S.has_geo = false;
   a.plane = S.plane;
+  a.prev = S.prev;
   a.cost = S.cost;
   a.support = S.support;
   if (num_seeds > 0) {
@@ -2272,6 +2296,7 @@ RECON_MVS_API int32_t recon_mvs_hip_autotune(recon_mvs_hip_context* ctx, int32_t
   if (st != RECON_MVS_OK) return st;
   PlaneState& S = *ctx->states[ref_view];
   a.plane = S.plane;
+  a.prev = S.prev;
   a.cost = S.cost;
   a.support = S.support;
   int best = 0;
@@ -2301,6 +2326,7 @@ RECON_MVS_API int32_t recon_mvs_hip_batch_create(recon_mvs_hip_context* ctx, int
     PlaneState& S = *ctx->states[jobs[j].ref_view];
     S.has_geo = false;
     args[j].plane = S.plane;
+    args[j].prev = S.prev;
     args[j].cost = S.cost;
     args[j].support = S.support;
   }
