@@ -310,16 +310,16 @@ void k_plane_sweep(PlaneArgs a) {
 constexpr int kBins = 64;                 // inverse-depth bins per pixel class
 constexpr int kBinnedEntries = 2048;  // sweep: 256 pixels x 8 candidates; init: 512 pixels x 1
 
-// LDS of the binned forms (about 48 KB: 3 workgroups per CU).  Entries hold
-// no hypothesis: the propagated ones (candidates 0-3) are kept per pixel, the
-// perturbed and random ones (4-7) and the init planes are regenerated from
-// the counter RNG by the evaluating thread -- the same operations on the same
-// inputs, so the same bits.
+// LDS of the binned forms (about 45 KB: 3 workgroups per CU).  Entries hold
+// no hypothesis: the propagated ones (candidates 0-3) are kept here per
+// pixel, the perturbed and random ones (4-7) in the pixel's slots of
+// PlaneArgs::cand and the init planes in the plane array, each written by
+// the owning thread before the sort and read by the scoring thread after it
+// (the workgroup barriers order the global writes and reads).
 template <int HALF, int FMT, int TW, int TH>
 struct BinnedLds {
   RefTile<HALF, FMT, TW, TH> tile;
   float4 prop[256 * 4];            // sweep: pixel t's propagated candidate c at [4 t + c]
-  float4 base[256];                // sweep: pixel t's plane at the start of the sweep
   float res[kBinnedEntries];       // cost of entry e (kCostInf: rejected)
   uint8_t nv[kBinnedEntries];      // its valid views
   uint16_t list[kBinnedEntries];   // entries in bin order
@@ -381,26 +381,12 @@ __device__ __forceinline__ void SlotWindow(const BinnedLds<HALF, FMT, TW, TH>& L
   win.full = (ws.z == static_cast<float>(RefWindow<HALF, STRIDE>::S));
 }
 
-// Perturbed or random candidate c (4-7) of pixel (x, y) in half-sweep
-// (color, iter), from the pixel's start plane `base` (SweepCandidate's).
-__device__ __forceinline__ bool RngCandidate(const PlaneArgs& a, int c, int x, int y, int color, int iter,
-                                             float4 base, Hyp* h) {
-  float ray[3];
-  PixelRay(a, x, y, ray);
-  const uint32_t key = RngKey(a.seed, static_cast<uint32_t>(1 + 2 * iter + color),
-                              static_cast<uint32_t>(y * a.w + x));
-  const float dscale = ldexpf(a.depth_perturb, -iter);
-  const float nscale = ldexpf(a.normal_perturb, -iter);
-  const int none[4] = {-1, -1, -1, -1};
-  return SweepCandidate(a, c, ray, key, base, dscale, nscale, none, none, h);
-}
-
 // Scores the sorted entries.  Init: entry e = pixel slot e (colour e >> 8,
 // thread e & 255); sweep: entry e = candidate e & 7 of thread e >> 3's pixel,
 // with early exit against its start cost.
 template <int HALF, int STRIDE, int FMT, int TW, int TH>
 __device__ __forceinline__ void ScoreEntries(const PlaneArgs& a, BinnedLds<HALF, FMT, TW, TH>& L, int x0,
-                                             int y0, int color, int iter, bool sweep) {
+                                             int y0, int color, bool sweep) {
   const int n = L.n;
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
     const int e = L.list[i];
@@ -411,20 +397,16 @@ __device__ __forceinline__ void ScoreEntries(const PlaneArgs& a, BinnedLds<HALF,
     RefWindow<HALF, STRIDE> win;
     SlotWindow<HALF, STRIDE, FMT, TW, TH>(L, s, x, x0, r, win);
     float reject_at = kCostInf;
-    Hyp h;
+    float4 p;
     if (sweep) {
       const float c0 = L.start_cost[s];
       reject_at = c0 < kCostInf ? fmaf(c0, 1.0f + 1e-5f, 1e-7f) : kCostInf;
       const int c = e & 7;
-      if (c < 4) {
-        const float4 p = L.prop[4 * s + c];
-        h = Hyp{p.x, p.y, p.z, p.w};
-      } else {
-        (void)RngCandidate(a, c, x, y, color, iter, L.base[s], &h);
-      }
+      p = c < 4 ? L.prop[4 * s + c] : a.cand[4 * (y * a.w + x) + (c - 4)];
     } else {
-      h = InitPlane(a, x, y);
+      p = a.plane[y * a.w + x];  // the init plane, written by the owning thread
     }
+    const Hyp h{p.x, p.y, p.z, p.w};
     int nv = 0;
     L.res[e] = PlaneCost<HALF, STRIDE, FMT>(a, x, y, win, h, &nv, nullptr, reject_at);
     L.nv[e] = static_cast<uint8_t>(min(nv, 255));
@@ -448,18 +430,18 @@ __device__ __forceinline__ void InitTileBinned(const PlaneArgs& a, BinnedLds<HAL
     RefWindow<HALF, STRIDE> win;
     L.tile.template window<STRIDE>(win, x, x0, r);
     L.wstat[s] = make_float4(win.sum_c, win.sum_c2, win.n, win.mean);
-    be[color] = (SortBin(a, InitPlane(a, x, y).d, win.full) << 16) | s;
+    const Hyp h = InitPlane(a, x, y);
+    a.plane[y * a.w + x] = make_float4(h.d, h.nx, h.ny, h.nz);
+    be[color] = (SortBin(a, h.d, win.full) << 16) | s;
   }
   BinEntries(L, be);
-  ScoreEntries<HALF, STRIDE, FMT, TW, TH>(a, L, x0, y0, 0, 0, false);
+  ScoreEntries<HALF, STRIDE, FMT, TW, TH>(a, L, x0, y0, 0, false);
   for (int color = 0; color < 2; ++color) {
     if (be[color] < 0) continue;
     int x, y, r;
     L.tile.pixel(x0, y0, color, &x, &y, &r);
     const int s = be[color] & 0xFFFF;
     const int idx = y * a.w + x;
-    const Hyp h = InitPlane(a, x, y);
-    a.plane[idx] = make_float4(h.d, h.nx, h.ny, h.nz);
     a.cost[idx] = L.res[s];
     a.support[idx] = L.nv[s];
     ClearPrev(a, idx);
@@ -495,7 +477,6 @@ __device__ __forceinline__ void SweepTileBinned(const PlaneArgs& a, BinnedLds<HA
     ccur = a.cost[idx];
     scur = a.support[idx];
     L.start_cost[t] = ccur;
-    L.base[t] = cur;
     const uint32_t key = RngKey(a.seed, static_cast<uint32_t>(1 + 2 * iter + color),
                                 static_cast<uint32_t>(idx));
     const float dscale = ldexpf(a.depth_perturb, -iter);
@@ -527,11 +508,12 @@ __device__ __forceinline__ void SweepTileBinned(const PlaneArgs& a, BinnedLds<HA
       }
       // PlaneCost scores these kCostInf without any view: never taken
       if (!(h.d >= a.min_depth && h.d <= a.max_depth) || !MakeHypGeom(a, x, y, h).front) continue;
+      if (c >= 4) a.cand[4 * idx + (c - 4)] = make_float4(h.d, h.nx, h.ny, h.nz);
       be[c] = (SortBin(a, h.d, win.full) << 16) | (t * 8 + c);
     }
   }
   BinEntries(L, be);
-  ScoreEntries<HALF, STRIDE, FMT, TW, TH>(a, L, x0, y0, color, iter, true);
+  ScoreEntries<HALF, STRIDE, FMT, TW, TH>(a, L, x0, y0, color, true);
   if (!in) return;
   int best = -1;
 #pragma unroll
@@ -544,13 +526,8 @@ __device__ __forceinline__ void SweepTileBinned(const PlaneArgs& a, BinnedLds<HA
       best = c;
     }
   }
-  if (best >= 4) {
-    Hyp h;
-    (void)RngCandidate(a, best, x, y, color, iter, L.base[t], &h);
-    cur = make_float4(h.d, h.nx, h.ny, h.nz);
-  } else if (best >= 0) {
-    cur = L.prop[4 * t + best];
-  }
+  if (best >= 4) cur = a.cand[4 * idx + (best - 4)];
+  else if (best >= 0) cur = L.prop[4 * t + best];
   a.plane[idx] = cur;
   a.cost[idx] = ccur;
   a.support[idx] = static_cast<uint8_t>(min(scur, 255));
@@ -1079,6 +1056,7 @@ void FreeState(PlaneState* s) {
   if (!s) return;
   if (s->plane) (void)hipFree(s->plane);
   if (s->prev) (void)hipFree(s->prev);
+  if (s->cand) (void)hipFree(s->cand);
   if (s->cost) (void)hipFree(s->cost);
   if (s->support) (void)hipFree(s->support);
   if (s->geo_support) (void)hipFree(s->geo_support);
@@ -1222,6 +1200,7 @@ int32_t EnsureState(recon_mvs_hip_context* ctx, int view) {
   const size_t P = static_cast<size_t>(V.w) * V.h;
   if (hipMalloc(&s->plane, P * sizeof(float4)) != hipSuccess ||
       hipMalloc(&s->prev, 4 * P * sizeof(float4)) != hipSuccess ||
+      hipMalloc(&s->cand, 4 * P * sizeof(float4)) != hipSuccess ||
       hipMalloc(&s->cost, P * sizeof(float)) != hipSuccess ||
       hipMalloc(&s->support, P) != hipSuccess || hipMalloc(&s->geo_support, P) != hipSuccess) {
     FreeState(s);
@@ -1917,6 +1896,7 @@ int32_t RunPlanePatchmatch(recon_mvs_hip_context* ctx, int32_t ref_view, int32_t
   S.has_geo = false;
   a.plane = S.plane;
   a.prev = S.prev;
+  a.cand = S.cand;
   a.cost = S.cost;
   a.support = S.support;
   if (num_seeds > 0) {
@@ -2297,6 +2277,7 @@ RECON_MVS_API int32_t recon_mvs_hip_autotune(recon_mvs_hip_context* ctx, int32_t
   PlaneState& S = *ctx->states[ref_view];
   a.plane = S.plane;
   a.prev = S.prev;
+  a.cand = S.cand;
   a.cost = S.cost;
   a.support = S.support;
   int best = 0;
@@ -2327,6 +2308,7 @@ RECON_MVS_API int32_t recon_mvs_hip_batch_create(recon_mvs_hip_context* ctx, int
     S.has_geo = false;
     args[j].plane = S.plane;
     args[j].prev = S.prev;
+    args[j].cand = S.cand;
     args[j].cost = S.cost;
     args[j].support = S.support;
   }
