@@ -417,6 +417,47 @@ def test_binned_phases_bit_identical(storage, top_k):
         assert np.array_equal(seeded[1][k], seeded[0][k], equal_nan=True), k
 
 
+def test_binned_ragged_sizes_bit_identical():
+    """Binned and fused forms agree bit for bit on images that no tile
+    schedule divides, narrower or shorter than a tile (pixels outside the
+    image take no entries), per view under every schedule and in one batch
+    of all the sizes; and the batch equals the per-view runs."""
+    sizes = [(101, 37), (47, 211), (150, 9), (64, 8)]
+    say("binned vs fused on ragged sizes", sizes)
+    cams, imgs, jobs = [], [], []
+    for w, h in sizes:
+        sc = scenes.sphere_planes(width=w, height=h, focal=2500.0 * w / 1920.0 * 4.0, supersample=1,
+                                  n_views=5)
+        base = len(cams)
+        cams += list(sc.cams)
+        imgs += list(sc.images)
+        jobs.append((base, [base + k for k in range(1, 5)]))
+    pm = plane.PlaneMVS(cams, imgs, storage="f16")
+    p = plane.default_params(patch_size=11, window_stride=2, top_k=2, iterations=2,
+                             min_depth_m=0.25, max_depth_m=0.60, value_scale=1.0,
+                             view_phased_iterations=0, occupancy_cap=-1)
+    singles = {}
+    for r, srcs in jobs:
+        outs = []
+        for binned, sched in ((0, 0), (5, 0), (5, 1), (5, 2), (0, 2)):
+            pm.patchmatch(r, srcs, p.copy(binned_phases=binned, tile_schedule=sched))
+            outs.append(pm.depth_map(r))
+        for o in outs[1:]:
+            for k in ("depth", "normals", "cost", "support"):
+                assert np.array_equal(o[k], outs[0][k], equal_nan=True), (r, k)
+        singles[r] = outs[0]
+    assert any(np.isfinite(o["depth"]).mean() > 0.5 for o in singles.values())
+    for binned in (5, 0):
+        b = pm.batch(jobs, p.copy(binned_phases=binned, tile_schedule=1))
+        b.run()
+        pm.synchronize()
+        for r, _ in jobs:
+            g = pm.depth_map(r)
+            for k in ("depth", "normals", "cost", "support"):
+                assert g[k].tobytes() == singles[r][k].tobytes(), (binned, r, k)
+        b.close()
+
+
 def test_u8_images_half_storage_bit_identical():
     """8-bit images stored as FP16, BF16 or 8 bits (all hold the integers
     0..255 exactly) give the FP32-storage results bit for bit: planes, costs,
