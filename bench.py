@@ -41,6 +41,7 @@ METRIC = "PatchMatch Gpix*view*iter/s + median depth error vs analytic truth, GP
 PEAK_FP32_TFLOPS = 157.3    # MI355X FP32 vector peak
 PEAK_FP64_TFLOPS = 78.6     # MI355X FP64 vector peak (reference mode)
 PEAK_HBM_GBS = 8000.0
+PEAK_LDS_GBS = 78643.2      # LDS: 128 B per clock per CU x 256 CUs x 2.4 GHz
 FLOP_PER_SAMPLE = 20        # warp 6 + bilinear 9 + accumulate 5 per window sample
 HYPS_PER_ITER = 8           # 4 propagated + 3 perturbed + 1 random
 SAMPLES_11x11_S2 = 36
@@ -94,14 +95,15 @@ def log(msg):
     sys.stderr.flush()
 
 
-def traffic_entry(key):
-    """HBM bytes per launch for `key` from profiles/hbm_traffic.txt, with the
-    profile it was measured in."""
+def traffic_entry(key, kind="traffic"):
+    """A counter figure per launch for `key` from profiles/hbm_traffic.txt
+    (`kind`: traffic = HBM bytes, lds = LDS bytes loaded, valu_issue = VALU
+    issue fraction), with the profile it was measured in."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic.txt")
     if os.path.exists(path):
         for line in open(path):
             f = line.split()
-            if len(f) == 4 and f[0] == "traffic" and f[1] == key:
+            if len(f) == 4 and f[0] == kind and f[1] == key:
                 return float(f[2]), "profiles/" + f[3]
     return None, None
 
@@ -116,6 +118,13 @@ def roofline(flop_per_launch, mean_ms, kernel, traffic_key, peak=PEAK_FP32_TFLOP
     if traffic and mean_ms:
         r["hbm_gbs"] = traffic / (mean_ms * 1e-3) / 1e9
         r["hbm_frac"] = r["hbm_gbs"] / PEAK_HBM_GBS
+    lds, _ = traffic_entry(traffic_key, "lds")
+    if lds and mean_ms:
+        r["lds_gbs"] = lds / (mean_ms * 1e-3) / 1e9
+        r["lds_frac"] = r["lds_gbs"] / PEAK_LDS_GBS
+    issue, _ = traffic_entry(traffic_key, "valu_issue")
+    if issue:
+        r["valu_issue"] = issue
     r.update(extra)
     return r
 
