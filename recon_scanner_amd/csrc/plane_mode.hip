@@ -287,8 +287,10 @@ void k_plane_sweep(PlaneArgs a) {
 // In the unconverged phases every pixel's hypotheses lie at random depths, so
 // the 64 lanes of a wave sample 64 windows scattered along as many epipolar
 // segments of each source: each gather touches ~64 cache lines and nothing is
-// reused across lanes.  Here the owning threads first write their pixel's
-// hypotheses (entries) to LDS, a counting sort puts the entries in 64 bins
+// reused across lanes.  Here the owning threads first generate their
+// pixel's hypotheses (entries; those that need no scoring are dropped, as in
+// SweepTile) and keep them where the scoring threads can read them (see
+// BinnedLds), a counting sort in LDS puts the entries in 64 bins
 // of inverse depth over [inv_min, inv_max] (the entries of pixels whose
 // reference window is cut by the image border after all the others), and
 // the workgroup evaluates the sorted list: a wave's 64 consecutive entries
@@ -307,8 +309,8 @@ void k_plane_sweep(PlaneArgs a) {
 // running best anyway; and the choice runs in candidate order with the same
 // strict comparison.
 // ---------------------------------------------------------------------------
-constexpr int kBins = 64;                 // inverse-depth bins per pixel class
-constexpr int kBinnedEntries = 2048;  // sweep: 256 pixels x 8 candidates; init: 512 pixels x 1
+constexpr int kBins = 64;              // inverse-depth bins per pixel class
+constexpr int kBinnedEntries = 2048;   // sweep: 256 pixels x 8 candidates; init: 512 pixels x 1
 
 // LDS of the binned forms (about 45 KB: 3 workgroups per CU).  Entries hold
 // no hypothesis: the propagated ones (candidates 0-3) are kept here per
