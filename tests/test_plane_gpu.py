@@ -376,22 +376,23 @@ def test_view_phased_sweeps_bit_identical(storage, top_k):
         assert np.array_equal(seeded[1][k], seeded[0][k], equal_nan=True), k
 
 
-@pytest.mark.parametrize("storage,top_k", [("f32", 4), ("f16", 4), ("u8", 4), ("f16", 0)])
-def test_binned_phases_bit_identical(storage, top_k):
+@pytest.mark.parametrize("storage,top_k,reach", [("f32", 4, 4), ("f16", 4, 4), ("u8", 4, 4), ("f16", 0, 4),
+                                                 ("bf16", 1, 1), ("f16", 2, 8)])
+def test_binned_phases_bit_identical(storage, top_k, reach):
     """Binned init and half-sweeps (each tile's hypotheses sorted by inverse
     depth in LDS and evaluated bin by bin; early exit against the cost from
     the start of the sweep) give the fused kernels' planes, costs and
     supports bit for bit: on every phase, on the leading two, under every
     tile schedule, for top-k and all-views aggregation, with random and
     seeded init, and under the tuned plan."""
-    say("binned vs fused phases, sphere_planes 480x360 11x11/2", storage, "top_k", top_k)
+    say("binned vs fused phases, sphere_planes 480x360 11x11/2", storage, "top_k", top_k, "reach", reach)
     sc = scenes.sphere_planes(width=480, height=360, focal=625.0, supersample=1)
     srcs = list(range(1, 9))
     imgs = [sc.u8(i) for i in range(sc.num_views)] if storage == "u8" else sc.images
     pm = plane.PlaneMVS(sc.cams, imgs, storage=storage)
     p = plane.default_params(patch_size=11, window_stride=2, top_k=top_k, iterations=3,
                              min_depth_m=0.25, max_depth_m=0.60, value_scale=1.0,
-                             view_phased_iterations=0, occupancy_cap=-1)
+                             view_phased_iterations=0, occupancy_cap=-1, propagation_reach=reach)
     outs = []
     for binned, sched in ((0, 0), (7, 0), (2, 0), (7, 1), (7, 2), (0, 2)):
         pm.patchmatch(0, srcs, p.copy(binned_phases=binned, tile_schedule=sched))
