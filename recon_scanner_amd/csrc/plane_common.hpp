@@ -30,7 +30,7 @@ struct PlaneState {
   int w = 0, h = 0;
   float4* plane = nullptr;  // (depth, nx, ny, nz)
   float4* prev = nullptr;   // per pixel, the propagated candidates of its last half-sweep (4)
-  float4* cand = nullptr;   // per pixel, binned half-sweeps' perturbed and random candidates (4)
+  float4* cand = nullptr;   // binned half-sweeps' perturbed and random candidates: 4 per pixel of a colour
   float* cost = nullptr;
   uint8_t* support = nullptr;
   uint8_t* geo_support = nullptr;
@@ -82,10 +82,10 @@ struct PlaneArgs {
   // run (its propagated candidates; NaN: none), or null: such a plane cannot
   // beat the pixel's cost, which only falls during a run
   float4* prev;
-  // per pixel 4 slots for the perturbed and random candidates (4-7) of a
-  // binned half-sweep: written by the pixel's owning thread, read by the
-  // thread that scores them (same workgroup); null where no binned
-  // half-sweep runs
+  // 4 slots per pixel of the active colour for the perturbed and random
+  // candidates (4-7) of a binned half-sweep: written by the pixel's owning
+  // thread, read by the thread that scores them (same workgroup); null
+  // where no binned half-sweep runs
   float4* cand;
   float* cost;
   uint8_t* support;

@@ -331,6 +331,13 @@ struct BinnedLds {
   int n;                           // entries in the list
 };
 
+// Slot of candidate c (4-7) of pixel (x, y) in PlaneArgs::cand: 4 slots per
+// pixel of the active colour, by its compact index y * ceil(w / 2) + x / 2
+// (the colour's pixels of a row have distinct x / 2).
+__device__ __forceinline__ int CandSlot(const PlaneArgs& a, int x, int y, int c) {
+  return 4 * (y * ((a.w + 1) >> 1) + (x >> 1)) + (c - 4);
+}
+
 // Sort bin of a hypothesis at depth d of a pixel whose reference window is
 // whole (`full`) or not: the bin of 1/d in [inv_min, inv_max] (clamped; NaN
 // -> 0), after all kBins of the whole windows when not.
@@ -404,7 +411,7 @@ __device__ __forceinline__ void ScoreEntries(const PlaneArgs& a, BinnedLds<HALF,
       const float c0 = L.start_cost[s];
       reject_at = c0 < kCostInf ? fmaf(c0, 1.0f + 1e-5f, 1e-7f) : kCostInf;
       const int c = e & 7;
-      p = c < 4 ? L.prop[4 * s + c] : a.cand[4 * (y * a.w + x) + (c - 4)];
+      p = c < 4 ? L.prop[4 * s + c] : a.cand[CandSlot(a, x, y, c)];
     } else {
       p = a.plane[y * a.w + x];  // the init plane, written by the owning thread
     }
@@ -510,7 +517,7 @@ __device__ __forceinline__ void SweepTileBinned(const PlaneArgs& a, BinnedLds<HA
       }
       // PlaneCost scores these kCostInf without any view: never taken
       if (!(h.d >= a.min_depth && h.d <= a.max_depth) || !MakeHypGeom(a, x, y, h).front) continue;
-      if (c >= 4) a.cand[4 * idx + (c - 4)] = make_float4(h.d, h.nx, h.ny, h.nz);
+      if (c >= 4) a.cand[CandSlot(a, x, y, c)] = make_float4(h.d, h.nx, h.ny, h.nz);
       be[c] = (SortBin(a, h.d, win.full) << 16) | (t * 8 + c);
     }
   }
@@ -528,7 +535,7 @@ __device__ __forceinline__ void SweepTileBinned(const PlaneArgs& a, BinnedLds<HA
       best = c;
     }
   }
-  if (best >= 4) cur = a.cand[4 * idx + (best - 4)];
+  if (best >= 4) cur = a.cand[CandSlot(a, x, y, best)];
   else if (best >= 0) cur = L.prop[4 * t + best];
   a.plane[idx] = cur;
   a.cost[idx] = ccur;
@@ -1202,7 +1209,7 @@ int32_t EnsureState(recon_mvs_hip_context* ctx, int view) {
   const size_t P = static_cast<size_t>(V.w) * V.h;
   if (hipMalloc(&s->plane, P * sizeof(float4)) != hipSuccess ||
       hipMalloc(&s->prev, 4 * P * sizeof(float4)) != hipSuccess ||
-      hipMalloc(&s->cand, 4 * P * sizeof(float4)) != hipSuccess ||
+      hipMalloc(&s->cand, 4 * static_cast<size_t>(V.h) * ((V.w + 1) / 2) * sizeof(float4)) != hipSuccess ||
       hipMalloc(&s->cost, P * sizeof(float)) != hipSuccess ||
       hipMalloc(&s->support, P) != hipSuccess || hipMalloc(&s->geo_support, P) != hipSuccess) {
     FreeState(s);
