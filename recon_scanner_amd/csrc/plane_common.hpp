@@ -29,7 +29,7 @@ struct ResidentView {
 struct PlaneState {
   int w = 0, h = 0;
   float4* plane = nullptr;  // (depth, nx, ny, nz)
-  float4* prev = nullptr;   // per pixel, the propagated candidates of its last half-sweep (4)
+  float4* prev = nullptr;   // the propagated candidates of each pixel's last half-sweep (4), by colour
   float4* cand = nullptr;   // binned half-sweeps' perturbed and random candidates: 4 per pixel of a colour
   float* cost = nullptr;
   uint8_t* support = nullptr;

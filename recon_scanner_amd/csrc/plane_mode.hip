@@ -43,12 +43,21 @@ __device__ __forceinline__ Hyp InitPlane(const PlaneArgs& a, int x, int y) {
   return h;
 }
 
-// A new run: pixel idx has been scored on no plane yet.
-__device__ __forceinline__ void ClearPrev(const PlaneArgs& a, int idx) {
+// First record of pixel (x, y) in PlaneArgs::prev: 4 per pixel, the pixels
+// of each colour together (colour (x + y) & 1, then the compact index
+// y * ceil(w / 2) + x / 2), so a half-sweep reads and writes one dense half.
+__device__ __forceinline__ int PrevSlot(const PlaneArgs& a, int x, int y) {
+  const int cw = (a.w + 1) >> 1;
+  return 4 * (((x + y) & 1) * a.h * cw + y * cw + (x >> 1));
+}
+
+// A new run: pixel (x, y) has been scored on no plane yet.
+__device__ __forceinline__ void ClearPrev(const PlaneArgs& a, int x, int y) {
   if (!a.prev) return;
   const float n = __builtin_nanf("");
+  const int s = PrevSlot(a, x, y);
 #pragma unroll
-  for (int k = 0; k < 4; ++k) a.prev[4 * idx + k] = make_float4(n, n, n, n);
+  for (int k = 0; k < 4; ++k) a.prev[s + k] = make_float4(n, n, n, n);
 }
 
 // Random init + initial cost of every pixel (both colours) of one tile.
@@ -71,7 +80,7 @@ __device__ __forceinline__ void InitTile(const PlaneArgs& a, RefTile<HALF, FMT, 
     a.plane[idx] = make_float4(h.d, h.nx, h.ny, h.nz);
     a.cost[idx] = c;
     a.support[idx] = static_cast<uint8_t>(min(nv, 255));
-    ClearPrev(a, idx);
+    ClearPrev(a, x, y);
   }
 }
 
@@ -173,17 +182,18 @@ __device__ __forceinline__ bool SamePlane4(float4 h, float4 p) {
   return h.x == p.x && h.y == p.y && h.z == p.z && h.w == p.w;
 }
 
-// Propagated candidates (bits of `ok`) of pixel idx that need no scoring:
-// equal to an earlier candidate, to the current plane `base`, or to a plane
-// the pixel was scored on in an earlier half-sweep of the run (a.prev); then
-// records this half-sweep's propagated candidates in a.prev.
-__device__ __forceinline__ int SkipPropagated(const PlaneArgs& a, int idx, const float4 (&pc)[4],
+// Propagated candidates (bits of `ok`) of pixel (x, y) that need no
+// scoring: equal to an earlier candidate, to the current plane `base`, or to
+// a plane the pixel was scored on in an earlier half-sweep of the run
+// (a.prev); then records this half-sweep's propagated candidates in a.prev.
+__device__ __forceinline__ int SkipPropagated(const PlaneArgs& a, int x, int y, const float4 (&pc)[4],
                                               const bool (&ok)[4], float4 base) {
   int skip = 0;
   float4 old[4];
   const float n = __builtin_nanf("");
+  const int ps = a.prev ? PrevSlot(a, x, y) : 0;
 #pragma unroll
-  for (int k = 0; k < 4; ++k) old[k] = a.prev ? a.prev[4 * idx + k] : make_float4(n, n, n, n);
+  for (int k = 0; k < 4; ++k) old[k] = a.prev ? a.prev[ps + k] : make_float4(n, n, n, n);
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
     bool same = SamePlane4(pc[c], base);
@@ -195,7 +205,7 @@ __device__ __forceinline__ int SkipPropagated(const PlaneArgs& a, int idx, const
   }
   if (a.prev)
 #pragma unroll
-    for (int c = 0; c < 4; ++c) a.prev[4 * idx + c] = ok[c] ? pc[c] : make_float4(n, n, n, n);
+    for (int c = 0; c < 4; ++c) a.prev[ps + c] = ok[c] ? pc[c] : make_float4(n, n, n, n);
   return skip;
 }
 
@@ -244,7 +254,7 @@ __device__ __forceinline__ void SweepTile(const PlaneArgs& a, RefTile<HALF, FMT,
       pok[c] = SweepCandidate(a, c, ray, key, base, dscale, nscale, src_x, src_y, &h);
       pc[c] = make_float4(h.d, h.nx, h.ny, h.nz);
     }
-    dup_mask = SkipPropagated(a, idx, pc, pok, base);
+    dup_mask = SkipPropagated(a, x, y, pc, pok, base);
   }
   for (int c = 0; c < 8; ++c) {
     if ((dup_mask >> c) & 1) continue;
@@ -453,7 +463,7 @@ __device__ __forceinline__ void InitTileBinned(const PlaneArgs& a, BinnedLds<HAL
     const int idx = y * a.w + x;
     a.cost[idx] = L.res[s];
     a.support[idx] = L.nv[s];
-    ClearPrev(a, idx);
+    ClearPrev(a, x, y);
   }
 }
 
@@ -503,7 +513,7 @@ __device__ __forceinline__ void SweepTileBinned(const PlaneArgs& a, BinnedLds<HA
       L.prop[4 * t + c] = pc[c];
     }
     // candidates that need no scoring are skipped, as in SweepTile
-    const int dup_mask = SkipPropagated(a, idx, pc, pok, base);
+    const int dup_mask = SkipPropagated(a, x, y, pc, pok, base);
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
       if ((dup_mask >> c) & 1) continue;
@@ -1208,7 +1218,7 @@ int32_t EnsureState(recon_mvs_hip_context* ctx, int view) {
   s->h = V.h;
   const size_t P = static_cast<size_t>(V.w) * V.h;
   if (hipMalloc(&s->plane, P * sizeof(float4)) != hipSuccess ||
-      hipMalloc(&s->prev, 4 * P * sizeof(float4)) != hipSuccess ||
+      hipMalloc(&s->prev, 4 * 2 * static_cast<size_t>(V.h) * ((V.w + 1) / 2) * sizeof(float4)) != hipSuccess ||
       hipMalloc(&s->cand, 4 * static_cast<size_t>(V.h) * ((V.w + 1) / 2) * sizeof(float4)) != hipSuccess ||
       hipMalloc(&s->cost, P * sizeof(float)) != hipSuccess ||
       hipMalloc(&s->support, P) != hipSuccess || hipMalloc(&s->geo_support, P) != hipSuccess) {
