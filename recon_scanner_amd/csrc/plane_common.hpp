@@ -295,7 +295,10 @@ constexpr int kMaxTopKNet = 8;  // top_k below the view count is limited to 8
 // zero-cost view never raises the mean of the k smallest.  Once that bound
 // exceeds `reject_at` the hypothesis cannot beat it: the remaining views are
 // skipped and kCostInf is returned (callers pass the cost to beat, with a
-// rounding margin, or kCostInf to always score fully).
+// rounding margin, or kCostInf to always score fully).  `live` (optional,
+// LDS): the bits of a cost that other threads may lower while this one runs,
+// each value the full cost of a hypothesis competing with this one; every
+// check also rejects against the value it reads, with the same margin.
 // Per-hypothesis constants of PlaneCost: the pixel's centre and the plane as
 // m = K_r^-T n / rho (FP64).  front = false: the plane does not face the
 // camera (no view can be valid).
@@ -616,6 +619,12 @@ __device__ __forceinline__ float ViewNcc(const PlaneArgs& a, int v, const HypGeo
 // seen), so those views skip it.  The divisions are reciprocal products: their
 // rounding (~2e-7 relative; none for a power-of-two top_k) lies far inside the
 // callers' margin (the bound is compared with the current cost x (1 + 1e-5)).
+// The early-exit threshold for beating cost c: c with a rounding margin
+// (kCostInf or above: never exit).
+__device__ __forceinline__ float RejectAbove(float c) {
+  return c < kCostInf ? fmaf(c, 1.0f + 1e-5f, 1e-7f) : kCostInf;
+}
+
 __device__ __forceinline__ float CostLowerBound(const PlaneArgs& a, int v, const CostAcc& acc) {
   const int u = a.nsrc - 1 - v;  // views still to come
   if (u >= a.top_k) return 0.0f;
@@ -655,7 +664,7 @@ __device__ __forceinline__ float FinishCost(const PlaneArgs& a, const CostAcc& a
 template <int HALF, int STRIDE, int FMT>
 __device__ float PlaneCost(const PlaneArgs& a, int x, int y, const RefWindow<HALF, STRIDE>& win,
                            const Hyp& h, int* valid_views_out, float* view_ncc,
-                           float reject_at = kCostInf) {
+                           float reject_at = kCostInf, const int* live = nullptr) {
   *valid_views_out = 0;
   if (!(h.d >= a.min_depth && h.d <= a.max_depth)) {
     if (view_ncc)
@@ -670,6 +679,8 @@ __device__ float PlaneCost(const PlaneArgs& a, int x, int y, const RefWindow<HAL
     for (int v = 0; v < a.nsrc; ++v) {
       const float ncc_v = ViewNcc<HALF, STRIDE, FMT>(a, v, hg, win, use_topk, acc);
       if (view_ncc) view_ncc[v] = ncc_v;
+      if (live) reject_at = fminf(reject_at, RejectAbove(__int_as_float(__hip_atomic_load(
+                                                 live, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))));
       if (reject_at < kCostInf && CostLowerBound(a, v, acc) > reject_at) return kCostInf;
     }
   } else if (view_ncc) {

@@ -313,11 +313,13 @@ void k_plane_sweep(PlaneArgs a) {
 //
 // Results are bit-identical to InitTile / SweepTile: every entry is scored
 // by the same PlaneCost operations (a pixel's result does not depend on
-// which lanes share its wave, see ViewNcc); the early exit uses the pixel's
-// cost from the start of the sweep, a bound no tighter than SweepTile's
-// running best, so it drops only entries SweepTile drops or that lose to the
-// running best anyway; and the choice runs in candidate order with the same
-// strict comparison.
+// which lanes share its wave, see ViewNcc); the early exit rejects against
+// the pixel's best cost so far (its cost at the start of the sweep, lowered
+// by each of its entries scored in full, in whatever order the waves finish
+// them), so an entry dropped there costs more than the start cost or than
+// another of the pixel's entries and is not the pick, as with SweepTile's
+// running best; and the choice runs in candidate order with the same strict
+// comparison.
 // ---------------------------------------------------------------------------
 constexpr int kBins = 64;              // inverse-depth bins per pixel class
 constexpr int kBinnedEntries = 2048;   // sweep: 256 pixels x 8 candidates; init: 512 pixels x 1
@@ -336,7 +338,8 @@ struct BinnedLds {
   uint8_t nv[kBinnedEntries];      // its valid views
   uint16_t list[kBinnedEntries];   // entries in bin order
   float4 wstat[512];               // per pixel slot: the reference window's sum_c, sum_c2, n, mean
-  float start_cost[256];           // sweep: each pixel's cost at the start of the sweep
+  int best_bits[256];              // sweep: float bits of each pixel's best cost so far (its cost at
+                                   // the start of the sweep, lowered as its entries are scored)
   int bin[2 * kBins];              // counts, then first positions (whole windows first, then by depth)
   int n;                           // entries in the list
 };
@@ -402,7 +405,7 @@ __device__ __forceinline__ void SlotWindow(const BinnedLds<HALF, FMT, TW, TH>& L
 
 // Scores the sorted entries.  Init: entry e = pixel slot e (colour e >> 8,
 // thread e & 255); sweep: entry e = candidate e & 7 of thread e >> 3's pixel,
-// with early exit against its start cost.
+// with early exit against the pixel's best cost so far.
 template <int HALF, int STRIDE, int FMT, int TW, int TH>
 __device__ __forceinline__ void ScoreEntries(const PlaneArgs& a, BinnedLds<HALF, FMT, TW, TH>& L, int x0,
                                              int y0, int color, bool sweep) {
@@ -418,8 +421,7 @@ __device__ __forceinline__ void ScoreEntries(const PlaneArgs& a, BinnedLds<HALF,
     float reject_at = kCostInf;
     float4 p;
     if (sweep) {
-      const float c0 = L.start_cost[s];
-      reject_at = c0 < kCostInf ? fmaf(c0, 1.0f + 1e-5f, 1e-7f) : kCostInf;
+      reject_at = RejectAbove(__int_as_float(L.best_bits[s]));
       const int c = e & 7;
       p = c < 4 ? L.prop[4 * s + c] : a.cand[CandSlot(a, x, y, c)];
     } else {
@@ -427,7 +429,12 @@ __device__ __forceinline__ void ScoreEntries(const PlaneArgs& a, BinnedLds<HALF,
     }
     const Hyp h{p.x, p.y, p.z, p.w};
     int nv = 0;
-    L.res[e] = PlaneCost<HALF, STRIDE, FMT>(a, x, y, win, h, &nv, nullptr, reject_at);
+    const float cost = PlaneCost<HALF, STRIDE, FMT>(a, x, y, win, h, &nv, nullptr, reject_at,
+                                                    sweep ? &L.best_bits[s] : nullptr);
+    // a full cost bounds the pixel's pick from above: the pixel's entries
+    // scored after it exit against it (exact: one that exits costs more)
+    if (sweep && cost < kCostInf) atomicMin(&L.best_bits[s], __float_as_int(cost));
+    L.res[e] = cost;
     L.nv[e] = static_cast<uint8_t>(min(nv, 255));
   }
   __syncthreads();
@@ -495,7 +502,7 @@ __device__ __forceinline__ void SweepTileBinned(const PlaneArgs& a, BinnedLds<HA
     cur = a.plane[idx];
     ccur = a.cost[idx];
     scur = a.support[idx];
-    L.start_cost[t] = ccur;
+    L.best_bits[t] = __float_as_int(ccur);
     const uint32_t key = RngKey(a.seed, static_cast<uint32_t>(1 + 2 * iter + color),
                                 static_cast<uint32_t>(idx));
     const float dscale = ldexpf(a.depth_perturb, -iter);
