@@ -299,11 +299,11 @@ constexpr int kMaxTopKNet = 8;  // top_k below the view count is limited to 8
 // LDS): the bits of a cost that other threads may lower while this one runs,
 // each value the full cost of a hypothesis competing with this one; every
 // check also rejects against the value it reads, with the same margin.
-// Per-hypothesis constants of PlaneCost: the pixel's centre and the plane as
-// m = K_r^-T n / rho (FP64).  front = false: the plane does not face the
-// camera (no view can be valid).
+// Per-hypothesis constants of PlaneCost: the pixel's centre q = (qx, qy, 1)
+// and the plane as m = K_r^-T n / rho, with mq = m . q (FP64).  front =
+// false: the plane does not face the camera (no view can be valid).
 struct HypGeom {
-  double qx, qy, D, m0, m1, m2;
+  double qx, qy, D, m0, m1, m2, mq;
   bool front;
 };
 
@@ -320,6 +320,7 @@ __device__ __forceinline__ HypGeom MakeHypGeom(const PlaneArgs& a, int x, int y,
   g.m0 = nx * a.kinv[0] * irho;
   g.m1 = ny * a.kinv[1] * irho;
   g.m2 = (nz + nx * a.kinv[2] + ny * a.kinv[3]) * irho;
+  g.mq = nd * irho;  // m0 qx + m1 qy + m2
   return g;
 }
 
@@ -397,22 +398,19 @@ __device__ __forceinline__ float ViewNcc(const PlaneArgs& a, int v, const HypGeo
   constexpr int S = RefWindow<HALF, STRIDE>::S;
   constexpr int kEB = ElemBytes<FMT>();
   constexpr int kLogEB = kEB == 8 ? 3 : kEB == 4 ? 2 : 1;
-  const double qx = hg.qx, qy = hg.qy, D = hg.D, m0 = hg.m0, m1 = hg.m1, m2 = hg.m2;
+  const double qx = hg.qx, qy = hg.qy, D = hg.D, m0 = hg.m0, m1 = hg.m1, mq = hg.mq;
   float* topk = acc.topk;
   float& csum = acc.csum;
   int& nvalid = acc.nvalid;
   float ncc_v = __builtin_nanf("");
-  // H = A + b m^T, centre projection
-  double H[9];
-#pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    H[3 * i + 0] = fma(a.b[v][i], m0, a.A[v][3 * i + 0]);
-    H[3 * i + 1] = fma(a.b[v][i], m1, a.A[v][3 * i + 1]);
-    H[3 * i + 2] = fma(a.b[v][i], m2, a.A[v][3 * i + 2]);
-  }
-  const double hc0 = fma(H[0], qx, fma(H[1], qy, H[2]));
-  const double hc1 = fma(H[3], qx, fma(H[4], qy, H[5]));
-  const double hc2 = fma(H[6], qx, fma(H[7], qy, H[8]));
+  // centre projection hc = H q, H = A + b m^T, as A q + b mq: each product
+  // and sum takes at most one per-view (uniform) operand, which VALU
+  // instructions read from scalar registers without copies
+  const double* Av = a.A[v];
+  const double* bv = a.b[v];
+  const double hc0 = fma(Av[1], qy, fma(Av[0], qx, bv[0] * mq)) + Av[2];
+  const double hc1 = fma(Av[4], qy, fma(Av[3], qx, bv[1] * mq)) + Av[5];
+  const double hc2 = fma(Av[7], qy, fma(Av[6], qx, bv[2] * mq)) + Av[8];
   if (hc2 * D > 1e-6) {
     const double ih = RcpF64(hc2);
     const double uc = hc0 * ih, vc = hc1 * ih;       // continuous source coords
@@ -421,11 +419,15 @@ __device__ __forceinline__ float ViewNcc(const PlaneArgs& a, int v, const HypGeo
       const double fsu = floor(su), fsv = floor(sv);
       const int iu = static_cast<int>(fsu), iv = static_cast<int>(fsv);
       const float fu = static_cast<float>(su - fsu), fv = static_cast<float>(sv - fsv);
-      const float g00 = static_cast<float>((H[0] - uc * H[6]) * ih);
-      const float g01 = static_cast<float>((H[1] - uc * H[7]) * ih);
-      const float g10 = static_cast<float>((H[3] - vc * H[6]) * ih);
-      const float g11 = static_cast<float>((H[4] - vc * H[7]) * ih);
-      const float h20 = static_cast<float>(H[6] * ih), h21 = static_cast<float>(H[7] * ih);
+      // the first two columns of H
+      const double H00 = Av[0] + bv[0] * m0, H01 = Av[1] + bv[0] * m1;
+      const double H10 = Av[3] + bv[1] * m0, H11 = Av[4] + bv[1] * m1;
+      const double H20 = Av[6] + bv[2] * m0, H21 = Av[7] + bv[2] * m1;
+      const float g00 = static_cast<float>(fma(-uc, H20, H00) * ih);
+      const float g01 = static_cast<float>(fma(-uc, H21, H01) * ih);
+      const float g10 = static_cast<float>(fma(-vc, H20, H10) * ih);
+      const float g11 = static_cast<float>(fma(-vc, H21, H11) * ih);
+      const float h20 = static_cast<float>(H20 * ih), h21 = static_cast<float>(H21 * ih);
       const int w = a.src_w[v], hgt = a.src_h[v], pitch = a.src_pitch[v];
       const float lox = static_cast<float>(-iu), hix = static_cast<float>(w - 1 - iu);
       const float loy = static_cast<float>(-iv), hiy = static_cast<float>(hgt - 1 - iv);
