@@ -75,6 +75,9 @@ struct PlaneArgs {
   int nsrc, top_k;
   const void* src_img[kMaxViews];
   int src_pitch[kMaxViews], src_w[kMaxViews], src_h[kMaxViews];
+  // the last sampler coordinate of each source: w - 1, h - 1, and both
+  // less the 1e-4 px margin of ViewNcc's inside test
+  double src_lim[kMaxViews][4];
   double A[kMaxViews][9];       // K_s R_rel K_r^-1 (row-major)
   double b[kMaxViews][3];       // K_s t_rel
   float4* plane;
@@ -429,8 +432,6 @@ __device__ __forceinline__ float ViewNcc(const PlaneArgs& a, int v, const HypGeo
       const float g11 = static_cast<float>(fma(-vc, H21, H11) * ih);
       const float h20 = static_cast<float>(H20 * ih), h21 = static_cast<float>(H21 * ih);
       const int w = a.src_w[v], hgt = a.src_h[v], pitch = a.src_pitch[v];
-      const float lox = static_cast<float>(-iu), hix = static_cast<float>(w - 1 - iu);
-      const float loy = static_cast<float>(-iv), hiy = static_cast<float>(hgt - 1 - iv);
       const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
           const_cast<void*>(a.src_img[v]), 0, pitch * hgt * kEB, 0x00020000);
       // byte offset of the window-centre element (iv, iu) and of one element
@@ -448,10 +449,13 @@ __device__ __forceinline__ float ViewNcc(const PlaneArgs& a, int v, const HypGeo
       const float bv = e1 * rd + 1e-3f;
       // e2 < 0.5 keeps every sample's denominator positive
       const bool den_ok = e2 < 0.5f;
-      const bool outside = den_ok && (fu + bu < lox || fu - bu > hix || fv + bv < loy || fv - bv > hiy);
-      const float mlox = lox + 1e-4f, mhix = hix - 1e-4f, mloy = loy + 1e-4f, mhiy = hiy - 1e-4f;
-      const bool fast = den_ok & win.full & (fu - bu >= mlox) & (fu + bu <= mhix) &
-                        (fv - bv >= mloy) & (fv + bv <= mhiy);
+      // the window's sampler-coordinate box, [su - bu, su + bu] x [sv - bv,
+      // sv + bv], against the source (FP64: the limits are per-view uniforms)
+      const double xl = su - static_cast<double>(bu), xh = su + static_cast<double>(bu);
+      const double yl = sv - static_cast<double>(bv), yh = sv + static_cast<double>(bv);
+      const double* lim = a.src_lim[v];
+      const bool outside = den_ok & ((xh < 0.0) | (xl > lim[0]) | (yh < 0.0) | (yl > lim[1]));
+      const bool fast = den_ok & win.full & (xl >= 1e-4) & (xh <= lim[2]) & (yl >= 1e-4) & (yh <= lim[3]);
       // source sums (ss, sss, srs: values shifted by k_src) and the
       // reference values of excluded samples, centred (xr, xrr), with their
       // count xcnt
@@ -492,8 +496,10 @@ __device__ __forceinline__ float ViewNcc(const PlaneArgs& a, int v, const HypGeo
         } else if (__ballot(!(win.full & den_ok)) == 0) {
           // some lane is not fast, but every lane's reference window is
           // whole and every denominator positive: only the source range is
-          // checked per sample; same operations as the general loop below
-          // for such lanes
+          // checked per sample (relative to (iu, iv)); same operations as
+          // the general loop below for such lanes
+          const float lox = static_cast<float>(-iu), hix = static_cast<float>(w - 1 - iu);
+          const float loy = static_cast<float>(-iv), hiy = static_cast<float>(hgt - 1 - iv);
           bool have_k = false;
           int ixcnt = 0;
 #pragma unroll 1
@@ -540,6 +546,8 @@ __device__ __forceinline__ float ViewNcc(const PlaneArgs& a, int v, const HypGeo
         } else {
           // some lane is not fast: each sample is checked; fast lanes take
           // the operations of the loop above
+          const float hiy = static_cast<float>(hgt - 1 - iv), loy = static_cast<float>(-iv);
+          const float hix = static_cast<float>(w - 1 - iu), lox = static_cast<float>(-iu);
           bool have_k = false;
 #pragma unroll 1
           for (int j = 0; j < NS; ++j) {

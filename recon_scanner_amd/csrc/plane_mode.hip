@@ -1162,6 +1162,10 @@ int32_t MakeArgs(recon_mvs_hip_context* ctx, int ref, int nsrc, const int32_t* s
     a->src_pitch[k] = S.pitch;
     a->src_w[k] = S.w;
     a->src_h[k] = S.h;
+    a->src_lim[k][0] = S.w - 1.0;
+    a->src_lim[k][1] = S.h - 1.0;
+    a->src_lim[k][2] = S.w - 1.0 - 1e-4;
+    a->src_lim[k][3] = S.h - 1.0 - 1e-4;
     // R_rel = R_s R_r^T, t_rel = t_s - R_rel t_r (FP64 composition per view pair)
     double Rrel[9], trel[3];
     for (int i = 0; i < 3; ++i)

@@ -233,7 +233,7 @@ def test_pose_refinement_recovers_misaligned_source():
         loss = (cost * valid).sum() / valid.sum()
         loss.backward()
         opt.step()
-        losses.append(float(loss))
+        losses.append(float(loss.detach()))
     pose = pose_of(rot, trn)
     # residual misalignment: Exp(pose) applied to the misaligned camera vs truth
     fixed = oracle.perturb_camera(cams[3], pose.detach().numpy()[2])
