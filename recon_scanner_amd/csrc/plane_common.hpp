@@ -463,13 +463,14 @@ __device__ __forceinline__ float ViewNcc(const PlaneArgs& a, int v, const HypGeo
       float k_src = 0.0f;
       if (!outside) {
         if (__ballot(!fast) == 0) {
-          // every lane is fast: no per-sample masks
-#pragma unroll 1
-          for (int j = 0; j < NS; ++j) {
+          // every lane is fast: no per-sample masks.  Software-pipelined:
+          // row j + 1's texels are requested before row j's are
+          // interpolated and summed (the same operations, in the same order,
+          // as one row at a time)
+          auto row_taps = [&](int j, Texel<FMT>* q, float* fxs, float* fys) {
             const float dy = static_cast<float>(-HALF + j * STRIDE);
             const float b0 = g01 * dy, b1 = g11 * dy, b2 = fmaf(h21, dy, 1.0f);
             int off[NS];
-            float fxs[NS], fys[NS];
 #pragma unroll
             for (int i = 0; i < NS; ++i) {
               const float dx = static_cast<float>(-HALF + i * STRIDE);
@@ -480,9 +481,10 @@ __device__ __forceinline__ float ViewNcc(const PlaneArgs& a, int v, const HypGeo
               fys[i] = __builtin_amdgcn_fractf(ty);
               off[i] = MadI24(FloorInt(ty), pitch_eb, (FloorInt(tx) << kLogEB) + obase);
             }
-            Texel<FMT> q[NS];
 #pragma unroll
             for (int i = 0; i < NS; ++i) q[i] = LoadTexel<FMT>(rsrc, off[i]);
+          };
+          auto sum_row = [&](int j, const Texel<FMT>* q, const float* fxs, const float* fys) {
 #pragma unroll
             for (int i = 0; i < NS; ++i) {
               const float b = Bilinear<FMT>(q[i], fxs[i], fys[i]);
@@ -492,6 +494,25 @@ __device__ __forceinline__ float ViewNcc(const PlaneArgs& a, int v, const HypGeo
               sss = fmaf(s, s, sss);
               srs = fmaf(win.at(i, j), s, srs);
             }
+          };
+          // two row buffers used in turn (rows of even j in a, odd j in b)
+          Texel<FMT> qa[NS], qb[NS];
+          float fxa[NS], fya[NS], fxb[NS], fyb[NS];
+          row_taps(0, qa, fxa, fya);
+          int j = 0;
+#pragma unroll 1
+          for (; j + 2 < NS; j += 2) {
+            row_taps(j + 1, qb, fxb, fyb);
+            sum_row(j, qa, fxa, fya);
+            row_taps(j + 2, qa, fxa, fya);
+            sum_row(j + 1, qb, fxb, fyb);
+          }
+          if constexpr (NS % 2 == 0) {
+            row_taps(NS - 1, qb, fxb, fyb);
+            sum_row(NS - 2, qa, fxa, fya);
+            sum_row(NS - 1, qb, fxb, fyb);
+          } else {
+            sum_row(NS - 1, qa, fxa, fya);
           }
         } else if (__ballot(!(win.full & den_ok)) == 0) {
           // some lane is not fast, but every lane's reference window is
