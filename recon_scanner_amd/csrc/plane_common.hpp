@@ -521,38 +521,32 @@ __device__ __forceinline__ float ViewNcc(const PlaneArgs& a, int v, const HypGeo
           // the general loop below for such lanes
           const float lox = static_cast<float>(-iu), hix = static_cast<float>(w - 1 - iu);
           const float loy = static_cast<float>(-iv), hiy = static_cast<float>(hgt - 1 - iv);
-          bool have_k = false;
+          // the shift is the first valid sample; NaN until one is seen
+          k_src = __builtin_nanf("");
           int ixcnt = 0;
-#pragma unroll 1
-          for (int j = 0; j < NS; ++j) {
+          // software-pipelined as the loop above
+          auto row_taps = [&](int j, Texel<FMT>* q, float* txs, float* tys) {
             const float dy = static_cast<float>(-HALF + j * STRIDE);
             const float b0 = g01 * dy, b1 = g11 * dy, b2 = fmaf(h21, dy, 1.0f);
-            float txs[NS], tys[NS];
+            int off[NS];
 #pragma unroll
             for (int i = 0; i < NS; ++i) {
               const float dx = static_cast<float>(-HALF + i * STRIDE);
               const float r = __builtin_amdgcn_rcpf(fmaf(h20, dx, b2));
               txs[i] = fmaf(fmaf(g00, dx, b0), r, fu);
               tys[i] = fmaf(fmaf(g10, dx, b1), r, fv);
-            }
-            int off[NS];
-            float fxs[NS], fys[NS];
-#pragma unroll
-            for (int i = 0; i < NS; ++i) {
-              fxs[i] = __builtin_amdgcn_fractf(txs[i]);
-              fys[i] = __builtin_amdgcn_fractf(tys[i]);
               off[i] = MadI24(FloorInt(tys[i]), pitch_eb, (FloorInt(txs[i]) << kLogEB) + obase);
             }
-            Texel<FMT> q[NS];
 #pragma unroll
             for (int i = 0; i < NS; ++i) q[i] = LoadTexel<FMT>(rsrc, off[i]);
+          };
+          auto sum_row = [&](int j, const Texel<FMT>* q, const float* txs, const float* tys) {
 #pragma unroll
             for (int i = 0; i < NS; ++i) {
-              const float b = Bilinear<FMT>(q[i], fxs[i], fys[i]);
+              const float b = Bilinear<FMT>(q[i], __builtin_amdgcn_fractf(txs[i]), __builtin_amdgcn_fractf(tys[i]));
               const float rv = win.at(i, j);
               const bool ok = fast | ((txs[i] >= lox) & (txs[i] <= hix) & (tys[i] >= loy) & (tys[i] <= hiy));
-              if (ok && !have_k) k_src = b;
-              have_k = have_k || ok;
+              k_src = (ok & (k_src != k_src)) ? b : k_src;
               const float s = ok ? b - k_src : 0.0f;
               const float x = ok ? 0.0f : rv - win.mean;
               ss += s;
@@ -562,6 +556,24 @@ __device__ __forceinline__ float ViewNcc(const PlaneArgs& a, int v, const HypGeo
               xrr = fmaf(x, x, xrr);
               ixcnt += ok ? 0 : 1;
             }
+          };
+          Texel<FMT> qa[NS], qb[NS];
+          float txa[NS], tya[NS], txb[NS], tyb[NS];
+          row_taps(0, qa, txa, tya);
+          int j = 0;
+#pragma unroll 1
+          for (; j + 2 < NS; j += 2) {
+            row_taps(j + 1, qb, txb, tyb);
+            sum_row(j, qa, txa, tya);
+            row_taps(j + 2, qa, txa, tya);
+            sum_row(j + 1, qb, txb, tyb);
+          }
+          if constexpr (NS % 2 == 0) {
+            row_taps(NS - 1, qb, txb, tyb);
+            sum_row(NS - 2, qa, txa, tya);
+            sum_row(NS - 1, qb, txb, tyb);
+          } else {
+            sum_row(NS - 1, qa, txa, tya);
           }
           xcnt = static_cast<float>(ixcnt);
         } else {
