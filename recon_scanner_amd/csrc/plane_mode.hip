@@ -361,15 +361,45 @@ __device__ __forceinline__ int SortBin(const PlaneArgs& a, float d, bool full) {
 
 // Counting sort of the workgroup's entries by bin.  Each thread passes its
 // NE entries as (bin << 16 | entry), -1 for none; on return L.list holds
-// them bin by bin (order inside a bin unspecified) and L.n their count.
+// them bin by bin and L.n their count.  Inside a bin a thread's entries are
+// consecutive (one reservation per distinct bin of the thread), so a wave's
+// lanes cover a few neighbouring pixels with all their candidates at that
+// depth rather than one candidate of 64 pixels (order across threads is the
+// order of the atomics, otherwise unspecified).
 template <int NE, int HALF, int FMT, int TW, int TH>
 __device__ __forceinline__ void BinEntries(BinnedLds<HALF, FMT, TW, TH>& L, const int (&be)[NE]) {
   const int t = threadIdx.x;
   if (t < 2 * kBins) L.bin[t] = 0;
   __syncthreads();
+  // one atomic per distinct bin of the thread, for all its entries there:
+  // a pixel's entries in a bin stay consecutive
   int rank[NE];
 #pragma unroll
-  for (int k = 0; k < NE; ++k) rank[k] = be[k] >= 0 ? atomicAdd(&L.bin[be[k] >> 16], 1) : 0;
+  for (int k = 0; k < NE; ++k) {
+    const int bk = be[k] >> 16;
+    int before = 0, cnt = 0;
+#pragma unroll
+    for (int m = 0; m < NE; ++m) {
+      const bool same = be[m] >= 0 && (be[m] >> 16) == bk;
+      before += (m < k && same) ? 1 : 0;
+      cnt += same ? 1 : 0;
+    }
+    rank[k] = (be[k] >= 0 && before == 0) ? atomicAdd(&L.bin[bk], cnt) : 0;
+  }
+#pragma unroll
+  for (int k = 0; k < NE; ++k) {
+    const int bk = be[k] >> 16;
+    int before = 0, base = 0;
+#pragma unroll
+    for (int m = 0; m < NE; ++m) {
+      const bool same = be[m] >= 0 && (be[m] >> 16) == bk;
+      if (m < k && same) {
+        base = before == 0 ? rank[m] : base;
+        ++before;
+      }
+    }
+    if (be[k] >= 0 && before > 0) rank[k] = base + before;
+  }
   __syncthreads();
   if (t == 0) {
     int s = 0;
