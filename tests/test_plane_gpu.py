@@ -380,8 +380,8 @@ def test_view_phased_sweeps_bit_identical(storage, top_k):
                                                  ("bf16", 1, 1), ("f16", 2, 8)])
 def test_binned_phases_bit_identical(storage, top_k, reach):
     """Binned init and half-sweeps (each tile's hypotheses sorted by inverse
-    depth in LDS and evaluated bin by bin; early exit against the cost from
-    the start of the sweep) give the fused kernels' planes, costs and
+    depth in LDS and evaluated bin by bin; early exit against each pixel's
+    best cost so far) give the fused kernels' planes, costs and
     supports bit for bit: on every phase, on the leading two, under every
     tile schedule, for top-k and all-views aggregation, with random and
     seeded init, and under the tuned plan."""
@@ -517,3 +517,68 @@ def _tuned_plan_case(sc, u8, srcs, storage):
         for k in ("depth", "normals", "cost", "support"):
             assert np.array_equal(o[k], outs[0][k], equal_nan=True), k
     assert np.isfinite(outs[0]["depth"]).mean() > 0.9
+
+
+def orbit_16_sources(res=(256, 192)):
+    """Reference 0 of a 64-camera orbit with its 8 neighbours on each side as
+    sources: 16, the most a reference takes (baselines up to 45 degrees)."""
+    sc, _ = scenes.orbit_batch(num_refs=1, num_cams=64, seed=7, min_res=res, max_res=res, src_range=(4, 4))
+    return sc, [d % 64 for d in range(-8, 9) if d != 0]
+
+
+@pytest.mark.parametrize("top_k", [8, 5])
+def test_eval_16_sources_top_k_above_4_matches_oracle(top_k):
+    """Per-view NCC and aggregated cost with 16 sources and top-k above 4
+    (the 8-wide insertion network), against the FP64 oracle at the bars
+    above."""
+    say("eval orbit 256x192 11x11/2, 16 sources, top_k", top_k)
+    sc, srcs = orbit_16_sources()
+    p = plane.default_params(patch_size=11, window_stride=2, top_k=top_k, min_depth_m=sc.min_depth,
+                             max_depth_m=sc.max_depth, value_scale=1.0)
+    compare_eval(sc, srcs, p, "f32", n=2000)
+
+
+def test_binned_16_sources_top8_bit_identical():
+    """Binned and fused forms agree bit for bit with 16 sources and top-8."""
+    say("binned vs fused, orbit 256x192 11x11/2, 16 sources, top-8")
+    sc, srcs = orbit_16_sources()
+    pm = plane.PlaneMVS(sc.cams, sc.images, storage="f16")
+    p = plane.default_params(patch_size=11, window_stride=2, top_k=8, iterations=2,
+                             min_depth_m=sc.min_depth, max_depth_m=sc.max_depth, value_scale=1.0,
+                             view_phased_iterations=0, occupancy_cap=-1)
+    outs = []
+    for binned, sched in ((0, 0), (5, 0), (5, 1), (5, 2)):
+        pm.patchmatch(0, srcs, p.copy(binned_phases=binned, tile_schedule=sched))
+        outs.append(pm.depth_map(0))
+    pm.patchmatch(0, srcs, p.copy(binned_phases=-1, tile_schedule=-1, occupancy_cap=0))
+    outs.append(pm.depth_map(0))
+    for o in outs[1:]:
+        for k in ("depth", "normals", "cost", "support"):
+            assert np.array_equal(o[k], outs[0][k], equal_nan=True), k
+    assert np.isfinite(outs[0]["depth"]).mean() > 0.5
+
+
+def test_batch_16_sources_top8_matches_per_view():
+    """A batch whose jobs take 16 sources each (top-8) equals per-view runs
+    bit for bit, over two graph replays."""
+    say("batch, orbit 64 cameras 192x144, 3 refs x 16 sources, top-8")
+    sc, _ = scenes.orbit_batch(num_refs=1, num_cams=64, seed=9, min_res=(192, 144), max_res=(192, 144),
+                               src_range=(4, 4))
+    jobs = [(r, [(r + d) % 64 for d in range(-8, 9) if d != 0]) for r in (0, 21, 42)]
+    p = plane.default_params(patch_size=11, window_stride=2, top_k=8, iterations=2,
+                             min_depth_m=sc.min_depth, max_depth_m=sc.max_depth, value_scale=1.0,
+                             tile_schedule=1)
+    pm = plane.PlaneMVS(sc.cams, sc.images, storage="f16")
+    singles = {}
+    for r, srcs in jobs:
+        pm.patchmatch(r, srcs, p.copy(binned_phases=0))
+        singles[r] = pm.depth_map(r)
+    b = pm.batch(jobs, p.copy(binned_phases=-1))
+    for rep in range(2):
+        b.run()
+        pm.synchronize()
+        for r, _ in jobs:
+            g = pm.depth_map(r)
+            for k in ("depth", "normals", "cost", "support"):
+                assert g[k].tobytes() == singles[r][k].tobytes(), (rep, r, k)
+    b.close()
