@@ -959,18 +959,29 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(2)
         double f1 = EvaluateTile<HALF>(p, rt, ray, c1).cost;
         double f2 = EvaluateTile<HALF>(p, rt, ray, c2).cost;
         for (int k = 0; k < iterations; ++k) {
-          if (f1 < f2) {
+          // each lane moves its bracket, then one evaluation serves both
+          // sides (lanes that went different ways would otherwise run the
+          // evaluation once per side); the same points in the same order
+          const bool left = f1 < f2;
+          double cn;
+          if (left) {
             hi = c2;
             c2 = c1;
             f2 = f1;
-            c1 = hi - inv_phi * (hi - lo);
-            f1 = EvaluateTile<HALF>(p, rt, ray, c1).cost;
+            cn = hi - inv_phi * (hi - lo);
           } else {
             lo = c1;
             c1 = c2;
             f1 = f2;
-            c2 = lo + inv_phi * (hi - lo);
-            f2 = EvaluateTile<HALF>(p, rt, ray, c2).cost;
+            cn = lo + inv_phi * (hi - lo);
+          }
+          const double fn = EvaluateTile<HALF>(p, rt, ray, cn).cost;
+          if (left) {
+            c1 = cn;
+            f1 = fn;
+          } else {
+            c2 = cn;
+            f2 = fn;
           }
         }
         const double cm = 0.5 * (lo + hi);
