@@ -918,14 +918,24 @@ __global__ __launch_bounds__(256) void k_plane_refine(PlaneArgs a, float inv_per
       int n1 = 0, n2 = 0;
       float f1 = eval(c1, &n1), f2 = eval(c2, &n2);
       for (int k = 0; k < iterations; ++k) {
-        if (f1 < f2) {
+        // each lane moves its bracket, then one evaluation serves both sides
+        // (lanes that went different ways would otherwise run it once per
+        // side); the same points in the same order
+        const bool left = f1 < f2;
+        float cn;
+        if (left) {
           hi = c2; c2 = c1; f2 = f1; n2 = n1;
-          c1 = hi - ip * (hi - lo);
-          f1 = eval(c1, &n1);
+          cn = hi - ip * (hi - lo);
         } else {
           lo = c1; c1 = c2; f1 = f2; n1 = n2;
-          c2 = lo + ip * (hi - lo);
-          f2 = eval(c2, &n2);
+          cn = lo + ip * (hi - lo);
+        }
+        int nn = 0;
+        const float fn = eval(cn, &nn);
+        if (left) {
+          c1 = cn; f1 = fn; n1 = nn;
+        } else {
+          c2 = cn; f2 = fn; n2 = nn;
         }
       }
       if (f1 < best_c) { best_c = f1; best_i = c1; best_s = n1; }
